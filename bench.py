@@ -1,0 +1,164 @@
+"""Benchmark of the MI355X particle step: body-steps/s at 400k bodies
+(P3M + collisions), one GPU.
+
+Workload: 400,000 FP32 bodies in walls on a 2880 x 1440 viewport, an
+exponential-disc galaxy (scale radius 200 px, circular velocities from the
+enclosed mass, 2% Gaussian dispersion), contact diameter 1 px, P3M grid
+1024 x 512 (2048 x 1024 mesh under walls), 16 collision substeps, friction
+0.2, seed 7. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line with the throughput, the roofline of the dominant
+kernel (per-kernel HIP events on the library's stream) and a CPU baseline
+(the oracle, timed on a bounded sample on the host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WIDTH, HEIGHT = 2880, 1440
+N_BODIES = 400_000
+GRID = (1024, 512)
+SUBSTEPS = 16
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per launch of each kernel, per unit it processes: the
+# logical FP32 SoA arrays a launch must read and write once (neighbour
+# gathers and list traffic excluded).
+#   count : read x, y (8 B), write the contact count (4 B)            12 B/body
+#   solve : read x, y, vx, vy, mass, friction, count (28 B),
+#           write x, y, vx, vy (16 B)                               44 B/body
+#   kick  : read x, y, vx, vy (16 B), write vx, vy, ax, ay (16 B)   32 B/body
+#   fft_columns : read + write one complex mesh value (16 B) and
+#                 read the spectral kernel (8 B)                    24 B/node
+#   fft_rows_forward : write one complex mesh value (8 B)            8 B/node
+#   fft_rows_inverse : read + write one complex mesh value (16 B)   16 B/node
+BYTES_PER_UNIT = {"count": 12, "solve": 44, "kick": 32, "drift_insert": 32, "fft_columns": 24,
+                  "fft_rows_forward": 8, "fft_rows_inverse": 16}
+
+
+def build_pipeline(n=N_BODIES):
+    from gravity_amd import CollisionConfig, GravityConfig, GravityFieldConfig, GravityPipeline, ParticleConfig
+    from gravity_amd import scenarios
+    bodies, mats = scenarios.exponential_disc(n, WIDTH, HEIGHT, scale_px=200.0, dispersion=0.02,
+                                              friction=0.2, seed=7)
+    cfg = GravityConfig(particles=ParticleConfig(capacity=n, initialBodies=n, seed=7),
+                        gravity=GravityFieldConfig(gridWidth=GRID[0], gridHeight=GRID[1]),
+                        collisions=CollisionConfig(maximumSubsteps=SUBSTEPS, contactDiameterPixels=1.0,
+                                                   friction=0.2, sleep=False))
+    pipe = GravityPipeline(cfg, WIDTH, HEIGHT, scenarioState=bodies, bounded=True)
+    pipe.upload(bodies, mats)
+    pipe.initializeAcceleration(n)
+    return pipe, bodies, mats
+
+
+def units_per_launch(name, pipe, n):
+    q = pipe.query()
+    if name.startswith("fft"):
+        mw, mh = q["meshWidth"], q["meshHeight"]
+        if name == "fft_columns":
+            return mw * mh
+        return mw * (q["gridHeight"] + 2)  # rows the walls leave live
+    return n
+
+
+def cpu_baseline(bodies, mats, steps=2):
+    """The oracle (single-threaded C restatement) on `steps` steps of the same workload."""
+    from oracle import oracle as orc
+    p = orc.params(WIDTH, HEIGHT, bounded=True, grid=GRID, diameter=1.0, substeps=SUBSTEPS)
+    sim = orc.Sim(p, bodies, mats)
+    sim.initialize_acceleration()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    dt = time.perf_counter() - t0
+    sim.close()
+    return len(bodies) * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    pipe, bodies, mats = build_pipeline()
+    n = N_BODIES
+    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        pipe.step(n)
+    pipe.synchronize()
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pipe.step(n)
+    pipe.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n * args.steps / elapsed
+
+    print(f"[bench] profiling {args.profile_steps} steps with per-kernel events", file=sys.stderr, flush=True)
+    stats = pipe.profile(args.profile_steps)
+    per_kernel = {name: (launches, ms) for name, launches, ms in stats}
+    dominant = max(per_kernel, key=lambda k: per_kernel[k][1])
+    launches, total_ms = per_kernel[dominant]
+    avg_s = total_ms / launches / 1e3
+    units = units_per_launch(dominant, pipe, n)
+    alg_bytes = BYTES_PER_UNIT.get(dominant, 0) * units
+    achieved = alg_bytes / avg_s / 1e9
+    b = pipe.bodies(n)
+    finite = bool(np.isfinite(b).all())
+    pipe.close()
+
+    cpu = None
+    if not args.no_cpu:
+        print(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps", file=sys.stderr, flush=True)
+        rate, secs = cpu_baseline(bodies, mats, args.cpu_steps)
+        cpu = {"value": rate, "unit": "body-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same 400k workload ({secs:.1f} s, single thread)"}
+
+    line = {
+        "metric": "body-steps/s at 400k bodies (P3M+collisions); batched & adjoint vs plain, 1 GPU",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "galaxy-400k: exponential disc, walls, 2880x1440, grid 1024x512, "
+                               "16 substeps, friction 0.2, diameter 1 px, seed 7",
+                   "bodies": n, "grid": list(GRID), "substeps": SUBSTEPS, "bounded": True,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": alg_bytes},
+        "kernels_ms_per_step": {k: v[1] / args.profile_steps for k, v in per_kernel.items()},
+        "finite": finite,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

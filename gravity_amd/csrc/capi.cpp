@@ -1,0 +1,238 @@
+// extern "C" boundary (include/gravity_hip.h): argument checks with the
+// reference's messages, exceptions mapped to return codes, a thread-local
+// error string.
+#include <cstring>
+#include <string>
+
+#include "engine.hpp"
+
+struct gs_sim {
+    gs::Engine* engine;
+};
+
+namespace {
+thread_local std::string g_error;
+
+int fail(int code, const char* msg) {
+    g_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_error.clear();
+        return GS_OK;
+    } catch (const gs::ArgumentError& e) {
+        return fail(GS_EINVAL, e.what());
+    } catch (const gs::StateError& e) {
+        return fail(GS_ESTATE, e.what());
+    } catch (const std::exception& e) {
+        return fail(GS_ESTATE, e.what());
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void gs_config_defaults(gs_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->viewport_width = 3440;
+    c->viewport_height = 1440;
+    c->bounded = 0;
+    c->mutual_gravity = 1;
+    c->liquid = 0;
+    c->capacity = 400000;
+    c->spatial_sort_interval = 2;
+    c->grid_width = 512;
+    c->grid_height = 256;
+    c->gravitational_constant = 0.035f;
+    c->softening_cells = 0.85f;
+    c->local_search_sigma_multiplier = 3.0f;
+    c->minimum_neighbor_radius_cells = 1;
+    c->maximum_neighbor_radius_cells = 6;
+    c->exact_neighbor_body_limit = 2;
+    c->maximum_home_body_interactions = 8;
+    c->minimum_distance_cell_fraction = 0.05f;
+    c->minimum_assignment_window_power = 0.08f;
+    c->time_step = 1.0f / 120.0f;
+    c->maximum_substeps = 16;
+    c->contact_diameter_pixels = 1.12838f;
+    c->neighbor_cell_radius = 1;
+    c->restitution_coefficient = 0.0f;
+    c->friction = 0.0f;
+    c->coincident_distance_squared_epsilon = 1e-12f;
+    c->neighbor_skin_pixels = 0.5f;
+    c->sleep = 1;
+    c->sleep_speed_pixels = 6.0f;
+    c->kernel_radius_diameters = 2.0f;
+    c->liquid_iterations = 12;
+    c->liquid_viscosity = 0.1f;
+    c->field_acceleration_x = 0.0f;
+    c->field_acceleration_y = 0.0f;
+    c->collisions_enabled = 1;
+}
+
+int gs_create(const gs_config* cfg, const float* state, int n, gs_sim** out) {
+    if (!cfg || !out) return fail(GS_EINVAL, "null argument");
+    if (n < 0) return fail(GS_EINVAL, "negative scenario size");
+    if (n > 0 && !state) return fail(GS_EINVAL, "scenario data is null");
+    *out = nullptr;
+    return guarded([&] {
+        std::vector<gs_config> cfgs{*cfg};
+        std::vector<const float*> states{state};
+        std::vector<int> counts{n};
+        auto* e = new gs::Engine(cfgs, states, counts);
+        *out = new gs_sim{e};
+    });
+}
+
+void gs_destroy(gs_sim* sim) {
+    if (!sim) return;
+    delete sim->engine;
+    delete sim;
+}
+
+#define SIM_OR_FAIL                                                    \
+    if (!sim || !sim->engine) return fail(GS_EINVAL, "null handle");   \
+    gs::Engine& e = *sim->engine;
+
+int gs_set_bool(gs_sim* sim, int key, int value) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        switch (key) {
+            case GS_BOUNDED: e.set_bounded(0, value != 0); break;
+            case GS_MUTUAL_GRAVITY: e.set_mutual(0, value != 0); break;
+            case GS_LIQUID: e.set_liquid(0, value != 0); break;
+            default: throw gs::ArgumentError("unknown property");
+        }
+    });
+}
+
+int gs_set_field(gs_sim* sim, float ax, float ay) {
+    SIM_OR_FAIL
+    return guarded([&] { e.set_field(0, ax, ay); });
+}
+
+int gs_set_substeps(gs_sim* sim, int substeps) {
+    SIM_OR_FAIL
+    return guarded([&] { e.set_substeps(substeps); });
+}
+
+int gs_initialize_acceleration(gs_sim* sim, int active) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        e.set_active(0, active);
+        e.initialize_acceleration();
+    });
+}
+
+int gs_step(gs_sim* sim, int active) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        e.set_active(0, active);
+        e.step();
+    });
+}
+
+int gs_upload(gs_sim* sim, int first, int count, const float* xyvv, const float* mats) {
+    SIM_OR_FAIL
+    if (count > 0 && !xyvv) return fail(GS_EINVAL, "null bodies");
+    return guarded([&] { e.upload(0, first, count, xyvv, mats); });
+}
+
+int gs_upload_accel(gs_sim* sim, int first, int count, const float* accel) {
+    SIM_OR_FAIL
+    if (count > 0 && !accel) return fail(GS_EINVAL, "null accelerations");
+    return guarded([&] { e.upload_accel(0, first, count, accel); });
+}
+
+int gs_download(gs_sim* sim, int what, int first, int count, void* dst) {
+    SIM_OR_FAIL
+    if (count > 0 && !dst) return fail(GS_EINVAL, "null destination");
+    return guarded([&] { e.download(0, what, first, count, dst); });
+}
+
+int gs_stop_bodies(gs_sim* sim, int active) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        e.set_active(0, active);
+        e.stop_bodies(0);
+    });
+}
+
+int gs_wake(gs_sim* sim) {
+    SIM_OR_FAIL
+    return GS_OK;
+}
+
+int gs_debug_resolve_collisions(gs_sim* sim, int count, float kick, float drift, int pressure,
+                                int substep) {
+    SIM_OR_FAIL
+    (void)substep;
+    return guarded([&] {
+        e.set_active(0, count);
+        e.debug_collide(kick, drift, pressure != 0);
+    });
+}
+
+int gs_debug_sort(gs_sim* sim, int count) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        e.set_active(0, count);
+        e.debug_sort();
+    });
+}
+
+int gs_debug_resolve_liquid(gs_sim* sim, int count) {
+    SIM_OR_FAIL
+    (void)count;
+    return fail(GS_ESTATE, "liquid passes are not built in this library");
+}
+
+int gs_synchronize(gs_sim* sim) {
+    SIM_OR_FAIL
+    return guarded([&] { e.synchronize(); });
+}
+
+int gs_query(gs_sim* sim, int* out) {
+    SIM_OR_FAIL
+    if (!out) return fail(GS_EINVAL, "null output");
+    const gs::HostSim& h = e.sim(0);
+    out[0] = h.cap;
+    out[1] = h.desc.mw;
+    out[2] = h.desc.mh;
+    out[3] = h.desc.pw;
+    out[4] = h.desc.ph;
+    out[5] = gs::CONTACT_SLOTS;
+    return GS_OK;
+}
+
+int gs_profile_steps(gs_sim* sim, int steps, gs_kernel_stat* out, int capacity, int* written) {
+    SIM_OR_FAIL
+    if (steps < 0 || (capacity > 0 && !out)) return fail(GS_EINVAL, "bad profile arguments");
+    return guarded([&] {
+        auto stats = e.profile(steps);
+        int k = 0;
+        for (auto& s : stats) {
+            if (k >= capacity) break;
+            std::memset(&out[k], 0, sizeof(gs_kernel_stat));
+            std::strncpy(out[k].name, std::get<0>(s).c_str(), sizeof(out[k].name) - 1);
+            out[k].launches = std::get<1>(s);
+            out[k].total_ms = std::get<2>(s);
+            k++;
+        }
+        if (written) *written = k;
+    });
+}
+
+int gs_set_graphs(gs_sim* sim, int enabled) {
+    SIM_OR_FAIL
+    e.set_graphs(enabled != 0);
+    return GS_OK;
+}
+
+const char* gs_last_error(void) { return g_error.c_str(); }
+
+}  // extern "C"

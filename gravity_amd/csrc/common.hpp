@@ -1,0 +1,94 @@
+// Shared definitions of the MI355X particle step: the device-resident
+// simulation descriptor, the batch block map and the exact float helpers.
+//
+// Every kernel serves a batch of independent simulations (a single
+// simulation is a batch of one). Bodies of all simulations live in one set of
+// SoA arrays; simulation s owns global indices [base, base + capacity), with
+// base a multiple of 256, so a 256-wide workgroup never spans two of them.
+// A launch over bodies walks a block table of int2 {sim, first body}.
+//
+// All HIP code is compiled with -ffp-contract=off: each float expression
+// rounds exactly as written, like the shaders' `precise` contact predicate,
+// and the CPU oracle reproduces it bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gs {
+
+constexpr int WG = 256;               // workgroup width of every body kernel
+constexpr int CONTACT_SLOTS = 32;     // contacts listed per body; more walk the grid
+constexpr float MAXIMUM_UNIT = 0.99999994f;  // largest float below 1
+
+// Per-simulation constants, mirrored from GravityConfig and the viewport
+// (K/GravityPipeline.kt:73-241). Derived floats are computed on the host
+// with the same operations the shaders apply to their uniforms.
+struct SimDesc {
+    // bodies
+    int n;              // active bodies
+    int base;           // first global body index
+    int cap;            // capacity
+    int bounded;        // walls instead of the torus
+    int mutual;         // mutual gravity
+    int liquid;
+    int collide;        // contact passes (0: drift-only gravity integrator)
+    int pad0;
+    float vw, vh;       // viewport, px
+    float aspect;       // vw / vh
+    float dt;           // step
+    float subDt;        // collision substep (effective substep dt)
+    // collisions
+    float diameter;
+    float diameter2;    // diameter * diameter
+    float restitution;
+    float coincidentEps;
+    int cgw, cgh;       // broad-phase grid (cells >= diameter + guard)
+    int cellBase;       // first cell of this simulation in the broad-phase heads
+    // spatial sort (S/spatial_sort_key.glsl)
+    int sortWb, sortHb;
+    int tileBase;       // first tile of this simulation in the tile counts
+    int rank0;          // bodies of earlier simulations (scan offset)
+    // P3M
+    int pw, ph;         // physical grid
+    int mw, mh;         // mesh (doubled under walls)
+    int physBase;       // first physical cell of this simulation
+    int meshBase;       // first mesh node of this simulation
+    int rx, ry;         // near-field radii, cells
+    int exactLimit;     // exactNeighborBodyLimit
+    int homeLimit;      // maximumHomeBodyInteractions
+    float G;
+    float totalMass;
+    float sigma;        // softeningCells / ph
+    float twoSigma2;    // 2 * sigma * sigma
+    float minDist2;     // (minimumDistanceCellFraction * min(cellW, cellH))^2
+    float invMassScale; // 1 / massScale
+    float invTotalMass; // 1 / totalMass, folded into the spectral multiply
+    float fieldX, fieldY;  // uniform field, normalised units
+    float massScale;    // fixed-point deposit scale, as float
+    int pad1;
+};
+
+// Global work tables (built on the host whenever the batch shape changes).
+struct BodyBlock { int sim; int first; };
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) {
+    return fminf(fmaxf(v, lo), hi);
+}
+__device__ __forceinline__ float fractf(float v) { return v - floorf(v); }
+// GLSL mod(x, y) = x - y * floor(x / y)
+__device__ __forceinline__ float glsl_mod(float x, float y) { return x - y * floorf(x / y); }
+__device__ __forceinline__ int wrap_axis(int c, int size) {
+    if (c < 0) c += size; else if (c >= size) c -= size;
+    return c;
+}
+// Float index of an already floored coordinate, clamped to [0, size) in float
+// first so that non-finite positions land on a valid cell.
+__device__ __forceinline__ int floor_index(float v, int size) {
+    float f = floorf(v);
+    if (!(f >= 0.0f)) return 0;
+    if (f > (float)(size - 1)) return size - 1;
+    return (int)f;
+}
+
+}  // namespace gs

@@ -1,0 +1,726 @@
+// Step engine (K/GravityPipeline.kt): device state, batch tables, the step
+// schedule, graph capture and per-launch event timing.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace gs {
+
+int fft_row_lines_per_group(int L);
+int fft_columns_per_group(int H, int W);
+
+static int round_up(int v, int m) { return (v + m - 1) / m * m; }
+static bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// K/GravityPipeline.kt:1647-1648
+static int sort_axis_bits(int extent) {
+    int tiles = std::max(1, extent / 4);
+    int bits = 31 - __builtin_clz((unsigned)tiles);
+    return std::min(bits, 10);
+}
+
+// K/GravityPipeline.kt:1546-1547
+static int mass_scale_for(float totalMass) {
+    double v = 4000000000.0 / (double)totalMass;
+    long long s = (long long)v;
+    return (int)std::min<long long>(65536, std::max<long long>(1, s));
+}
+
+static void validate(const gs_config& c) {
+    auto req = [](bool ok, const char* msg) { if (!ok) throw ArgumentError(msg); };
+    req(c.viewport_width > 0 && c.viewport_height > 0, "viewport must be positive");
+    req(c.capacity > 0, "capacity must be positive");
+    req(c.spatial_sort_interval >= 0, "sort interval must be non-negative");
+    req(pow2(c.grid_width) && c.grid_width >= 16 && c.grid_width <= 2048,
+        "grid width must be a power of two in 16..2048");
+    req(pow2(c.grid_height) && c.grid_height >= 16 && c.grid_height <= 2048,
+        "grid height must be a power of two in 16..2048");
+    req(c.gravitational_constant >= 0.0f && std::isfinite(c.gravitational_constant), "gravity");
+    req(c.softening_cells > 0.0f && std::isfinite(c.softening_cells), "softening");
+    req(c.local_search_sigma_multiplier > 0.0f, "near-radius-sigma");
+    req(c.minimum_neighbor_radius_cells >= 1, "near-radius-min");
+    req(c.maximum_neighbor_radius_cells >= c.minimum_neighbor_radius_cells, "near-radius-max");
+    req(c.exact_neighbor_body_limit >= 1, "near-exact-bodies");
+    req(c.maximum_home_body_interactions >= 1, "near-home-bodies");
+    req(c.minimum_distance_cell_fraction > 0.0f, "near-min-distance-fraction");
+    req(c.minimum_assignment_window_power > 0.0f && c.minimum_assignment_window_power <= 1.0f,
+        "cic-window-floor");
+    req(c.time_step > 0.0f && std::isfinite(c.time_step), "dt");
+    req(c.maximum_substeps >= 1 && c.maximum_substeps <= 16, "collision substeps must be in 1..16");
+    req(c.contact_diameter_pixels >= 0.5f && c.contact_diameter_pixels <= 16.0f,
+        "contact diameter must be in 0.5..16.0 pixels");
+    req(c.neighbor_cell_radius >= 1 && c.neighbor_cell_radius <= 4, "collision-neighbor-radius");
+    req(c.restitution_coefficient >= 0.0f && c.restitution_coefficient <= 1.0f, "restitution");
+    req(c.friction >= 0.0f && std::isfinite(c.friction), "friction");
+    req(c.coincident_distance_squared_epsilon > 0.0f, "coincident-distance-epsilon");
+    req(c.neighbor_skin_pixels > 0.0f, "collision-skin");
+    req(c.kernel_radius_diameters >= 1.5f && c.kernel_radius_diameters <= 4.0f,
+        "liquid kernel must be 1.5..4 diameters");
+    req(c.liquid_iterations >= 1 && c.liquid_iterations <= 16, "liquid iterations must be in 1..16");
+    req(c.liquid_viscosity >= 0.0f && c.liquid_viscosity <= 1.0f, "liquid viscosity must be in 0..1");
+    req(!c.bounded || 2 * std::max(c.grid_width, c.grid_height) <= 2048,
+        "Walls need an FFT grid of at most 1024 cells per axis");
+}
+
+void Engine::check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw StateError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+static T* dalloc(size_t count) {
+    T* p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess)
+        throw StateError("hipMalloc failed");
+    if (hipMemset(p, 0, count * sizeof(T)) != hipSuccess) throw StateError("hipMemset failed");
+    return p;
+}
+
+Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float*>& states,
+               const std::vector<int>& counts) {
+    if (cfgs.empty()) throw ArgumentError("no simulation");
+    int substeps = cfgs[0].maximum_substeps;
+    for (const auto& c : cfgs) {
+        validate(c);
+        if (c.maximum_substeps != substeps || c.time_step != cfgs[0].time_step ||
+            c.spatial_sort_interval != cfgs[0].spatial_sort_interval)
+            throw ArgumentError("a batch shares dt, substeps and sort interval");
+    }
+    sims_.resize(cfgs.size());
+    int slot = 0, cells = 0, tiles = 0, phys = 0, mesh = 0;
+    for (size_t i = 0; i < cfgs.size(); i++) {
+        HostSim& h = sims_[i];
+        h.cfg = cfgs[i];
+        h.cap = cfgs[i].capacity;
+        h.base = slot;
+        slot += round_up(h.cap, WG);
+        h.bounded = cfgs[i].bounded;
+        h.mutual = cfgs[i].mutual_gravity;
+        h.liquid = cfgs[i].liquid;
+        h.substeps = cfgs[i].maximum_substeps;
+        h.fieldX = cfgs[i].field_acceleration_x;
+        h.fieldY = cfgs[i].field_acceleration_y;
+        int n0 = counts.size() > i ? counts[i] : 0;
+        if (n0 < 0 || n0 > h.cap) throw ArgumentError("scenario exceeds capacity");
+        h.n = n0;
+        float vw = (float)h.cfg.viewport_width, vh = (float)h.cfg.viewport_height;
+        float mx = std::max(vw, vh);
+        float guard = 64.0f * (std::nextafter(mx, INFINITY) - mx);
+        int cgw = std::max(1, (int)std::floor(vw / (h.cfg.contact_diameter_pixels + guard)));
+        int cgh = std::max(1, (int)std::floor(vh / (h.cfg.contact_diameter_pixels + guard)));
+        h.cells = cgw * cgh;
+        h.desc.cellBase = cells;
+        cells += h.cells;
+        int wb = sort_axis_bits(h.cfg.viewport_width), hb = sort_axis_bits(h.cfg.viewport_height);
+        h.tiles = 1 << (wb + hb);
+        h.desc.tileBase = tiles;
+        tiles += h.tiles;
+        h.phys = h.cfg.grid_width * h.cfg.grid_height;
+        h.desc.physBase = phys;
+        phys += h.phys;
+        bool wallsOk = 2 * std::max(h.cfg.grid_width, h.cfg.grid_height) <= 2048;
+        h.meshMax = h.phys * (wallsOk ? 4 : 1);
+        h.desc.meshBase = mesh;
+        mesh += h.meshMax;
+    }
+    totalSlots_ = std::max(slot, WG);
+    totalCells_ = cells;
+    totalTiles_ = tiles;
+    totalPhys_ = phys;
+    totalMesh_ = mesh;
+
+    check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "stream");
+    size_t S = (size_t)totalSlots_;
+    for (int b = 0; b < 2; b++) {
+        pos_[b] = dalloc<float2>(S);
+        vel_[b] = dalloc<float2>(S);
+        acc_[b] = dalloc<float2>(S);
+        mat_[b] = dalloc<float4>(S);
+        grid_[b].heads = dalloc<uint32_t>((size_t)totalCells_);
+        grid_[b].nodes = dalloc<float4>(S);
+        grid_[b].cellOf = dalloc<int>(S);
+    }
+    press_ = dalloc<float>(S);
+    counts_ = dalloc<uint32_t>(S);
+    contacts_ = dalloc<uint32_t>(S * CONTACT_SLOTS);
+    tileCount_ = dalloc<uint32_t>((size_t)totalTiles_ + 1);
+    tilePrefix_ = dalloc<uint32_t>((size_t)totalTiles_ + 1);
+    keyOf_ = dalloc<uint32_t>(S);
+    slotSrc_ = dalloc<uint32_t>(S);
+    dest_ = dalloc<uint32_t>(S);
+    int scanMax = std::max(scan_totals_size(totalTiles_), scan_totals_size(totalPhys_));
+    scanTmp_ = dalloc<uint32_t>((size_t)scanMax);
+    cellCount_ = dalloc<uint32_t>((size_t)totalPhys_ + 1);
+    cellPrefix_ = dalloc<uint32_t>((size_t)totalPhys_ + 1);
+    homeOf_ = dalloc<uint32_t>(S);
+    slotBody_ = dalloc<uint32_t>(S);
+    cellMass_ = dalloc<uint32_t>((size_t)totalPhys_);
+    cellBody_ = dalloc<float4>(S);
+    spec_ = dalloc<float2>((size_t)totalMesh_);
+    green_ = dalloc<float2>((size_t)totalMesh_);
+    tw_ = dalloc<float2>((size_t)twLen_);
+    fallback_ = dalloc<float2>(65536);
+    dSims_ = dalloc<SimDesc>(sims_.size());
+    int maxBlocks = totalSlots_ / WG;
+    dBlocks_ = dalloc<BodyBlock>((size_t)maxBlocks);
+    int maxLines = 0;
+    for (auto& h : sims_) maxLines += 2 * std::max(h.cfg.grid_width, h.cfg.grid_height) * 2 + 2;
+    dRowsF_ = dalloc<LineWork>((size_t)maxLines);
+    dRowsI_ = dalloc<LineWork>((size_t)maxLines);
+    dCols_ = dalloc<LineWork>((size_t)maxLines);
+
+    std::vector<float> host;
+    fft_twiddles(twLen_, host);
+    check(hipMemcpy(tw_, host.data(), host.size() * 4, hipMemcpyHostToDevice), "twiddles");
+    fallback_normals(host);
+    check(hipMemcpy(fallback_, host.data(), host.size() * 4, hipMemcpyHostToDevice), "normals");
+
+    // Default materials: unit mass, the configured friction, not solid
+    // (K/GravityPipeline.kt:1494-1504); the start state of the scenario.
+    for (size_t i = 0; i < sims_.size(); i++) {
+        HostSim& h = sims_[i];
+        std::vector<float> m((size_t)4 * h.cap);
+        for (int k = 0; k < h.cap; k++) {
+            m[4 * k] = 1.0f;
+            m[4 * k + 1] = h.cfg.friction;
+            m[4 * k + 2] = 0.0f;
+            m[4 * k + 3] = 0.0f;
+        }
+        for (int b = 0; b < 2; b++)
+            check(hipMemcpy(mat_[b] + h.base, m.data(), m.size() * 4, hipMemcpyHostToDevice), "mat");
+        if (states.size() > i && states[i] && h.n > 0) upload((int)i, 0, h.n, states[i], nullptr);
+    }
+    greenKeys_.resize(sims_.size(), GreenKey{0, 0, 0, 0, 0, 0, 0, 0});
+    for (auto& h : sims_) fill_desc(h);
+    for (int s = 0; s < (int)sims_.size(); s++) ensure_green(s);
+    push_descs();
+    rebuild_tables();
+    check(hipStreamSynchronize(st_), "init");
+}
+
+Engine::~Engine() {
+    invalidate_graphs();
+    for (auto& e : events_) {
+        hipEventDestroy(e.second.first);
+        hipEventDestroy(e.second.second);
+    }
+    void* ptrs[] = {pos_[0], pos_[1], vel_[0], vel_[1], acc_[0], acc_[1], mat_[0], mat_[1],
+                    grid_[0].heads, grid_[1].heads, grid_[0].nodes, grid_[1].nodes,
+                    grid_[0].cellOf, grid_[1].cellOf, press_, counts_, contacts_, tileCount_,
+                    tilePrefix_, keyOf_, slotSrc_, dest_, scanTmp_, cellCount_, cellPrefix_,
+                    homeOf_, slotBody_, cellMass_, cellBody_, spec_, green_, tw_, fallback_,
+                    dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (st_) hipStreamDestroy(st_);
+}
+
+void Engine::fill_desc(HostSim& h) {
+    SimDesc& d = h.desc;
+    const gs_config& c = h.cfg;
+    d.n = h.n;
+    d.base = h.base;
+    d.cap = h.cap;
+    d.bounded = h.bounded;
+    d.mutual = h.mutual;
+    d.liquid = h.liquid;
+    d.collide = c.collisions_enabled;
+    d.vw = (float)c.viewport_width;
+    d.vh = (float)c.viewport_height;
+    d.aspect = d.vw / d.vh;
+    d.dt = c.time_step;
+    d.subDt = c.time_step / (float)h.substeps;
+    d.diameter = c.contact_diameter_pixels;
+    d.diameter2 = d.diameter * d.diameter;
+    d.restitution = c.restitution_coefficient;
+    d.coincidentEps = c.coincident_distance_squared_epsilon;
+    float mx = std::max(d.vw, d.vh);
+    float guard = 64.0f * (std::nextafter(mx, INFINITY) - mx);
+    d.cgw = std::max(1, (int)std::floor(d.vw / (d.diameter + guard)));
+    d.cgh = std::max(1, (int)std::floor(d.vh / (d.diameter + guard)));
+    d.sortWb = sort_axis_bits(c.viewport_width);
+    d.sortHb = sort_axis_bits(c.viewport_height);
+    d.pw = c.grid_width;
+    d.ph = c.grid_height;
+    d.mw = h.bounded ? 2 * d.pw : d.pw;
+    d.mh = h.bounded ? 2 * d.ph : d.ph;
+    // K/GravityPipeline.kt:153-169
+    float nx = c.local_search_sigma_multiplier * c.softening_cells * (float)d.pw /
+               ((float)d.ph * d.aspect);
+    float ny = c.softening_cells * c.local_search_sigma_multiplier;
+    d.rx = std::min(std::max((int)std::ceil(nx), c.minimum_neighbor_radius_cells),
+                    c.maximum_neighbor_radius_cells);
+    d.ry = std::min(std::max((int)std::ceil(ny), c.minimum_neighbor_radius_cells),
+                    c.maximum_neighbor_radius_cells);
+    d.exactLimit = c.exact_neighbor_body_limit;
+    d.homeLimit = c.maximum_home_body_interactions;
+    d.G = c.gravitational_constant;
+    d.totalMass = std::max(1.0f, (float)h.n);
+    d.sigma = c.softening_cells / (float)d.ph;
+    d.twoSigma2 = 2.0f * d.sigma * d.sigma;
+    float cw = d.aspect / (float)d.pw, ch = 1.0f / (float)d.ph;
+    float md = c.minimum_distance_cell_fraction * std::min(cw, ch);
+    d.minDist2 = md * md;
+    int ms = mass_scale_for(d.totalMass);
+    d.massScale = (float)ms;
+    d.invMassScale = 1.0f / (float)ms;
+    d.invTotalMass = 1.0f / d.totalMass;
+    d.fieldX = h.fieldX / d.vw;
+    d.fieldY = h.fieldY / d.vh;
+}
+
+void Engine::push_descs() {
+    int rank0 = 0;
+    std::vector<SimDesc> all;
+    for (auto& h : sims_) {
+        h.desc.rank0 = rank0;
+        rank0 += h.n;
+        all.push_back(h.desc);
+    }
+    check(hipMemcpyAsync(dSims_, all.data(), all.size() * sizeof(SimDesc), hipMemcpyHostToDevice,
+                         st_), "descriptors");
+    check(hipStreamSynchronize(st_), "descriptors");
+}
+
+void Engine::ensure_green(int s) {
+    HostSim& h = sims_[s];
+    if (!h.mutual) return;
+    const SimDesc& d = h.desc;
+    GreenKey key{d.mw, d.mh, d.pw, d.ph, d.aspect, h.cfg.gravitational_constant,
+                 h.cfg.softening_cells, h.cfg.minimum_assignment_window_power};
+    if (greenKeys_[s] == key) return;
+    std::vector<float> g;
+    green_spectrum(key, g);
+    check(hipMemcpy(green_ + d.meshBase, g.data(), g.size() * 4, hipMemcpyHostToDevice), "green");
+    greenKeys_[s] = key;
+}
+
+void Engine::rebuild_tables() {
+    std::vector<BodyBlock> blocks;
+    std::vector<LineWork> rowsF, rowsI, cols;
+    anyMutual_ = anyLiquid_ = anyCollide_ = false;
+    gravitySlots_ = 0;
+    for (int s = 0; s < (int)sims_.size(); s++) {
+        const HostSim& h = sims_[s];
+        for (int i = 0; i < h.n; i += WG) blocks.push_back({s, h.base + i});
+        anyLiquid_ |= h.liquid != 0;
+        anyCollide_ |= h.cfg.collisions_enabled != 0;
+        if (!h.mutual) continue;
+        anyMutual_ = true;
+        gravitySlots_ += h.n;
+        const SimDesc& d = h.desc;
+        int active = h.bounded ? d.ph + 2 : d.mh;
+        int nl = fft_row_lines_per_group(d.mw);
+        for (int a = 0; a < active; a += nl) {
+            rowsF.push_back({s, a, std::min(nl, active - a), 0});
+            rowsI.push_back({s, a, std::min(nl, active - a), 0});
+        }
+        int cb = fft_columns_per_group(d.mh, d.mw);
+        for (int c = 0; c < d.mw; c += cb) cols.push_back({s, c, cb, 0});
+    }
+    bodyBlocks_ = (int)blocks.size();
+    rowsF_ = (int)rowsF.size();
+    rowsI_ = (int)rowsI.size();
+    cols_ = (int)cols.size();
+    if (!blocks.empty())
+        check(hipMemcpy(dBlocks_, blocks.data(), blocks.size() * sizeof(BodyBlock),
+                        hipMemcpyHostToDevice), "blocks");
+    if (!rowsF.empty()) {
+        check(hipMemcpy(dRowsF_, rowsF.data(), rowsF.size() * sizeof(LineWork), hipMemcpyHostToDevice), "rows");
+        check(hipMemcpy(dRowsI_, rowsI.data(), rowsI.size() * sizeof(LineWork), hipMemcpyHostToDevice), "rows");
+        check(hipMemcpy(dCols_, cols.data(), cols.size() * sizeof(LineWork), hipMemcpyHostToDevice), "cols");
+    }
+    shape_.sims = dSims_;
+    shape_.blocks = dBlocks_;
+    shape_.bodyBlocks = bodyBlocks_;
+    shape_.planeStride = totalSlots_;
+    shape_.guardLimit = totalSlots_ + 1;
+    invalidate_graphs();
+}
+
+void Engine::invalidate_graphs() {
+    for (auto& g : graphs_) hipGraphExecDestroy(g.second.exec);
+    graphs_.clear();
+}
+
+void Engine::clear_grids() {
+    for (int b = 0; b < 2; b++)
+        check(hipMemsetAsync(grid_[b].heads, 0, sizeof(uint32_t) * (size_t)totalCells_, st_), "grid");
+}
+
+void Engine::set_active(int s, int n) {
+    HostSim& h = sims_.at(s);
+    if (n < 0 || n > h.cap) throw ArgumentError("active body count out of range");
+    if (n == h.n) return;
+    h.n = n;
+    fill_desc(h);
+    push_descs();
+    clear_grids();
+    rebuild_tables();
+}
+
+void Engine::set_bounded(int s, bool v) {
+    HostSim& h = sims_.at(s);
+    if ((h.bounded != 0) == v) return;
+    if (v && 2 * std::max(h.cfg.grid_width, h.cfg.grid_height) > 2048)
+        throw ArgumentError("Walls need an FFT grid of at most 1024 cells per axis");
+    h.bounded = v;
+    fill_desc(h);
+    ensure_green(s);
+    push_descs();
+    rebuild_tables();
+}
+
+void Engine::set_mutual(int s, bool v) {
+    HostSim& h = sims_.at(s);
+    if ((h.mutual != 0) == v) return;
+    h.mutual = v;
+    fill_desc(h);
+    ensure_green(s);
+    push_descs();
+    rebuild_tables();
+}
+
+void Engine::set_liquid(int s, bool v) {
+    HostSim& h = sims_.at(s);
+    if ((h.liquid != 0) == v) return;
+    h.liquid = v;
+    fill_desc(h);
+    push_descs();
+    rebuild_tables();
+}
+
+void Engine::set_field(int s, float ax, float ay) {
+    HostSim& h = sims_.at(s);
+    if (h.fieldX == ax && h.fieldY == ay) return;
+    h.fieldX = ax;
+    h.fieldY = ay;
+    fill_desc(h);
+    push_descs();
+    invalidate_graphs();
+}
+
+void Engine::set_substeps(int substeps) {
+    for (auto& h : sims_) {
+        int v = std::min(std::max(substeps, 1), h.cfg.maximum_substeps);
+        if (v == h.substeps) continue;
+        h.substeps = v;
+        fill_desc(h);
+    }
+    push_descs();
+    invalidate_graphs();
+}
+
+void Engine::normalize() {
+    // K/GravityPipeline.kt:1326-1331: gravity follows the active mass.
+    bool changed = false;
+    for (auto& h : sims_) {
+        float mass = std::max(1.0f, (float)h.n);
+        if (mass == h.desc.totalMass) continue;
+        fill_desc(h);
+        changed = true;
+    }
+    if (changed) {
+        push_descs();
+        invalidate_graphs();
+    }
+}
+
+void Engine::upload(int s, int first, int count, const float* xyvv, const float* mats) {
+    HostSim& h = sims_.at(s);
+    if (first < 0 || count < 0 || first > h.cap || count > h.cap - first)
+        throw ArgumentError("upload range exceeds capacity");
+    if (count == 0) return;
+    std::vector<float2> p((size_t)count), v((size_t)count);
+    for (int i = 0; i < count; i++) {
+        p[i] = make_float2(xyvv[4 * i], xyvv[4 * i + 1]);
+        v[i] = make_float2(xyvv[4 * i + 2], xyvv[4 * i + 3]);
+    }
+    // Both ping-pong buffers, as uploadBodies writes bodies and collisionBodies.
+    for (int b = 0; b < 2; b++) {
+        check(hipMemcpyAsync(pos_[b] + h.base + first, p.data(), count * 8, hipMemcpyHostToDevice, st_), "upload");
+        check(hipMemcpyAsync(vel_[b] + h.base + first, v.data(), count * 8, hipMemcpyHostToDevice, st_), "upload");
+        if (mats)
+            check(hipMemcpyAsync(mat_[b] + h.base + first, mats, (size_t)count * 16, hipMemcpyHostToDevice, st_), "upload");
+    }
+    check(hipStreamSynchronize(st_), "upload");
+}
+
+void Engine::upload_accel(int s, int first, int count, const float* accel) {
+    HostSim& h = sims_.at(s);
+    if (first < 0 || count < 0 || first > h.cap || count > h.cap - first)
+        throw ArgumentError("upload range exceeds capacity");
+    if (count == 0) return;
+    check(hipMemcpy(acc_[auxPar_] + h.base + first, accel, (size_t)count * 8, hipMemcpyHostToDevice), "accel");
+}
+
+void Engine::download(int s, int what, int first, int count, void* dst) {
+    HostSim& h = sims_.at(s);
+    check(hipStreamSynchronize(st_), "download");
+    if (what == GS_FORCE_MESH) {
+        int nodes = h.desc.mw * h.desc.mh;
+        if (first < 0 || count < 0 || first + count > nodes) throw ArgumentError("mesh range");
+        check(hipMemcpy(dst, spec_ + h.desc.meshBase + first, (size_t)count * 8, hipMemcpyDeviceToHost), "mesh");
+        return;
+    }
+    if (first < 0 || count < 0 || first > h.cap || count > h.cap - first)
+        throw ArgumentError("download range exceeds capacity");
+    if (count == 0) return;
+    size_t o = (size_t)h.base + first;
+    switch (what) {
+        case GS_BODIES_XYVV: {
+            std::vector<float2> p((size_t)count), v((size_t)count);
+            check(hipMemcpy(p.data(), pos_[bodyPar_] + o, count * 8, hipMemcpyDeviceToHost), "download");
+            check(hipMemcpy(v.data(), vel_[bodyPar_] + o, count * 8, hipMemcpyDeviceToHost), "download");
+            float* out = (float*)dst;
+            for (int i = 0; i < count; i++) {
+                out[4 * i] = p[i].x; out[4 * i + 1] = p[i].y;
+                out[4 * i + 2] = v[i].x; out[4 * i + 3] = v[i].y;
+            }
+            return;
+        }
+        case GS_PRESSURES:
+            check(hipMemcpy(dst, press_ + o, (size_t)count * 4, hipMemcpyDeviceToHost), "download");
+            return;
+        case GS_ACCEL:
+            check(hipMemcpy(dst, acc_[auxPar_] + o, (size_t)count * 8, hipMemcpyDeviceToHost), "download");
+            return;
+        case GS_MATERIALS:
+            check(hipMemcpy(dst, mat_[auxPar_] + o, (size_t)count * 16, hipMemcpyDeviceToHost), "download");
+            return;
+        case GS_CONTACT_COUNTS:
+            check(hipMemcpy(dst, counts_ + o, (size_t)count * 4, hipMemcpyDeviceToHost), "download");
+            return;
+        case GS_SORT_DEST:
+            check(hipMemcpy(dst, dest_ + o, (size_t)count * 4, hipMemcpyDeviceToHost), "download");
+            return;
+        case GS_CONTACTS: {
+            std::vector<uint32_t> plane((size_t)count);
+            uint32_t* out = (uint32_t*)dst;
+            for (int k = 0; k < CONTACT_SLOTS; k++) {
+                check(hipMemcpy(plane.data(), contacts_ + (size_t)k * totalSlots_ + o, (size_t)count * 4,
+                                hipMemcpyDeviceToHost), "download");
+                for (int i = 0; i < count; i++) out[(size_t)i * CONTACT_SLOTS + k] = plane[i] - (uint32_t)h.base;
+            }
+            return;
+        }
+        default:
+            throw ArgumentError("unknown buffer");
+    }
+}
+
+void Engine::synchronize() { check(hipStreamSynchronize(st_), "synchronize"); }
+
+template <typename F>
+void Engine::timed(const char* name, F&& f) {
+    if (!profiling_) { f(); return; }
+    hipEvent_t a, b;
+    check(hipEventCreate(&a), "event");
+    check(hipEventCreate(&b), "event");
+    check(hipEventRecord(a, st_), "event");
+    f();
+    check(hipEventRecord(b, st_), "event");
+    events_.push_back({name, {a, b}});
+}
+
+// ---------------------------------------------------------------------------
+// Schedule
+
+void Engine::enqueue_sort() {
+    // K/GravityPipeline.kt:1269-1319
+    int o = bodyPar_ ^ 1, a = auxPar_ ^ 1;
+    timed("sort_count", [&] { launch_sort_count(st_, shape_, pos_[bodyPar_], tileCount_, keyOf_); });
+    timed("sort_scan", [&] { launch_scan(st_, tileCount_, tilePrefix_, totalTiles_, scanTmp_); });
+    timed("sort_scatter", [&] { launch_sort_scatter(st_, shape_, keyOf_, tileCount_, tilePrefix_, slotSrc_); });
+    timed("sort_canon", [&] {
+        launch_sort_canon(st_, shape_, keyOf_, tilePrefix_, slotSrc_, pos_[bodyPar_], vel_[bodyPar_],
+                          acc_[auxPar_], mat_[auxPar_], pos_[o], vel_[o], acc_[a], mat_[a], dest_);
+    });
+    bodyPar_ = o;
+    auxPar_ = a;
+}
+
+void Engine::enqueue_force_and_kick(bool initial) {
+    // K/GravityPipeline.kt:907-973 then :1213-1261
+    if (anyMutual_) {
+        timed("bin", [&] { launch_bin(st_, shape_, pos_[bodyPar_], cellCount_, homeOf_); });
+        timed("cell_scan", [&] { launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_); });
+        timed("cell_scatter", [&] { launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_); });
+        timed("cell_canon", [&] {
+            launch_cell_canon(st_, gravitySlots_, homeOf_, cellPrefix_, slotBody_, pos_[bodyPar_],
+                              mat_[auxPar_], cellBody_, cellMass_);
+        });
+        timed("fft_rows_forward", [&] {
+            launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_);
+        });
+        timed("fft_columns", [&] { launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_); });
+        timed("fft_rows_inverse", [&] { launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_); });
+    }
+    timed("kick", [&] {
+        launch_kick(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_], mat_[auxPar_], spec_,
+                    cellPrefix_, cellMass_, cellBody_, initial ? 1 : 0);
+    });
+}
+
+// One stand-alone pass (the test hook resolveCollisions).
+void Engine::enqueue_pass(float kickDt, float driftDt, bool writePressure) {
+    int g = gridPar_;
+    timed("drift_insert", [&] {
+        launch_drift_insert(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_], kickDt, driftDt, 1, grid_[g]);
+    });
+    timed("count", [&] {
+        launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+    });
+    timed("solve", [&] {
+        launch_solve(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], pos_[bodyPar_ ^ 1], vel_[bodyPar_ ^ 1],
+                     mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, writePressure ? 1 : 0,
+                     0.0f, 0, grid_[g ^ 1]);
+    });
+    bodyPar_ ^= 1;
+    gridPar_ ^= 1;
+}
+
+void Engine::enqueue_step(bool sortDue) {
+    // K/GravityPipeline.kt:445-494
+    if (sortDue) enqueue_sort();
+    const HostSim& h0 = sims_[0];
+    const float dt = h0.cfg.time_step;
+    const float sub = h0.desc.subDt;
+    const int S = h0.substeps;
+    if (anyCollide_) {
+        timed("drift_insert", [&] {
+            launch_drift_insert(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_], dt, sub, 1, grid_[gridPar_]);
+        });
+        for (int k = 0; k < S; k++) {
+            int g = gridPar_;
+            timed("count", [&] {
+                launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+            });
+            bool last = k == S - 1;
+            float nextDrift = last ? 0.0f : sub;
+            int insertNext = last ? (anyLiquid_ ? 0 : 1) : 1;
+            timed("solve", [&] {
+                launch_solve(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], pos_[bodyPar_ ^ 1], vel_[bodyPar_ ^ 1],
+                             mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, 0, nextDrift,
+                             insertNext, grid_[g ^ 1]);
+            });
+            bodyPar_ ^= 1;
+            gridPar_ ^= 1;
+        }
+        if (anyLiquid_) {
+            timed("insert", [&] {
+                launch_drift_insert(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_], 0.0f, 0.0f, 1, grid_[gridPar_]);
+            });
+        }
+        enqueue_force_and_kick(false);
+        // The final zero-drift pass writes pressures (K/GravityPipeline.kt:480-486).
+        int g = gridPar_;
+        timed("count", [&] {
+            launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+        });
+        timed("solve", [&] {
+            launch_solve(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], pos_[bodyPar_ ^ 1], vel_[bodyPar_ ^ 1],
+                         mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, 1, 0.0f, 0,
+                         grid_[g ^ 1]);
+        });
+        bodyPar_ ^= 1;
+        gridPar_ ^= 1;
+    } else {
+        // Gravity-only configurations: kick-drift-kick with plain drifts.
+        for (int k = 0; k < S; k++) {
+            timed("drift", [&] {
+                launch_drift_insert(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_],
+                                    k == 0 ? dt : 0.0f, sub, 0, grid_[gridPar_]);
+            });
+        }
+        enqueue_force_and_kick(false);
+    }
+}
+
+void Engine::initialize_acceleration() {
+    normalize();
+    enqueue_force_and_kick(true);
+    check(hipGetLastError(), "initialize acceleration");
+}
+
+void Engine::step() {
+    normalize();
+    const HostSim& h0 = sims_[0];
+    int interval = h0.cfg.spatial_sort_interval;
+    bool sortDue = false;
+    if (interval > 0) sortDue = (sims_[0].steps % interval) == 0;
+    for (auto& h : sims_) h.steps++;
+    if (bodyBlocks_ == 0) return;
+    if (!useGraphs_ || profiling_) {
+        enqueue_step(sortDue);
+        check(hipGetLastError(), "step");
+        return;
+    }
+    int key = bodyPar_ | (auxPar_ << 1) | (gridPar_ << 2) | ((sortDue ? 1 : 0) << 3);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        int bp = bodyPar_, ap = auxPar_, gp = gridPar_;
+        hipGraph_t graph;
+        check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
+        enqueue_step(sortDue);
+        check(hipStreamEndCapture(st_, &graph), "capture");
+        hipGraphExec_t exec;
+        check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+        hipGraphDestroy(graph);
+        // Capture only recorded the launches: keep where the schedule ends
+        // and start again from where it began.
+        CapturedStep c{exec, bodyPar_, auxPar_, gridPar_};
+        bodyPar_ = bp; auxPar_ = ap; gridPar_ = gp;
+        it = graphs_.emplace(key, c).first;
+    }
+    check(hipGraphLaunch(it->second.exec, st_), "graph launch");
+    bodyPar_ = it->second.bodyPar;
+    auxPar_ = it->second.auxPar;
+    gridPar_ = it->second.gridPar;
+}
+
+void Engine::stop_bodies(int s) {
+    // S/stop_bodies.comp: zero velocities in both ping-pong buffers.
+    HostSim& h = sims_.at(s);
+    for (int b = 0; b < 2; b++)
+        check(hipMemsetAsync(vel_[b] + h.base, 0, (size_t)h.n * 8, st_), "stop");
+}
+
+void Engine::debug_collide(float kickDt, float driftDt, bool writePressure) {
+    normalize();
+    if (bodyBlocks_ == 0) return;
+    enqueue_pass(kickDt, driftDt, writePressure);
+    check(hipGetLastError(), "collide");
+}
+
+void Engine::debug_sort() {
+    if (bodyBlocks_ == 0) return;
+    enqueue_sort();
+    check(hipGetLastError(), "sort");
+}
+
+std::vector<std::tuple<std::string, int, double>> Engine::profile(int steps) {
+    profiling_ = true;
+    for (int i = 0; i < steps; i++) step();
+    profiling_ = false;
+    check(hipStreamSynchronize(st_), "profile");
+    std::map<std::string, std::pair<int, double>> acc;
+    std::vector<std::string> order;
+    for (auto& e : events_) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, e.second.first, e.second.second), "elapsed");
+        if (!acc.count(e.first)) order.push_back(e.first);
+        acc[e.first].first += 1;
+        acc[e.first].second += ms;
+        hipEventDestroy(e.second.first);
+        hipEventDestroy(e.second.second);
+    }
+    events_.clear();
+    std::vector<std::tuple<std::string, int, double>> out;
+    for (auto& n : order) out.emplace_back(n, acc[n].first, acc[n].second);
+    return out;
+}
+
+}  // namespace gs

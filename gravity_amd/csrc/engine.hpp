@@ -1,0 +1,133 @@
+// The step engine: owns the device state of a batch of simulations and
+// replays the reference's step schedule (K/GravityPipeline.kt:445-494) as a
+// sequence of launches on one stream, captured as HIP graphs.
+#pragma once
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/gravity_hip.h"
+#include "common.hpp"
+#include "green.hpp"
+#include "kernels.hpp"
+
+namespace gs {
+
+// Mirrors Kotlin's IllegalArgumentException / IllegalStateException.
+struct ArgumentError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct StateError : std::runtime_error { using std::runtime_error::runtime_error; };
+
+struct HostSim {
+    gs_config cfg;
+    int n = 0;          // active bodies
+    int cap = 0;
+    int base = 0;       // first global body index
+    int bounded = 0, mutual = 1, liquid = 0;
+    int substeps = 16;
+    float fieldX = 0.0f, fieldY = 0.0f;   // px/s^2
+    long long steps = 0;                  // physicsSteps
+    int cells = 0;                        // broad-phase cells
+    int tiles = 0;                        // sort tiles
+    int phys = 0;                         // physical P3M cells
+    int meshMax = 0;                      // mesh nodes reserved (walls double)
+    SimDesc desc{};
+};
+
+class Engine {
+public:
+    Engine(const std::vector<gs_config>& cfgs, const std::vector<const float*>& states,
+           const std::vector<int>& counts);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    int sims() const { return (int)sims_.size(); }
+    const HostSim& sim(int s) const { return sims_.at(s); }
+
+    void set_active(int s, int n);
+    void set_bounded(int s, bool v);
+    void set_mutual(int s, bool v);
+    void set_liquid(int s, bool v);
+    void set_field(int s, float ax, float ay);
+    void set_substeps(int substeps);
+    void set_graphs(bool on) { useGraphs_ = on; }
+
+    void upload(int s, int first, int count, const float* xyvv, const float* mats);
+    void upload_accel(int s, int first, int count, const float* accel);
+    void download(int s, int what, int first, int count, void* dst);
+    void synchronize();
+
+    void initialize_acceleration();
+    void step();
+    void stop_bodies(int s);
+    void debug_collide(float kickDt, float driftDt, bool writePressure);
+    void debug_sort();
+
+    // Eager steps with an event pair around every launch.
+    std::vector<std::tuple<std::string, int, double>> profile(int steps);
+    int mesh_w(int s) const { return sims_.at(s).desc.mw; }
+    int mesh_h(int s) const { return sims_.at(s).desc.mh; }
+
+private:
+    void check(hipError_t e, const char* what);
+    void fill_desc(HostSim& h);
+    void push_descs();
+    void rebuild_tables();
+    void normalize();
+    void ensure_green(int s);
+    void enqueue_step(bool sortDue);
+    void enqueue_sort();
+    void enqueue_force_and_kick(bool initial);
+    void enqueue_pass(float kickDt, float driftDt, bool writePressure);
+    void clear_grids();
+    void invalidate_graphs();
+    template <typename F> void timed(const char* name, F&& f);
+
+    std::vector<HostSim> sims_;
+    hipStream_t st_ = nullptr;
+    int totalSlots_ = 0;      // body slots (sum of 256-rounded capacities)
+    int totalCells_ = 0, totalTiles_ = 0, totalPhys_ = 0, totalMesh_ = 0;
+    int twLen_ = 2048;
+    bool useGraphs_ = true;
+
+    // bodies (ping-pong)
+    float2* pos_[2] = {nullptr, nullptr};
+    float2* vel_[2] = {nullptr, nullptr};
+    float2* acc_[2] = {nullptr, nullptr};
+    float4* mat_[2] = {nullptr, nullptr};
+    int bodyPar_ = 0, auxPar_ = 0, gridPar_ = 0;
+    float* press_ = nullptr;
+    uint32_t* counts_ = nullptr;
+    uint32_t* contacts_ = nullptr;
+    CollideGrid grid_[2] = {};
+    // sort
+    uint32_t *tileCount_ = nullptr, *tilePrefix_ = nullptr, *keyOf_ = nullptr;
+    uint32_t *slotSrc_ = nullptr, *dest_ = nullptr, *scanTmp_ = nullptr;
+    // P3M
+    uint32_t *cellCount_ = nullptr, *cellPrefix_ = nullptr, *homeOf_ = nullptr;
+    uint32_t *slotBody_ = nullptr, *cellMass_ = nullptr;
+    float4* cellBody_ = nullptr;
+    float2 *spec_ = nullptr, *green_ = nullptr, *tw_ = nullptr, *fallback_ = nullptr;
+    std::vector<GreenKey> greenKeys_;
+    // tables
+    SimDesc* dSims_ = nullptr;
+    BodyBlock* dBlocks_ = nullptr;
+    LineWork *dRowsF_ = nullptr, *dCols_ = nullptr, *dRowsI_ = nullptr;
+    int bodyBlocks_ = 0, rowsF_ = 0, cols_ = 0, rowsI_ = 0, gravitySlots_ = 0;
+    bool anyMutual_ = false, anyLiquid_ = false, anyCollide_ = false;
+    LaunchShape shape_{};
+
+    // graphs keyed by (bodyPar, auxPar, gridPar, sortDue), with the parities
+    // the captured schedule ends in
+    struct CapturedStep { hipGraphExec_t exec; int bodyPar, auxPar, gridPar; };
+    std::map<int, CapturedStep> graphs_;
+
+    // profiling
+    bool profiling_ = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events_;
+};
+
+}  // namespace gs

@@ -1,0 +1,297 @@
+// P3M mesh: deposit, 2D FFT, Green's-function multiply, inverse 2D FFT
+// (S/clear_grid.comp, S/build_grid.comp:53-69, S/fft_init.comp,
+// S/fft_shared.comp, S/convolve.comp; K/GravityPipeline.kt:907-973).
+//
+// Three launches per force build:
+//   rows forward : per mesh row, gather the fixed-point CIC deposit of the
+//                  bodies of the three physical rows that can reach it
+//                  (integer LDS atomics: exact and order-free), convert to
+//                  float (S/fft_init.comp) and transform along x;
+//   columns      : per block of columns, forward transform along y, multiply
+//                  by the precomputed spectrum G(k) = deconvolution / (W H) *
+//                  (Kx(k) + i Ky(k)) / M, inverse transform along y;
+//   rows inverse : per needed mesh row, inverse transform along x. Because
+//                  the x and y force kernels are real, one complex inverse
+//                  yields Fx in the real part and Fy in the imaginary part.
+// Transforms are Stockham autosort in LDS with radix-8 butterflies (radix 4
+// or 2 for the last stage) and twiddles from an FP64-rounded table. Under
+// walls the mesh doubles; rows and columns that hold only zero padding are
+// neither loaded nor transformed in the forward row pass, and only rows the
+// force gather reads are produced by the inverse passes.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gs {
+
+constexpr int ROW_LDS = 4096;   // complex values per row workgroup
+constexpr int COL_LDS = 8192;   // complex values per column workgroup
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// Multiply by exp(DIR i pi / 2): -i forward, +i inverse.
+template <int DIR>
+__device__ __forceinline__ float2 mul_w4(float2 a) {
+    return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft2(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+    float2 a0 = cadd(v0, v2), a1 = csub(v0, v2);
+    float2 a2 = cadd(v1, v3), a3 = mul_w4<DIR>(csub(v1, v3));
+    v0 = cadd(a0, a2);
+    v2 = csub(a0, a2);
+    v1 = cadd(a1, a3);
+    v3 = csub(a1, a3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft8(float2* v) {
+    const float c = 0.70710678118654752440f;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        a[k] = cadd(v[k], v[k + 4]);
+        b[k] = csub(v[k], v[k + 4]);
+    }
+    b[1] = cmul(b[1], make_float2(c, DIR * c));
+    b[2] = mul_w4<DIR>(b[2]);
+    b[3] = cmul(b[3], make_float2(-c, DIR * c));
+    dft4<DIR>(a[0], a[1], a[2], a[3]);
+    dft4<DIR>(b[0], b[1], b[2], b[3]);
+    v[0] = a[0]; v[2] = a[1]; v[4] = a[2]; v[6] = a[3];
+    v[1] = b[0]; v[3] = b[1]; v[5] = b[2]; v[7] = b[3];
+}
+
+// One Stockham stage of radix R on nl lines of length L held in LDS;
+// element k of line l lives at buf[l * ls + k * es]. Q butterflies a lane.
+// jFast: consecutive lanes take consecutive butterflies of one line (rows);
+// otherwise the same butterfly of consecutive lines (columns).
+template <int DIR, int R, int Q>
+__device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int ls, int es, int Ns,
+                                               bool jFast, const float2* __restrict__ tw,
+                                               int twLen) {
+    const int perLine = L / R;
+    const int total = perLine * nl;
+    float2 v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        int bi = (int)threadIdx.x + q * WG;
+        if (bi < total) {
+            int line = jFast ? bi / perLine : bi % nl, j = jFast ? bi % perLine : bi / nl;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = buf[line * ls + (j + r * perLine) * es];
+        }
+    }
+    __syncthreads();
+    const int twStep = twLen / (Ns * R);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        int bi = (int)threadIdx.x + q * WG;
+        if (bi < total) {
+            int line = jFast ? bi / perLine : bi % nl, j = jFast ? bi % perLine : bi / nl;
+            int k = j & (Ns - 1);
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    float2 w = tw[r * k * twStep];
+                    if (DIR > 0) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            if (R == 8) dft8<DIR>(v[q]);
+            else if (R == 4) dft4<DIR>(v[q][0], v[q][1], v[q][2], v[q][3]);
+            else dft2<DIR>(v[q]);
+            int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[line * ls + (base + r * Ns) * es] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Transforms nl lines (nl * L <= 8192). DIR = -1 forward, +1 inverse
+// (unscaled, like S/fft_shared.comp).
+template <int DIR>
+__device__ void lds_fft(float2* buf, int L, int nl, int ls, int es, const float2* __restrict__ tw,
+                        int twLen) {
+    const bool jFast = es == 1;
+    int Ns = 1;
+    while (Ns < L) {
+        int rem = L / Ns;
+        if (rem >= 8) {
+            stockham_stage<DIR, 8, 4>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
+            Ns *= 8;
+        } else if (rem == 4) {
+            stockham_stage<DIR, 4, 8>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
+            Ns *= 4;
+        } else {
+            stockham_stage<DIR, 2, 16>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
+            Ns *= 2;
+        }
+    }
+}
+
+// Mesh row of forward/inverse work row a: all rows on the torus; under walls
+// rows 0..ph and the last one (cells at -1 wrap there).
+__device__ __forceinline__ int mesh_row(const SimDesc& s, int a) {
+    if (!s.bounded) return a;
+    return a <= s.ph ? a : s.mh - 1;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(WG) k_fft_rows_forward(const SimDesc* __restrict__ sims,
+                                                         const LineWork* __restrict__ work,
+                                                         const uint32_t* __restrict__ cellPrefix,
+                                                         const float4* __restrict__ cellBody,
+                                                         float2* __restrict__ spec,
+                                                         const float2* __restrict__ tw, int twLen) {
+    __shared__ float2 data[ROW_LDS];
+    __shared__ uint32_t mass[ROW_LDS];
+    LineWork w = work[blockIdx.x];
+    const SimDesc& s = sims[w.sim];
+    const int L = s.mw;
+    for (int i = (int)threadIdx.x; i < w.count * L; i += WG) mass[i] = 0u;
+    __syncthreads();
+    const float scale = s.massScale;
+    for (int l = 0; l < w.count; l++) {
+        int y = mesh_row(s, w.first + l);
+        // Physical rows whose bodies can deposit into mesh row y.
+        int r0, r1;
+        if (s.bounded) {
+            if (y == s.mh - 1) { r0 = 0; r1 = 0; }
+            else { r0 = y - 1 < 0 ? 0 : y - 1; r1 = y + 1 > s.ph - 1 ? s.ph - 1 : y + 1; }
+        } else if (s.ph < 3) {
+            r0 = 0; r1 = s.ph - 1;
+        } else {
+            r0 = y - 1; r1 = y + 1;
+        }
+        for (int rr = r0; rr <= r1; rr++) {
+            int r = s.bounded || s.ph < 3 ? rr : wrap_axis(rr, s.ph);
+            int c0 = s.physBase + r * s.pw;
+            int t0 = (int)cellPrefix[c0], t1 = (int)cellPrefix[c0 + s.pw];
+            for (int t = t0 + (int)threadIdx.x; t < t1; t += WG) {
+                float4 bd = cellBody[t];
+                // S/build_grid.comp:53-69, round() half to even.
+                float gx = bd.x * (float)s.pw - 0.5f, gy = bd.y * (float)s.ph - 0.5f;
+                float flx = floorf(gx), fly = floorf(gy);
+                int bx = (int)flx, by = (int)fly;
+                float fx = gx - flx, fy = gy - fly;
+                float wgt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy,
+                                fx * fy};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    int my = wrap_axis(by + (k >> 1), s.mh);
+                    if (my != y) continue;
+                    int mx = wrap_axis(bx + (k & 1), s.mw);
+                    float wk = wgt[k] * bd.z;
+                    uint32_t q = (uint32_t)rintf(wk * scale);
+                    atomicAdd(&mass[l * L + mx], q);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < w.count * L; i += WG)
+        data[i] = make_float2((float)mass[i] * s.invMassScale, 0.0f);
+    __syncthreads();
+    lds_fft<-1>(data, L, w.count, L, 1, tw, twLen);
+    for (int i = (int)threadIdx.x; i < w.count * L; i += WG) {
+        int l = i / L, x = i - l * L;
+        int y = mesh_row(s, w.first + l);
+        spec[s.meshBase + y * s.mw + x] = data[i];
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_fft_columns(const SimDesc* __restrict__ sims,
+                                                    const LineWork* __restrict__ work,
+                                                    float2* __restrict__ spec,
+                                                    const float2* __restrict__ green,
+                                                    const float2* __restrict__ tw, int twLen) {
+    __shared__ float2 data[COL_LDS];
+    LineWork w = work[blockIdx.x];
+    const SimDesc& s = sims[w.sim];
+    const int H = s.mh, CB = w.count, c0 = w.first;
+    const float2* src = spec + s.meshBase;
+    for (int i = (int)threadIdx.x; i < CB * H; i += WG) {
+        int y = i / CB, c = i - y * CB;
+        bool live = !s.bounded || y <= s.ph || y == H - 1;
+        data[i] = live ? src[y * s.mw + c0 + c] : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    lds_fft<-1>(data, H, CB, 1, CB, tw, twLen);
+    const float2* G = green + s.meshBase;
+    for (int i = (int)threadIdx.x; i < CB * H; i += WG) {
+        int y = i / CB, c = i - y * CB;
+        float2 z = cmul(data[i], G[y * s.mw + c0 + c]);
+        data[i] = make_float2(z.x * s.invTotalMass, z.y * s.invTotalMass);
+    }
+    __syncthreads();
+    lds_fft<1>(data, H, CB, 1, CB, tw, twLen);
+    float2* dst = spec + s.meshBase;
+    for (int i = (int)threadIdx.x; i < CB * H; i += WG) {
+        int y = i / CB, c = i - y * CB;
+        bool live = !s.bounded || y <= s.ph || y == H - 1;
+        if (live) dst[y * s.mw + c0 + c] = data[i];
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_fft_rows_inverse(const SimDesc* __restrict__ sims,
+                                                         const LineWork* __restrict__ work,
+                                                         float2* __restrict__ spec,
+                                                         const float2* __restrict__ tw, int twLen) {
+    __shared__ float2 data[ROW_LDS];
+    LineWork w = work[blockIdx.x];
+    const SimDesc& s = sims[w.sim];
+    const int L = s.mw;
+    for (int i = (int)threadIdx.x; i < w.count * L; i += WG) {
+        int l = i / L, x = i - l * L;
+        data[i] = spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x];
+    }
+    __syncthreads();
+    lds_fft<1>(data, L, w.count, L, 1, tw, twLen);
+    for (int i = (int)threadIdx.x; i < w.count * L; i += WG) {
+        int l = i / L, x = i - l * L;
+        spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = data[i];
+    }
+}
+
+void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
+                             const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
+                             const float2* twiddles, int twiddleLength) {
+    if (items == 0) return;
+    hipLaunchKernelGGL(k_fft_rows_forward, dim3(items), dim3(WG), 0, st, sims, work, cellPrefix,
+                       cellBody, spec, twiddles, twiddleLength);
+}
+
+void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
+                        float2* spec, const float2* green, const float2* twiddles,
+                        int twiddleLength) {
+    if (items == 0) return;
+    hipLaunchKernelGGL(k_fft_columns, dim3(items), dim3(WG), 0, st, sims, work, spec, green,
+                       twiddles, twiddleLength);
+}
+
+void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
+                             float2* spec, const float2* twiddles, int twiddleLength) {
+    if (items == 0) return;
+    hipLaunchKernelGGL(k_fft_rows_inverse, dim3(items), dim3(WG), 0, st, sims, work, spec,
+                       twiddles, twiddleLength);
+}
+
+int fft_row_lines_per_group(int L) { return L >= ROW_LDS ? 1 : ROW_LDS / L; }
+int fft_columns_per_group(int H, int W) {
+    int cb = COL_LDS / H;
+    return cb < W ? cb : W;
+}
+
+}  // namespace gs

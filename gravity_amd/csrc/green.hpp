@@ -1,0 +1,25 @@
+// Host-built tables of the step: force-kernel spectrum, FFT twiddles and
+// the coincident-pair normals.
+#pragma once
+
+#include <vector>
+
+namespace gs {
+
+struct GreenKey {
+    int mw, mh, pw, ph;
+    float aspect, G, sigmaCells, windowFloor;
+    bool operator==(const GreenKey& o) const {
+        return mw == o.mw && mh == o.mh && pw == o.pw && ph == o.ph && aspect == o.aspect &&
+               G == o.G && sigmaCells == o.sigmaCells && windowFloor == o.windowFloor;
+    }
+};
+
+// Interleaved complex float G(k) for a mw x mh mesh at unit total mass.
+void green_spectrum(const GreenKey& key, std::vector<float>& out);
+// exp(-2 pi i m / length), m < length, interleaved complex float.
+void fft_twiddles(int length, std::vector<float>& out);
+// cos/sin of the 65536 fallback angles, interleaved.
+void fallback_normals(std::vector<float>& out);
+
+}  // namespace gs

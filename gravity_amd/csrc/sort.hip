@@ -1,0 +1,219 @@
+// Device-wide exclusive scan and the spatial sort
+// (S/spatial_sort_{key.glsl,count,scan,scatter,materials}.comp,
+// K/GravityPipeline.kt:1269-1319).
+//
+// The sort is a counting sort on the reference's Morton tile key. The
+// scatter hands out ranks by counting every tile back down to zero (so the
+// counts never need clearing); a canonicalisation pass then orders each tile
+// by original body index, which makes the permutation deterministic and
+// stable. Positions, velocities, accelerations and materials move bit for bit.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gs {
+
+constexpr int SCAN_BLOCK = 1024;  // values per workgroup (256 lanes x 4)
+
+__device__ __forceinline__ uint32_t wave_inclusive(uint32_t v) {
+    int lane = (int)threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, (unsigned)o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive prefix of v over the workgroup; *total receives the sum.
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t waveSums[WG / 64];
+    uint32_t inc = wave_inclusive(v);
+    int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    if (lane == 63) waveSums[w] = inc;
+    __syncthreads();
+    uint32_t off = 0u, tot = 0u;
+#pragma unroll
+    for (int i = 0; i < WG / 64; i++) {
+        uint32_t s = waveSums[i];
+        if (i < w) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+// Local exclusive prefix of each 1024-value block; block sums to totals.
+__global__ void __launch_bounds__(WG) k_scan_local(const uint32_t* __restrict__ in,
+                                                   uint32_t* __restrict__ out, int len,
+                                                   uint32_t* __restrict__ totals) {
+    int base = (int)blockIdx.x * SCAN_BLOCK + (int)threadIdx.x * 4;
+    uint32_t v[4];
+    uint32_t sum = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int idx = base + i;
+        v[i] = idx < len ? in[idx] : 0u;
+        sum += v[i];
+    }
+    uint32_t total;
+    uint32_t before = block_exclusive(sum, &total);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int idx = base + i;
+        if (idx <= len) out[idx] = before;
+        before += v[i];
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive prefix of the block totals, in place.
+__global__ void __launch_bounds__(WG) k_scan_totals(uint32_t* totals, int blocks) {
+    uint32_t carry = 0u;
+    for (int chunk = 0; chunk < blocks; chunk += WG * 4) {
+        int base = chunk + (int)threadIdx.x * 4;
+        uint32_t v[4];
+        uint32_t sum = 0u;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[i] = base + i < blocks ? totals[base + i] : 0u;
+            sum += v[i];
+        }
+        uint32_t total;
+        uint32_t before = block_exclusive(sum, &total) + carry;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (base + i < blocks) totals[base + i] = before;
+            before += v[i];
+        }
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int len,
+                                                 const uint32_t* __restrict__ totals) {
+    int base = (int)blockIdx.x * SCAN_BLOCK;
+    uint32_t add = totals[blockIdx.x];
+    for (int i = (int)threadIdx.x; i < SCAN_BLOCK; i += WG) {
+        int idx = base + i;
+        if (idx <= len) out[idx] += add;
+    }
+}
+
+int scan_totals_size(int len) { return (len + 1 + SCAN_BLOCK - 1) / SCAN_BLOCK; }
+
+void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals) {
+    int blocks = scan_totals_size(len);
+    hipLaunchKernelGGL(k_scan_local, dim3(blocks), dim3(WG), 0, st, in, out, len, totals);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(WG), 0, st, totals, blocks);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals);
+}
+
+// ---------------------------------------------------------------------------
+// Spatial sort
+
+__device__ __forceinline__ uint32_t spread_bits(uint32_t v) {
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// S/spatial_sort_key.glsl:16-25
+__device__ __forceinline__ uint32_t sort_key(float x, float y, int wb, int hb) {
+    int sx = 1 << wb, sy = 1 << hb;
+    uint32_t tx = (uint32_t)floor_index(x * (float)sx, sx);
+    uint32_t ty = (uint32_t)floor_index(y * (float)sy, sy);
+    int common = wb < hb ? wb : hb;
+    uint32_t low = (1u << common) - 1u;
+    uint32_t morton = spread_bits(tx & low) | (spread_bits(ty & low) << 1);
+    return morton | (((tx | ty) >> common) << (2 * common));
+}
+
+// S/spatial_sort_count.comp:8-12
+__global__ void __launch_bounds__(WG) k_sort_count(const SimDesc* __restrict__ sims,
+                                                   const BodyBlock* __restrict__ blocks,
+                                                   const float2* __restrict__ pos,
+                                                   uint32_t* tileCount, uint32_t* keyOf) {
+    BodyBlock b = blocks[blockIdx.x];
+    const SimDesc& s = sims[b.sim];
+    int g = b.first + (int)threadIdx.x;
+    if (g - s.base >= s.n) return;
+    float2 p = pos[g];
+    uint32_t t = (uint32_t)s.tileBase + sort_key(p.x, p.y, s.sortWb, s.sortHb);
+    keyOf[g] = t;
+    atomicAdd(&tileCount[t], 1u);
+}
+
+// S/spatial_sort_scatter.comp:14-28: counting down returns every count to 0.
+__global__ void __launch_bounds__(WG) k_sort_scatter(const SimDesc* __restrict__ sims,
+                                                     const BodyBlock* __restrict__ blocks,
+                                                     const uint32_t* __restrict__ keyOf,
+                                                     uint32_t* tileCount,
+                                                     const uint32_t* __restrict__ tilePrefix,
+                                                     uint32_t* __restrict__ slotSrc) {
+    BodyBlock b = blocks[blockIdx.x];
+    const SimDesc& s = sims[b.sim];
+    int g = b.first + (int)threadIdx.x;
+    if (g - s.base >= s.n) return;
+    uint32_t t = keyOf[g];
+    uint32_t rank = atomicSub(&tileCount[t], 1u) - 1u;
+    int slot = s.base + (int)(tilePrefix[t] - (uint32_t)s.rank0 + rank);
+    slotSrc[slot] = (uint32_t)g;
+}
+
+// Orders every tile by original index and moves the bodies.
+__global__ void __launch_bounds__(WG) k_sort_canon(
+    const SimDesc* __restrict__ sims, const BodyBlock* __restrict__ blocks,
+    const uint32_t* __restrict__ keyOf, const uint32_t* __restrict__ tilePrefix,
+    const uint32_t* __restrict__ slotSrc, const float2* __restrict__ posIn,
+    const float2* __restrict__ velIn, const float2* __restrict__ accIn,
+    const float4* __restrict__ matIn, float2* __restrict__ posOut, float2* __restrict__ velOut,
+    float2* __restrict__ accOut, float4* __restrict__ matOut, uint32_t* __restrict__ dest) {
+    BodyBlock b = blocks[blockIdx.x];
+    const SimDesc& s = sims[b.sim];
+    int d = b.first + (int)threadIdx.x;
+    if (d - s.base >= s.n) return;
+    uint32_t src = slotSrc[d];
+    uint32_t t = keyOf[src];
+    int s0 = s.base + (int)(tilePrefix[t] - (uint32_t)s.rank0);
+    int s1 = s.base + (int)(tilePrefix[t + 1] - (uint32_t)s.rank0);
+    int rank = 0;
+    for (int e = s0; e < s1; e++) rank += slotSrc[e] < src ? 1 : 0;
+    int fin = s0 + rank;
+    posOut[fin] = posIn[src];
+    velOut[fin] = velIn[src];
+    accOut[fin] = accIn[src];
+    matOut[fin] = matIn[src];
+    dest[src] = (uint32_t)(fin - s.base);
+}
+
+void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float2* pos,
+                       uint32_t* tileCount, uint32_t* keyOf) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_sort_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                       pos, tileCount, keyOf);
+}
+
+void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
+                         uint32_t* tileCount, const uint32_t* tilePrefix, uint32_t* slotSrc) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_sort_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                       keyOf, tileCount, tilePrefix, slotSrc);
+}
+
+void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
+                       const uint32_t* tilePrefix, const uint32_t* slotSrc,
+                       const float2* posIn, const float2* velIn, const float2* accIn,
+                       const float4* matIn, float2* posOut, float2* velOut, float2* accOut,
+                       float4* matOut, uint32_t* dest) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_sort_canon, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                       keyOf, tilePrefix, slotSrc, posIn, velIn, accIn, matIn, posOut, velOut,
+                       accOut, matOut, dest);
+}
+
+}  // namespace gs

@@ -1,0 +1,177 @@
+/*
+ * gravity_hip.h: C ABI of the MI355X particle step of Gravity.
+ *
+ * The drop-in boundary for the reference's OpenGL compute path: each entry
+ * point names the Kotlin member of dev.gravity.GravityPipeline (or
+ * GravityConfig) it replaces, as file:line in the reference
+ * (K/ = src/main/kotlin/dev/gravity/). A JNI shim maps them one to one onto a
+ * Kotlin facade with the same members.
+ *
+ * Conventions (K/GravityPipeline.kt:73-81 and the Kotlin require/check):
+ *   - 0 is success; GS_EINVAL mirrors IllegalArgumentException (a bad
+ *     argument), GS_ESTATE mirrors IllegalStateException (a GPU or state
+ *     failure). gs_last_error() holds the message of the last failure on the
+ *     calling thread.
+ *   - Bodies are in the reference's units: position / viewport and
+ *     velocity / viewport per axis; xyvv arrays are the reference's
+ *     vec4(x, y, vx, vy) records, materials vec4(mass, friction, solid, group).
+ *   - A handle owns every device buffer and one HIP stream; calls on one
+ *     handle must not overlap (the reference's single GL thread).
+ *   - Everything is asynchronous on the handle's stream except gs_download,
+ *     gs_synchronize and the diagnostics, which block.
+ */
+#ifndef GRAVITY_HIP_H
+#define GRAVITY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_OK 0
+#define GS_EINVAL (-1)
+#define GS_ESTATE (-2)
+
+/* Flat copy of the GravityConfig fields the step reads
+ * (K/GravityConfig.kt:610-860) plus the viewport and start state. */
+typedef struct gs_config {
+    /* viewport and start (K/GravityPipeline.kt:73-81, StartConfig) */
+    int viewport_width;
+    int viewport_height;
+    int bounded;                        /* walls; false = torus */
+    int mutual_gravity;
+    int liquid;
+    /* particles (ParticleConfig, K/GravityConfig.kt:610-626) */
+    int capacity;
+    int spatial_sort_interval;
+    /* gravity (GravityFieldConfig, K/GravityConfig.kt:636-671) */
+    int grid_width;
+    int grid_height;
+    float gravitational_constant;
+    float softening_cells;
+    float local_search_sigma_multiplier;
+    int minimum_neighbor_radius_cells;
+    int maximum_neighbor_radius_cells;
+    int exact_neighbor_body_limit;
+    int maximum_home_body_interactions;
+    float minimum_distance_cell_fraction;
+    float minimum_assignment_window_power;
+    /* collisions (CollisionConfig, K/GravityConfig.kt:673-738) */
+    float time_step;
+    int maximum_substeps;
+    float contact_diameter_pixels;
+    int neighbor_cell_radius;
+    float restitution_coefficient;
+    float friction;
+    float coincident_distance_squared_epsilon;
+    float neighbor_skin_pixels;
+    int sleep;
+    float sleep_speed_pixels;
+    /* liquid (LiquidConfig, K/GravityConfig.kt:838-855) */
+    float kernel_radius_diameters;
+    int liquid_iterations;
+    float liquid_viscosity;
+    /* uniform field in px/s^2 (GravityPipeline.fieldAccelerationX/Y, :108-109) */
+    float field_acceleration_x;
+    float field_acceleration_y;
+    /* Extension without a reference counterpart: 0 replaces the contact
+     * passes by plain drifts (the gravity-only benchmark configurations). */
+    int collisions_enabled;
+} gs_config;
+
+typedef struct gs_sim gs_sim;
+
+/* GravityConfig() defaults (K/GravityConfig.kt:16-26, 610-855) on a
+ * 3440 x 1440 torus with mutual gravity, as the reference's benchmarks run. */
+void gs_config_defaults(gs_config *cfg);
+
+/* GravityPipeline(config, viewportWidth, viewportHeight, profiler,
+ * scenarioState, bounded) (K/GravityPipeline.kt:73-81, init :338-364).
+ * scenario_xyvv (nullable) holds n_scenario normalised bodies; the rest of
+ * the capacity is zero. Accelerations start at zero, as after init. */
+int gs_create(const gs_config *cfg, const float *scenario_xyvv, int n_scenario, gs_sim **out);
+
+/* GravityPipeline.close() (K/GravityPipeline.kt:1556). */
+void gs_destroy(gs_sim *sim);
+
+/* Property setters (K/GravityPipeline.kt:97-121): BOUNDED throws when the
+ * FFT axis would exceed 2048 (supportsWalls, :1628). */
+#define GS_BOUNDED 1
+#define GS_MUTUAL_GRAVITY 2
+#define GS_LIQUID 3
+int gs_set_bool(gs_sim *sim, int key, int value);
+
+/* fieldAccelerationX / fieldAccelerationY (K/GravityPipeline.kt:108-109). */
+int gs_set_field(gs_sim *sim, float ax_px, float ay_px);
+
+/* collisionSubsteps, coerced to 1..maximum (K/GravityPipeline.kt:147-151). */
+int gs_set_substeps(gs_sim *sim, int substeps);
+
+/* initializeAcceleration(activeBodies) (K/GravityPipeline.kt:366-370). */
+int gs_initialize_acceleration(gs_sim *sim, int active);
+
+/* step(activeBodies) (K/GravityPipeline.kt:445-494): one physics step. */
+int gs_step(gs_sim *sim, int active);
+
+/* Host upload of bodies first..first+count (uploadBodies,
+ * K/GravityPipeline.kt:733-784; the test hook CollisionGpuAccess.upload,
+ * T/CollisionGpuSupport.kt:51-66). materials may be NULL. */
+int gs_upload(gs_sim *sim, int first, int count, const float *xyvv, const float *materials);
+
+/* Host upload of accelerations (vec2 per body, normalised units). */
+int gs_upload_accel(gs_sim *sim, int first, int count, const float *accel);
+
+/* Blocking readback (the buffers the tests map, T/CollisionGpuSupport.kt:113-133). */
+#define GS_BODIES_XYVV 0     /* float[4 * count] */
+#define GS_PRESSURES 1       /* float[count] */
+#define GS_ACCEL 2           /* float[2 * count] */
+#define GS_MATERIALS 3       /* float[4 * count] */
+#define GS_CONTACT_COUNTS 4  /* uint32[count] */
+#define GS_SORT_DEST 5       /* uint32[count], new index of each body after the last sort */
+#define GS_FORCE_MESH 6      /* float[2 * mesh nodes]: (Fx, Fy) per node after a force build */
+#define GS_CONTACTS 7        /* uint32[32 * count]: ascending contact slots, body-major */
+int gs_download(gs_sim *sim, int what, int first, int count, void *dst);
+
+/* stopBodies(activeBodies) (K/GravityPipeline.kt:612, S/stop_bodies.comp). */
+int gs_stop_bodies(gs_sim *sim, int active);
+
+/* wake() (K/GravityPipeline.kt:610). */
+int gs_wake(gs_sim *sim);
+
+/* Test hooks mirroring the reflection hooks of T/CollisionGpuSupport.kt:68-90:
+ * resolveCollisions(count, kickDt, driftDt, writePressure, substep) (one
+ * pass; substep < 0 is the final pass), sortBodies(count), buildForce +
+ * applyKick of initializeAcceleration. */
+int gs_debug_resolve_collisions(gs_sim *sim, int count, float kick_dt, float drift_dt,
+                                int write_pressure, int substep);
+int gs_debug_sort(gs_sim *sim, int count);
+int gs_debug_resolve_liquid(gs_sim *sim, int count);
+
+/* Waits for the handle's stream. */
+int gs_synchronize(gs_sim *sim);
+
+/* Capacity and mesh of a handle: out[0] capacity, out[1..2] mesh w, h,
+ * out[3..4] physical grid w, h, out[5] contact slots. */
+int gs_query(gs_sim *sim, int *out);
+
+/* Per-kernel timing of `steps` eager steps with HIP events on the handle's
+ * stream (the reference's GpuProfiler stages, K/GpuTimer.kt:17-40). */
+typedef struct gs_kernel_stat {
+    char name[40];
+    int launches;
+    double total_ms;
+} gs_kernel_stat;
+int gs_profile_steps(gs_sim *sim, int steps, gs_kernel_stat *out, int capacity, int *written);
+
+/* Steps through one captured HIP graph per step (default on); 0 launches
+ * every kernel eagerly. */
+int gs_set_graphs(gs_sim *sim, int enabled);
+
+/* Message of the last failure on this thread, or "" (Kotlin exception text). */
+const char *gs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
