@@ -5,7 +5,8 @@
 // Here a pass is two launches:
 //   count : clears the previous pass's grid cells, walks the 3x3 stencil of
 //           this pass's grid, writes the contact count and the ascending list
-//           of the CONTACT_SLOTS smallest contact indices;
+//           of the CONTACT_SLOTS smallest contact indices (kept sorted in
+//           registers while walking);
 //   solve : Jacobi correction in ascending contact order (a body with more
 //           contacts continues past its slots by an ordered grid walk), walls,
 //           integration, then, fused, the next pass's drift and its insertion
@@ -13,6 +14,7 @@
 // The broad phase is a linked-cell grid of cells at least diameter + guard
 // wide, rebuilt every pass; heads of a pass are cleared by the next pass's
 // count kernel, so no clearing launch is needed. Two grid buffers alternate.
+// Bodies are float4 (x, y, vx, vy): a neighbour is one 16-byte load.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -50,80 +52,79 @@ __device__ __forceinline__ float pair_distance_squared(int id, float px, float p
     return dx * dx + dy * dy;
 }
 
-// Stencil of one axis: 3 cells around c (wrapped on the torus, cut at walls),
-// or every cell of a grid narrower than 3.
-struct AxisRange { int first, count; };
-__device__ __forceinline__ AxisRange axis_range(int c, int size) {
-    if (size < 3) return {0, size};
-    return {c - 1, 3};
+// The heads of the 3x3 stencil around the cell of (px, py), loaded together
+// (independent loads); cells cut by a wall, or repeated on a grid narrower
+// than 3, read as empty.
+__device__ __forceinline__ void stencil_heads(const SimDesc& s, float px, float py,
+                                              const GridBuf& grid, uint32_t (&head)[9]) {
+    int cx = floor_index(px * (float)s.cgw, s.cgw);
+    int cy = floor_index(py * (float)s.cgh, s.cgh);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        int y = s.cgh < 3 ? j : cy + j - 1;
+        bool oky = s.cgh < 3 ? j < s.cgh : (!s.bounded || (y >= 0 && y < s.cgh));
+        if (!s.bounded) y = wrap_axis(y, s.cgh);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            int x = s.cgw < 3 ? i : cx + i - 1;
+            bool okx = s.cgw < 3 ? i < s.cgw : (!s.bounded || (x >= 0 && x < s.cgw));
+            if (!s.bounded) x = wrap_axis(x, s.cgw);
+            head[j * 3 + i] = oky && okx ? grid.heads[s.cellBase + x + y * s.cgw] : 0u;
+        }
+    }
 }
 
 template <typename F>
 __device__ __forceinline__ void walk_stencil(const SimDesc& s, float px, float py,
                                              const GridBuf& grid, int guardLimit, F&& visit) {
-    int cx = floor_index(px * (float)s.cgw, s.cgw);
-    int cy = floor_index(py * (float)s.cgh, s.cgh);
-    AxisRange ax = axis_range(cx, s.cgw), ay = axis_range(cy, s.cgh);
+    uint32_t head[9];
+    stencil_heads(s, px, py, grid, head);
     int guard = 0;
-    for (int j = 0; j < ay.count; j++) {
-        int y = ay.first + j;
-        if (s.bounded) { if (y < 0 || y >= s.cgh) continue; }
-        else y = wrap_axis(y, s.cgh);
-        for (int i = 0; i < ax.count; i++) {
-            int x = ax.first + i;
-            if (s.bounded) { if (x < 0 || x >= s.cgw) continue; }
-            else x = wrap_axis(x, s.cgw);
-            uint32_t e = grid.heads[s.cellBase + x + y * s.cgw];
-            while (e != 0u && guard < guardLimit) {
-                int other = (int)e - 1;
-                float4 nd = grid.nodes[other];
-                e = __float_as_uint(nd.z);
-                guard++;
-                visit(other, nd.x, nd.y);
-            }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        uint32_t e = head[k];
+        while (e != 0u && guard < guardLimit) {
+            int other = (int)e - 1;
+            float4 nd = grid.nodes[other];
+            e = __float_as_uint(nd.z);
+            guard++;
+            visit(other, nd.x, nd.y);
         }
     }
-}
-
-__device__ __forceinline__ bool is_hard(const SimDesc& s, const float4* mats, int a, int b) {
-    if (!s.liquid) return true;
-    return mats[a].z != 0.0f || mats[b].z != 0.0f;
 }
 
 // ---------------------------------------------------------------------------
 // Drift (S/collision_cache_drift.comp:79-91) and insertion into the grid.
 __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__ sims,
                                                      const BodyBlock* __restrict__ blocks,
-                                                     float2* pos, float2* vel,
+                                                     float4* __restrict__ body,
                                                      const float2* __restrict__ acc,
                                                      float kickDt, float driftDt, int insert,
                                                      GridBuf grid) {
-    BodyBlock b = blocks[blockIdx.x];
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
-    float2 p = pos[g];
+    float4 v = body[g];
     if (kickDt != 0.0f || driftDt != 0.0f) {
-        float2 v = vel[g];
         if (kickDt != 0.0f) {
             float2 a = acc[g];
             float h = 0.5f * kickDt;
-            v.x += a.x * h;
-            v.y += a.y * h;
-            vel[g] = v;
+            v.z += a.x * h;
+            v.w += a.y * h;
         }
-        p.x += v.x * driftDt;
-        p.y += v.y * driftDt;
+        v.x += v.z * driftDt;
+        v.y += v.w * driftDt;
         if (s.bounded) {
-            p.x = clampf(p.x, 0.0f, MAXIMUM_UNIT);
-            p.y = clampf(p.y, 0.0f, MAXIMUM_UNIT);
+            v.x = clampf(v.x, 0.0f, MAXIMUM_UNIT);
+            v.y = clampf(v.y, 0.0f, MAXIMUM_UNIT);
         } else {
-            p.x = fractf(p.x);
-            p.y = fractf(p.y);
+            v.x = fractf(v.x);
+            v.y = fractf(v.y);
         }
-        pos[g] = p;
+        body[g] = v;
     }
-    if (insert) grid_insert(s, g, p.x, p.y, grid);
+    if (insert) grid_insert(s, g, v.x, v.y, grid);
 }
 
 // ---------------------------------------------------------------------------
@@ -136,37 +137,37 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
                                               uint32_t* __restrict__ counts,
                                               uint32_t* __restrict__ contacts, int planeStride,
                                               int guardLimit) {
-    __shared__ uint32_t slots[CONTACT_SLOTS][WG];
-    BodyBlock b = blocks[blockIdx.x];
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
-    int t = (int)threadIdx.x;
-    int g = b.first + t;
+    int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
     if (clearPrev) prev.heads[prev.cellOf[g]] = 0u;
     float4 own = grid.nodes[g];
     float px = own.x, py = own.y;
     bool ownFluid = s.liquid && mats[g].z == 0.0f;
+    uint32_t slot[CONTACT_SLOTS];
+#pragma unroll
+    for (int k = 0; k < CONTACT_SLOTS; k++) slot[k] = 0xffffffffu;
     uint32_t n = 0;
     walk_stencil(s, px, py, grid, guardLimit, [&](int other, float ox, float oy) {
         if (other == g) return;
         if (ownFluid && mats[other].z == 0.0f) return;
         float d2 = pair_distance_squared(g, px, py, other, ox, oy, s);
         if (!(d2 < s.diameter2)) return;
-        // Keep the CONTACT_SLOTS smallest indices, ascending.
-        uint32_t u = (uint32_t)other;
-        int k = n < (uint32_t)CONTACT_SLOTS ? (int)n : CONTACT_SLOTS;
-        if (k == CONTACT_SLOTS && u > slots[CONTACT_SLOTS - 1][t]) { n++; return; }
-        int at = k == CONTACT_SLOTS ? CONTACT_SLOTS - 1 : k;
-        while (at > 0 && slots[at - 1][t] > u) {
-            slots[at][t] = slots[at - 1][t];
-            at--;
-        }
-        slots[at][t] = u;
         n++;
+        // Insertion keeping the CONTACT_SLOTS smallest indices, ascending.
+        uint32_t u = (uint32_t)other;
+#pragma unroll
+        for (int k = 0; k < CONTACT_SLOTS; k++) {
+            uint32_t lo = min(u, slot[k]);
+            u = max(u, slot[k]);
+            slot[k] = lo;
+        }
     });
     counts[g] = n;
-    int k = n < (uint32_t)CONTACT_SLOTS ? (int)n : CONTACT_SLOTS;
-    for (int i = 0; i < k; i++) contacts[(size_t)i * planeStride + g] = slots[i][t];
+#pragma unroll
+    for (int k = 0; k < CONTACT_SLOTS; k++)
+        if ((uint32_t)k < n) contacts[(size_t)k * planeStride + g] = slot[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -175,6 +176,17 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
 struct Accum {
     float pcx, pcy, vcx, vcy, pressure;
 };
+
+// 1/k for small contact counts, each the correctly rounded quotient (the
+// same float the division gives), read instead of dividing.
+constexpr int RECIP_TABLE = 64;
+struct RecipTable {
+    float v[RECIP_TABLE];
+    constexpr RecipTable() : v() {
+        for (int k = 0; k < RECIP_TABLE; k++) v[k] = k ? 1.0f / (float)k : 1.0f;
+    }
+};
+__constant__ RecipTable c_recip = RecipTable();
 
 // S/collision_material.glsl:128-135
 __device__ __forceinline__ void friction_correction(float rvx, float rvy, float nx, float ny,
@@ -220,20 +232,16 @@ __device__ __forceinline__ void wall_contact(const SimDesc& s, float distance, f
     }
 }
 
-__device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int other,
-                                                   const float2* __restrict__ posIn,
-                                                   const float2* __restrict__ velIn,
-                                                   const float4* __restrict__ mats,
-                                                   const uint32_t* __restrict__ counts,
-                                                   const float2* __restrict__ fallback,
-                                                   float ppx, float ppy, float vpx, float vpy,
-                                                   float4 ownMat, uint32_t ownContacts,
-                                                   int writePressure, Accum& acc) {
-    float4 om = mats[other];
+// One contact of the Jacobi sum, with the other body's data already loaded.
+__device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int other, float4 ob,
+                                                   float4 om, uint32_t oc,
+                                                   const float2* __restrict__ fallback, float ppx,
+                                                   float ppy, float vpx, float vpy, float4 ownMat,
+                                                   uint32_t ownContacts, int writePressure,
+                                                   Accum& acc) {
     if (s.liquid && !(ownMat.z != 0.0f || om.z != 0.0f)) return;
-    float2 op = posIn[other];
-    float dx = ppx - op.x * s.vw;
-    float dy = ppy - op.y * s.vh;
+    float dx = ppx - ob.x * s.vw;
+    float dy = ppy - ob.y * s.vh;
     if (!s.bounded) {
         dx -= floorf(dx / s.vw + 0.5f) * s.vw;
         dy -= floorf(dy / s.vh + 0.5f) * s.vh;
@@ -255,12 +263,11 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
         ny = id < other ? c.y : -c.y;
     }
     float penetration = s.diameter - distance;
-    uint32_t oc = counts[other];
     uint32_t crowd = ownContacts > oc ? ownContacts : oc;
-    float pairWeight = 1.0f / (float)(crowd > 1u ? crowd : 1u);
+    uint32_t weightCount = crowd > 1u ? crowd : 1u;
+    float pairWeight = weightCount < RECIP_TABLE ? c_recip.v[weightCount] : 1.0f / (float)weightCount;
     float share = om.x / (ownMat.x + om.x);
-    float2 ov = velIn[other];
-    float ovx = ov.x * s.vw, ovy = ov.y * s.vh;
+    float ovx = ob.z * s.vw, ovy = ob.w * s.vh;
     float rvx = vpx - ovx, rvy = vpy - ovy;
     float rvn = rvx * nx + rvy * ny;
     float push = share * penetration * pairWeight;
@@ -288,12 +295,12 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
     }
 }
 
+constexpr int SOLVE_BATCH = 8;  // contacts whose loads are issued together
+
 __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               const BodyBlock* __restrict__ blocks,
-                                              const float2* __restrict__ posIn,
-                                              const float2* __restrict__ velIn,
-                                              float2* __restrict__ posOut,
-                                              float2* __restrict__ velOut,
+                                              const float4* __restrict__ bodyIn,
+                                              float4* __restrict__ bodyOut,
                                               const float4* __restrict__ mats,
                                               const uint32_t* __restrict__ counts,
                                               const uint32_t* __restrict__ contacts,
@@ -301,23 +308,43 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               const float2* __restrict__ fallback,
                                               float* __restrict__ pressures, int writePressure,
                                               float nextDriftDt, int insertNext, GridBuf next) {
-    BodyBlock b = blocks[blockIdx.x];
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
-    float2 p = posIn[g];
-    float2 v = velIn[g];
+    float4 me = bodyIn[g];
     float4 ownMat = mats[g];
-    float ppx = p.x * s.vw, ppy = p.y * s.vh;
-    float vpx = v.x * s.vw, vpy = v.y * s.vh;
+    float ppx = me.x * s.vw, ppy = me.y * s.vh;
+    float vpx = me.z * s.vw, vpy = me.w * s.vh;
     Accum acc{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t own = counts[g];
     int listed = own < (uint32_t)CONTACT_SLOTS ? (int)own : CONTACT_SLOTS;
     uint32_t last = 0u;
-    for (int i = 0; i < listed; i++) {
-        last = contacts[(size_t)i * planeStride + g];
-        accumulate_contact(s, g, (int)last, posIn, velIn, mats, counts, fallback, ppx, ppy, vpx,
-                           vpy, ownMat, own, writePressure, acc);
+    // All slot indices first, then the contacts' data a batch at a time, so
+    // the gathers of a batch are in flight together.
+    uint32_t id[CONTACT_SLOTS];
+#pragma unroll
+    for (int k = 0; k < CONTACT_SLOTS; k++)
+        id[k] = k < listed ? contacts[(size_t)k * planeStride + g] : (uint32_t)g;
+#pragma unroll
+    for (int i0 = 0; i0 < CONTACT_SLOTS; i0 += SOLVE_BATCH) {
+        if (i0 >= listed) break;
+        float4 ob[SOLVE_BATCH], om[SOLVE_BATCH];
+        uint32_t oc[SOLVE_BATCH];
+#pragma unroll
+        for (int q = 0; q < SOLVE_BATCH; q++) {
+            ob[q] = bodyIn[id[i0 + q]];
+            om[q] = mats[id[i0 + q]];
+            oc[q] = counts[id[i0 + q]];
+        }
+#pragma unroll
+        for (int q = 0; q < SOLVE_BATCH; q++) {
+            if (i0 + q < listed) {
+                last = id[i0 + q];
+                accumulate_contact(s, g, (int)last, ob[q], om[q], oc[q], fallback, ppx, ppy, vpx,
+                                   vpy, ownMat, own, writePressure, acc);
+            }
+        }
     }
     if (own > (uint32_t)CONTACT_SLOTS) {
         // More contacts than slots: take the rest in ascending index, one
@@ -325,17 +352,17 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
         bool ownFluid = s.liquid && ownMat.z == 0.0f;
         for (uint32_t rest = own - (uint32_t)CONTACT_SLOTS; rest > 0u; rest--) {
             uint32_t best = 0xffffffffu;
-            walk_stencil(s, p.x, p.y, grid, guardLimit, [&](int other, float ox, float oy) {
+            walk_stencil(s, me.x, me.y, grid, guardLimit, [&](int other, float ox, float oy) {
                 uint32_t u = (uint32_t)other;
                 if (other == g || u <= last || u >= best) return;
                 if (ownFluid && mats[other].z == 0.0f) return;
-                float d2 = pair_distance_squared(g, p.x, p.y, other, ox, oy, s);
+                float d2 = pair_distance_squared(g, me.x, me.y, other, ox, oy, s);
                 if (d2 < s.diameter2) best = u;
             });
             if (best == 0xffffffffu) break;
             last = best;
-            accumulate_contact(s, g, (int)best, posIn, velIn, mats, counts, fallback, ppx, ppy, vpx,
-                               vpy, ownMat, own, writePressure, acc);
+            accumulate_contact(s, g, (int)best, bodyIn[best], mats[best], counts[best], fallback,
+                               ppx, ppy, vpx, vpy, ownMat, own, writePressure, acc);
         }
     }
     if (s.bounded) {
@@ -350,35 +377,47 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     ppy += acc.pcy;
     vpx += acc.vcx + acc.pcx / s.subDt;
     vpy += acc.vcy + acc.pcy / s.subDt;
-    float2 op, ov;
+    float4 o;
     if (s.bounded) {
-        op.x = clampf(ppx / s.vw, 0.0f, MAXIMUM_UNIT);
-        op.y = clampf(ppy / s.vh, 0.0f, MAXIMUM_UNIT);
+        o.x = clampf(ppx / s.vw, 0.0f, MAXIMUM_UNIT);
+        o.y = clampf(ppy / s.vh, 0.0f, MAXIMUM_UNIT);
     } else {
         ppx = glsl_mod(ppx, s.vw);
         ppy = glsl_mod(ppy, s.vh);
-        op.x = ppx / s.vw;
-        op.y = ppy / s.vh;
+        o.x = ppx / s.vw;
+        o.y = ppy / s.vh;
     }
-    ov.x = vpx / s.vw;
-    ov.y = vpy / s.vh;
+    o.z = vpx / s.vw;
+    o.w = vpy / s.vh;
     if (writePressure && !(s.liquid && ownMat.z == 0.0f)) pressures[g] = acc.pressure;
     // The next pass's drift (S/collision_cache_drift.comp:83-91; only the
     // first pass of a step kicks, so there is none here).
     if (nextDriftDt != 0.0f) {
-        op.x += ov.x * nextDriftDt;
-        op.y += ov.y * nextDriftDt;
+        o.x += o.z * nextDriftDt;
+        o.y += o.w * nextDriftDt;
         if (s.bounded) {
-            op.x = clampf(op.x, 0.0f, MAXIMUM_UNIT);
-            op.y = clampf(op.y, 0.0f, MAXIMUM_UNIT);
+            o.x = clampf(o.x, 0.0f, MAXIMUM_UNIT);
+            o.y = clampf(o.y, 0.0f, MAXIMUM_UNIT);
         } else {
-            op.x = fractf(op.x);
-            op.y = fractf(op.y);
+            o.x = fractf(o.x);
+            o.y = fractf(o.y);
         }
     }
-    posOut[g] = op;
-    velOut[g] = ov;
-    if (insertNext) grid_insert(s, g, op.x, op.y, next);
+    bodyOut[g] = o;
+    if (insertNext) grid_insert(s, g, o.x, o.y, next);
+}
+
+// S/stop_bodies.comp:9-15: zero the velocity of every active body in both
+// ping-pong buffers.
+__global__ void __launch_bounds__(WG) k_stop(const SimDesc* __restrict__ sims,
+                                             const BodyBlock* __restrict__ blocks,
+                                             float4* __restrict__ a, float4* __restrict__ b) {
+    BodyBlock k = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[k.sim];
+    int g = k.first + (int)threadIdx.x;
+    if (g - s.base >= s.n) return;
+    a[g].z = 0.0f; a[g].w = 0.0f;
+    b[g].z = 0.0f; b[g].w = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -386,12 +425,16 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
 
 static inline GridBuf grid_of(const CollideGrid& c) { return GridBuf{c.heads, c.nodes, c.cellOf}; }
 
-void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float2* pos, float2* vel,
-                         const float2* acc, float kickDt, float driftDt, int insert,
-                         const CollideGrid& grid) {
+void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_stop, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, a, b);
+}
+
+void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
+                         float kickDt, float driftDt, int insert, const CollideGrid& grid) {
     if (ls.bodyBlocks == 0) return;
     hipLaunchKernelGGL(k_drift_insert, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       pos, vel, acc, kickDt, driftDt, insert, grid_of(grid));
+                       body, acc, kickDt, driftDt, insert, grid_of(grid));
 }
 
 void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
@@ -403,16 +446,15 @@ void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid
                        ls.planeStride, ls.guardLimit);
 }
 
-void launch_solve(hipStream_t st, const LaunchShape& ls, const float2* posIn, const float2* velIn,
-                  float2* posOut, float2* velOut, const float4* mats, const uint32_t* counts,
-                  const uint32_t* contacts, const CollideGrid& grid, const float2* fallback,
-                  float* pressures, int writePressure, float nextDriftDt, int insertNext,
-                  const CollideGrid& next) {
+void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, float4* bodyOut,
+                  const float4* mats, const uint32_t* counts, const uint32_t* contacts,
+                  const CollideGrid& grid, const float2* fallback, float* pressures,
+                  int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, posIn,
-                       velIn, posOut, velOut, mats, counts, contacts, ls.planeStride,
-                       grid_of(grid), ls.guardLimit, fallback, pressures, writePressure,
-                       nextDriftDt, insertNext, grid_of(next));
+    hipLaunchKernelGGL(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, bodyIn,
+                       bodyOut, mats, counts, contacts, ls.planeStride, grid_of(grid),
+                       ls.guardLimit, fallback, pressures, writePressure, nextDriftDt, insertNext,
+                       grid_of(next));
 }
 
 }  // namespace gs

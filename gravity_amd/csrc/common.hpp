@@ -18,7 +18,8 @@
 namespace gs {
 
 constexpr int WG = 256;               // workgroup width of every body kernel
-constexpr int CONTACT_SLOTS = 32;     // contacts listed per body; more walk the grid
+constexpr int CONTACT_SLOTS = 16;     // contacts listed per body; more walk the grid
+constexpr int XCDS = 8;               // accelerator dies of an MI355X, each with its own L2
 constexpr float MAXIMUM_UNIT = 0.99999994f;  // largest float below 1
 
 // Per-simulation constants, mirrored from GravityConfig and the viewport
@@ -67,10 +68,45 @@ struct SimDesc {
     float fieldX, fieldY;  // uniform field, normalised units
     float massScale;    // fixed-point deposit scale, as float
     int pad1;
+    // Reciprocals of the kick (FP32-tolerance path; see k_kick).
+    float invAspect;    // 1 / aspect
+    float invTwoSigma2; // 1 / (2 sigma^2)
+    float invPw, invPh; // 1 / pw, 1 / ph
+    float gOverM;       // G / totalMass
+    int pad2, pad3, pad4;
 };
 
 // Global work tables (built on the host whenever the batch shape changes).
 struct BodyBlock { int sim; int first; };
+
+// Workgroups are dealt to the XCDs round robin (workgroup i runs on XCD
+// i % 8). Body blocks are spatially sorted, so hand each XCD runs of
+// XCD_CHUNK consecutive blocks: neighbour gathers within a run hit that
+// XCD's own L2, while runs from all over the domain keep dense and sparse
+// regions spread over every XCD. Blocks past the last full round keep their
+// index (the map is a bijection of [0, n)).
+constexpr int XCD_CHUNK = 8;
+__device__ __forceinline__ int xcd_block(int i, int n) {
+    const int round = XCDS * XCD_CHUNK;
+    if (i >= n / round * round) return i;
+    int xcd = i % XCDS, local = i / XCDS;
+    return ((local / XCD_CHUNK) * XCDS + xcd) * XCD_CHUNK + local % XCD_CHUNK;
+}
+
+// Every active body of a launch over nblocks body blocks, as f(sim, g).
+// The grid may be smaller than nblocks (a multiple of 8, so a workgroup
+// keeps its XCD): each workgroup then strides over several blocks.
+template <typename F>
+__device__ __forceinline__ void for_each_body(const SimDesc* __restrict__ sims,
+                                              const BodyBlock* __restrict__ blocks, int nblocks,
+                                              F&& f) {
+    for (int lb = (int)blockIdx.x; lb < nblocks; lb += (int)gridDim.x) {
+        BodyBlock b = blocks[xcd_block(lb, nblocks)];
+        const SimDesc& s = sims[b.sim];
+        int g = b.first + (int)threadIdx.x;
+        if (g - s.base < s.n) f(s, g);
+    }
+}
 
 __device__ __forceinline__ float clampf(float v, float lo, float hi) {
     return fminf(fmaxf(v, lo), hi);

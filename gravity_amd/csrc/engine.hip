@@ -134,8 +134,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "stream");
     size_t S = (size_t)totalSlots_;
     for (int b = 0; b < 2; b++) {
-        pos_[b] = dalloc<float2>(S);
-        vel_[b] = dalloc<float2>(S);
+        body_[b] = dalloc<float4>(S);
         acc_[b] = dalloc<float2>(S);
         mat_[b] = dalloc<float4>(S);
         grid_[b].heads = dalloc<uint32_t>((size_t)totalCells_);
@@ -206,7 +205,7 @@ Engine::~Engine() {
         hipEventDestroy(e.second.first);
         hipEventDestroy(e.second.second);
     }
-    void* ptrs[] = {pos_[0], pos_[1], vel_[0], vel_[1], acc_[0], acc_[1], mat_[0], mat_[1],
+    void* ptrs[] = {body_[0], body_[1], acc_[0], acc_[1], mat_[0], mat_[1],
                     grid_[0].heads, grid_[1].heads, grid_[0].nodes, grid_[1].nodes,
                     grid_[0].cellOf, grid_[1].cellOf, press_, counts_, contacts_, tileCount_,
                     tilePrefix_, keyOf_, slotSrc_, dest_, scanTmp_, cellCount_, cellPrefix_,
@@ -269,6 +268,11 @@ void Engine::fill_desc(HostSim& h) {
     d.invTotalMass = 1.0f / d.totalMass;
     d.fieldX = h.fieldX / d.vw;
     d.fieldY = h.fieldY / d.vh;
+    d.invAspect = (float)(1.0 / (double)d.aspect);
+    d.invTwoSigma2 = (float)(1.0 / (2.0 * (double)d.sigma * (double)d.sigma));
+    d.invPw = 1.0f / (float)d.pw;
+    d.invPh = 1.0f / (float)d.ph;
+    d.gOverM = (float)((double)d.G / (double)d.totalMass);
 }
 
 void Engine::push_descs() {
@@ -433,15 +437,9 @@ void Engine::upload(int s, int first, int count, const float* xyvv, const float*
     if (first < 0 || count < 0 || first > h.cap || count > h.cap - first)
         throw ArgumentError("upload range exceeds capacity");
     if (count == 0) return;
-    std::vector<float2> p((size_t)count), v((size_t)count);
-    for (int i = 0; i < count; i++) {
-        p[i] = make_float2(xyvv[4 * i], xyvv[4 * i + 1]);
-        v[i] = make_float2(xyvv[4 * i + 2], xyvv[4 * i + 3]);
-    }
     // Both ping-pong buffers, as uploadBodies writes bodies and collisionBodies.
     for (int b = 0; b < 2; b++) {
-        check(hipMemcpyAsync(pos_[b] + h.base + first, p.data(), count * 8, hipMemcpyHostToDevice, st_), "upload");
-        check(hipMemcpyAsync(vel_[b] + h.base + first, v.data(), count * 8, hipMemcpyHostToDevice, st_), "upload");
+        check(hipMemcpyAsync(body_[b] + h.base + first, xyvv, (size_t)count * 16, hipMemcpyHostToDevice, st_), "upload");
         if (mats)
             check(hipMemcpyAsync(mat_[b] + h.base + first, mats, (size_t)count * 16, hipMemcpyHostToDevice, st_), "upload");
     }
@@ -470,17 +468,9 @@ void Engine::download(int s, int what, int first, int count, void* dst) {
     if (count == 0) return;
     size_t o = (size_t)h.base + first;
     switch (what) {
-        case GS_BODIES_XYVV: {
-            std::vector<float2> p((size_t)count), v((size_t)count);
-            check(hipMemcpy(p.data(), pos_[bodyPar_] + o, count * 8, hipMemcpyDeviceToHost), "download");
-            check(hipMemcpy(v.data(), vel_[bodyPar_] + o, count * 8, hipMemcpyDeviceToHost), "download");
-            float* out = (float*)dst;
-            for (int i = 0; i < count; i++) {
-                out[4 * i] = p[i].x; out[4 * i + 1] = p[i].y;
-                out[4 * i + 2] = v[i].x; out[4 * i + 3] = v[i].y;
-            }
+        case GS_BODIES_XYVV:
+            check(hipMemcpy(dst, body_[bodyPar_] + o, (size_t)count * 16, hipMemcpyDeviceToHost), "download");
             return;
-        }
         case GS_PRESSURES:
             check(hipMemcpy(dst, press_ + o, (size_t)count * 4, hipMemcpyDeviceToHost), "download");
             return;
@@ -531,12 +521,12 @@ void Engine::timed(const char* name, F&& f) {
 void Engine::enqueue_sort() {
     // K/GravityPipeline.kt:1269-1319
     int o = bodyPar_ ^ 1, a = auxPar_ ^ 1;
-    timed("sort_count", [&] { launch_sort_count(st_, shape_, pos_[bodyPar_], tileCount_, keyOf_); });
+    timed("sort_count", [&] { launch_sort_count(st_, shape_, body_[bodyPar_], tileCount_, keyOf_); });
     timed("sort_scan", [&] { launch_scan(st_, tileCount_, tilePrefix_, totalTiles_, scanTmp_); });
     timed("sort_scatter", [&] { launch_sort_scatter(st_, shape_, keyOf_, tileCount_, tilePrefix_, slotSrc_); });
     timed("sort_canon", [&] {
-        launch_sort_canon(st_, shape_, keyOf_, tilePrefix_, slotSrc_, pos_[bodyPar_], vel_[bodyPar_],
-                          acc_[auxPar_], mat_[auxPar_], pos_[o], vel_[o], acc_[a], mat_[a], dest_);
+        launch_sort_canon(st_, shape_, keyOf_, tilePrefix_, slotSrc_, body_[bodyPar_],
+                          acc_[auxPar_], mat_[auxPar_], body_[o], acc_[a], mat_[a], dest_);
     });
     bodyPar_ = o;
     auxPar_ = a;
@@ -545,11 +535,11 @@ void Engine::enqueue_sort() {
 void Engine::enqueue_force_and_kick(bool initial) {
     // K/GravityPipeline.kt:907-973 then :1213-1261
     if (anyMutual_) {
-        timed("bin", [&] { launch_bin(st_, shape_, pos_[bodyPar_], cellCount_, homeOf_); });
+        timed("bin", [&] { launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_); });
         timed("cell_scan", [&] { launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_); });
         timed("cell_scatter", [&] { launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_); });
         timed("cell_canon", [&] {
-            launch_cell_canon(st_, gravitySlots_, homeOf_, cellPrefix_, slotBody_, pos_[bodyPar_],
+            launch_cell_canon(st_, gravitySlots_, homeOf_, cellPrefix_, slotBody_, body_[bodyPar_],
                               mat_[auxPar_], cellBody_, cellMass_);
         });
         timed("fft_rows_forward", [&] {
@@ -559,7 +549,7 @@ void Engine::enqueue_force_and_kick(bool initial) {
         timed("fft_rows_inverse", [&] { launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_); });
     }
     timed("kick", [&] {
-        launch_kick(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_], mat_[auxPar_], spec_,
+        launch_kick(st_, shape_, body_[bodyPar_], acc_[auxPar_], spec_,
                     cellPrefix_, cellMass_, cellBody_, initial ? 1 : 0);
     });
 }
@@ -568,13 +558,13 @@ void Engine::enqueue_force_and_kick(bool initial) {
 void Engine::enqueue_pass(float kickDt, float driftDt, bool writePressure) {
     int g = gridPar_;
     timed("drift_insert", [&] {
-        launch_drift_insert(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_], kickDt, driftDt, 1, grid_[g]);
+        launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], kickDt, driftDt, 1, grid_[g]);
     });
     timed("count", [&] {
         launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
     });
     timed("solve", [&] {
-        launch_solve(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], pos_[bodyPar_ ^ 1], vel_[bodyPar_ ^ 1],
+        launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
                      mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, writePressure ? 1 : 0,
                      0.0f, 0, grid_[g ^ 1]);
     });
@@ -591,7 +581,7 @@ void Engine::enqueue_step(bool sortDue) {
     const int S = h0.substeps;
     if (anyCollide_) {
         timed("drift_insert", [&] {
-            launch_drift_insert(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_], dt, sub, 1, grid_[gridPar_]);
+            launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], dt, sub, 1, grid_[gridPar_]);
         });
         for (int k = 0; k < S; k++) {
             int g = gridPar_;
@@ -602,7 +592,7 @@ void Engine::enqueue_step(bool sortDue) {
             float nextDrift = last ? 0.0f : sub;
             int insertNext = last ? (anyLiquid_ ? 0 : 1) : 1;
             timed("solve", [&] {
-                launch_solve(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], pos_[bodyPar_ ^ 1], vel_[bodyPar_ ^ 1],
+                launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
                              mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, 0, nextDrift,
                              insertNext, grid_[g ^ 1]);
             });
@@ -611,7 +601,7 @@ void Engine::enqueue_step(bool sortDue) {
         }
         if (anyLiquid_) {
             timed("insert", [&] {
-                launch_drift_insert(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_], 0.0f, 0.0f, 1, grid_[gridPar_]);
+                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], 0.0f, 0.0f, 1, grid_[gridPar_]);
             });
         }
         enqueue_force_and_kick(false);
@@ -621,7 +611,7 @@ void Engine::enqueue_step(bool sortDue) {
             launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
         });
         timed("solve", [&] {
-            launch_solve(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], pos_[bodyPar_ ^ 1], vel_[bodyPar_ ^ 1],
+            launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
                          mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, 1, 0.0f, 0,
                          grid_[g ^ 1]);
         });
@@ -631,7 +621,7 @@ void Engine::enqueue_step(bool sortDue) {
         // Gravity-only configurations: kick-drift-kick with plain drifts.
         for (int k = 0; k < S; k++) {
             timed("drift", [&] {
-                launch_drift_insert(st_, shape_, pos_[bodyPar_], vel_[bodyPar_], acc_[auxPar_],
+                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_],
                                     k == 0 ? dt : 0.0f, sub, 0, grid_[gridPar_]);
             });
         }
@@ -684,8 +674,8 @@ void Engine::step() {
 void Engine::stop_bodies(int s) {
     // S/stop_bodies.comp: zero velocities in both ping-pong buffers.
     HostSim& h = sims_.at(s);
-    for (int b = 0; b < 2; b++)
-        check(hipMemsetAsync(vel_[b] + h.base, 0, (size_t)h.n * 8, st_), "stop");
+    launch_stop(st_, shape_, body_[0], body_[1]);
+    (void)h;
 }
 
 void Engine::debug_collide(float kickDt, float driftDt, bool writePressure) {

@@ -93,9 +93,8 @@ private:
     int twLen_ = 2048;
     bool useGraphs_ = true;
 
-    // bodies (ping-pong)
-    float2* pos_[2] = {nullptr, nullptr};
-    float2* vel_[2] = {nullptr, nullptr};
+    // bodies (ping-pong): float4 (x, y, vx, vy) in the reference's units
+    float4* body_[2] = {nullptr, nullptr};
     float2* acc_[2] = {nullptr, nullptr};
     float4* mat_[2] = {nullptr, nullptr};
     int bodyPar_ = 0, auxPar_ = 0, gridPar_ = 0;

@@ -13,18 +13,22 @@
 //   rows inverse : per needed mesh row, inverse transform along x. Because
 //                  the x and y force kernels are real, one complex inverse
 //                  yields Fx in the real part and Fy in the imaginary part.
-// Transforms are Stockham autosort in LDS with radix-8 butterflies (radix 4
-// or 2 for the last stage) and twiddles from an FP64-rounded table. Under
-// walls the mesh doubles; rows and columns that hold only zero padding are
-// neither loaded nor transformed in the forward row pass, and only rows the
-// force gather reads are produced by the inverse passes.
+// Transforms are Stockham autosort in LDS: one lane owns one radix-8
+// butterfly of a stage (radix 4 or 2 for the last stage, a few butterflies a
+// lane), so a lane holds 8 complex values and many waves share a CU.
+// Twiddles come from an FP64-rounded table. Under walls the mesh doubles;
+// rows that hold only zero padding are neither loaded nor transformed in
+// the forward row pass, and only rows the force gather reads are produced by
+// the inverse passes.
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace gs {
 
 constexpr int ROW_LDS = 4096;   // complex values per row workgroup
+constexpr int ROW_NT = 512;     // lanes per row workgroup
 constexpr int COL_LDS = 8192;   // complex values per column workgroup
+constexpr int COL_NT = 1024;    // lanes per column workgroup
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -73,10 +77,11 @@ __device__ __forceinline__ void dft8(float2* v) {
 }
 
 // One Stockham stage of radix R on nl lines of length L held in LDS;
-// element k of line l lives at buf[l * ls + k * es]. Q butterflies a lane.
-// jFast: consecutive lanes take consecutive butterflies of one line (rows);
+// element k of line l lives at buf[l * ls + k * es]. With NT lanes and at
+// most E values in LDS a lane takes Q = E / (R NT) butterflies. jFast:
+// consecutive lanes take consecutive butterflies of one line (rows);
 // otherwise the same butterfly of consecutive lines (columns).
-template <int DIR, int R, int Q>
+template <int DIR, int R, int Q, int NT>
 __device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int ls, int es, int Ns,
                                                bool jFast, const float2* __restrict__ tw,
                                                int twLen) {
@@ -85,7 +90,7 @@ __device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int l
     float2 v[Q][R];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        int bi = (int)threadIdx.x + q * WG;
+        int bi = (int)threadIdx.x + q * NT;
         if (bi < total) {
             int line = jFast ? bi / perLine : bi % nl, j = jFast ? bi % perLine : bi / nl;
 #pragma unroll
@@ -96,7 +101,7 @@ __device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int l
     const int twStep = twLen / (Ns * R);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        int bi = (int)threadIdx.x + q * WG;
+        int bi = (int)threadIdx.x + q * NT;
         if (bi < total) {
             int line = jFast ? bi / perLine : bi % nl, j = jFast ? bi % perLine : bi / nl;
             int k = j & (Ns - 1);
@@ -119,9 +124,9 @@ __device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int l
     __syncthreads();
 }
 
-// Transforms nl lines (nl * L <= 8192). DIR = -1 forward, +1 inverse
+// Transforms nl lines (nl * L <= E). DIR = -1 forward, +1 inverse
 // (unscaled, like S/fft_shared.comp).
-template <int DIR>
+template <int DIR, int NT, int E>
 __device__ void lds_fft(float2* buf, int L, int nl, int ls, int es, const float2* __restrict__ tw,
                         int twLen) {
     const bool jFast = es == 1;
@@ -129,13 +134,13 @@ __device__ void lds_fft(float2* buf, int L, int nl, int ls, int es, const float2
     while (Ns < L) {
         int rem = L / Ns;
         if (rem >= 8) {
-            stockham_stage<DIR, 8, 4>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
+            stockham_stage<DIR, 8, (E / (8 * NT) > 0 ? E / (8 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
             Ns *= 8;
         } else if (rem == 4) {
-            stockham_stage<DIR, 4, 8>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
+            stockham_stage<DIR, 4, (E / (4 * NT) > 0 ? E / (4 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
             Ns *= 4;
         } else {
-            stockham_stage<DIR, 2, 16>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
+            stockham_stage<DIR, 2, (E / (2 * NT) > 0 ? E / (2 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
             Ns *= 2;
         }
     }
@@ -149,18 +154,18 @@ __device__ __forceinline__ int mesh_row(const SimDesc& s, int a) {
 }
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(WG) k_fft_rows_forward(const SimDesc* __restrict__ sims,
-                                                         const LineWork* __restrict__ work,
-                                                         const uint32_t* __restrict__ cellPrefix,
-                                                         const float4* __restrict__ cellBody,
-                                                         float2* __restrict__ spec,
-                                                         const float2* __restrict__ tw, int twLen) {
+__global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __restrict__ sims,
+                                                             const LineWork* __restrict__ work,
+                                                             const uint32_t* __restrict__ cellPrefix,
+                                                             const float4* __restrict__ cellBody,
+                                                             float2* __restrict__ spec,
+                                                             const float2* __restrict__ tw, int twLen) {
     __shared__ float2 data[ROW_LDS];
     __shared__ uint32_t mass[ROW_LDS];
-    LineWork w = work[blockIdx.x];
+    LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
-    for (int i = (int)threadIdx.x; i < w.count * L; i += WG) mass[i] = 0u;
+    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) mass[i] = 0u;
     __syncthreads();
     const float scale = s.massScale;
     for (int l = 0; l < w.count; l++) {
@@ -179,7 +184,7 @@ __global__ void __launch_bounds__(WG) k_fft_rows_forward(const SimDesc* __restri
             int r = s.bounded || s.ph < 3 ? rr : wrap_axis(rr, s.ph);
             int c0 = s.physBase + r * s.pw;
             int t0 = (int)cellPrefix[c0], t1 = (int)cellPrefix[c0 + s.pw];
-            for (int t = t0 + (int)threadIdx.x; t < t1; t += WG) {
+            for (int t = t0 + (int)threadIdx.x; t < t1; t += ROW_NT) {
                 float4 bd = cellBody[t];
                 // S/build_grid.comp:53-69, round() half to even.
                 float gx = bd.x * (float)s.pw - 0.5f, gy = bd.y * (float)s.ph - 0.5f;
@@ -201,65 +206,65 @@ __global__ void __launch_bounds__(WG) k_fft_rows_forward(const SimDesc* __restri
         }
     }
     __syncthreads();
-    for (int i = (int)threadIdx.x; i < w.count * L; i += WG)
+    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT)
         data[i] = make_float2((float)mass[i] * s.invMassScale, 0.0f);
     __syncthreads();
-    lds_fft<-1>(data, L, w.count, L, 1, tw, twLen);
-    for (int i = (int)threadIdx.x; i < w.count * L; i += WG) {
+    lds_fft<-1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, tw, twLen);
+    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
         int l = i / L, x = i - l * L;
         int y = mesh_row(s, w.first + l);
         spec[s.meshBase + y * s.mw + x] = data[i];
     }
 }
 
-__global__ void __launch_bounds__(WG) k_fft_columns(const SimDesc* __restrict__ sims,
-                                                    const LineWork* __restrict__ work,
-                                                    float2* __restrict__ spec,
-                                                    const float2* __restrict__ green,
-                                                    const float2* __restrict__ tw, int twLen) {
+__global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restrict__ sims,
+                                                        const LineWork* __restrict__ work,
+                                                        float2* __restrict__ spec,
+                                                        const float2* __restrict__ green,
+                                                        const float2* __restrict__ tw, int twLen) {
     __shared__ float2 data[COL_LDS];
-    LineWork w = work[blockIdx.x];
+    LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int H = s.mh, CB = w.count, c0 = w.first;
     const float2* src = spec + s.meshBase;
-    for (int i = (int)threadIdx.x; i < CB * H; i += WG) {
+    for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
         int y = i / CB, c = i - y * CB;
         bool live = !s.bounded || y <= s.ph || y == H - 1;
         data[i] = live ? src[y * s.mw + c0 + c] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    lds_fft<-1>(data, H, CB, 1, CB, tw, twLen);
+    lds_fft<-1, COL_NT, COL_LDS>(data, H, CB, 1, CB, tw, twLen);
     const float2* G = green + s.meshBase;
-    for (int i = (int)threadIdx.x; i < CB * H; i += WG) {
+    for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
         int y = i / CB, c = i - y * CB;
         float2 z = cmul(data[i], G[y * s.mw + c0 + c]);
         data[i] = make_float2(z.x * s.invTotalMass, z.y * s.invTotalMass);
     }
     __syncthreads();
-    lds_fft<1>(data, H, CB, 1, CB, tw, twLen);
+    lds_fft<1, COL_NT, COL_LDS>(data, H, CB, 1, CB, tw, twLen);
     float2* dst = spec + s.meshBase;
-    for (int i = (int)threadIdx.x; i < CB * H; i += WG) {
+    for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
         int y = i / CB, c = i - y * CB;
         bool live = !s.bounded || y <= s.ph || y == H - 1;
         if (live) dst[y * s.mw + c0 + c] = data[i];
     }
 }
 
-__global__ void __launch_bounds__(WG) k_fft_rows_inverse(const SimDesc* __restrict__ sims,
-                                                         const LineWork* __restrict__ work,
-                                                         float2* __restrict__ spec,
-                                                         const float2* __restrict__ tw, int twLen) {
+__global__ void __launch_bounds__(ROW_NT) k_fft_rows_inverse(const SimDesc* __restrict__ sims,
+                                                             const LineWork* __restrict__ work,
+                                                             float2* __restrict__ spec,
+                                                             const float2* __restrict__ tw, int twLen) {
     __shared__ float2 data[ROW_LDS];
-    LineWork w = work[blockIdx.x];
+    LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
-    for (int i = (int)threadIdx.x; i < w.count * L; i += WG) {
+    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
         int l = i / L, x = i - l * L;
         data[i] = spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x];
     }
     __syncthreads();
-    lds_fft<1>(data, L, w.count, L, 1, tw, twLen);
-    for (int i = (int)threadIdx.x; i < w.count * L; i += WG) {
+    lds_fft<1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, tw, twLen);
+    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
         int l = i / L, x = i - l * L;
         spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = data[i];
     }
@@ -269,7 +274,7 @@ void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
                              const float2* twiddles, int twiddleLength) {
     if (items == 0) return;
-    hipLaunchKernelGGL(k_fft_rows_forward, dim3(items), dim3(WG), 0, st, sims, work, cellPrefix,
+    hipLaunchKernelGGL(k_fft_rows_forward, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
                        cellBody, spec, twiddles, twiddleLength);
 }
 
@@ -277,14 +282,14 @@ void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* wor
                         float2* spec, const float2* green, const float2* twiddles,
                         int twiddleLength) {
     if (items == 0) return;
-    hipLaunchKernelGGL(k_fft_columns, dim3(items), dim3(WG), 0, st, sims, work, spec, green,
+    hipLaunchKernelGGL(k_fft_columns, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green,
                        twiddles, twiddleLength);
 }
 
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              float2* spec, const float2* twiddles, int twiddleLength) {
     if (items == 0) return;
-    hipLaunchKernelGGL(k_fft_rows_inverse, dim3(items), dim3(WG), 0, st, sims, work, spec,
+    hipLaunchKernelGGL(k_fft_rows_inverse, dim3(items), dim3(ROW_NT), 0, st, sims, work, spec,
                        twiddles, twiddleLength);
 }
 

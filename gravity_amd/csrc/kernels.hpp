@@ -29,36 +29,34 @@ void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uin
 int scan_totals_size(int len);
 
 // ---- spatial sort ----
-void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float2* pos,
+void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float4* body,
                        uint32_t* tileCount, uint32_t* keyOf);
 void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
                          uint32_t* tileCount, const uint32_t* tilePrefix, uint32_t* slotSrc);
 void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
                        const uint32_t* tilePrefix, const uint32_t* slotSrc,
-                       const float2* posIn, const float2* velIn, const float2* accIn,
-                       const float4* matIn, float2* posOut, float2* velOut, float2* accOut,
-                       float4* matOut, uint32_t* dest);
+                       const float4* bodyIn, const float2* accIn, const float4* matIn,
+                       float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest);
 
 // ---- collisions ----
-void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float2* pos, float2* vel,
-                         const float2* acc, float kickDt, float driftDt, int insert,
-                         const CollideGrid& grid);
+void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b);
+void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
+                         float kickDt, float driftDt, int insert, const CollideGrid& grid);
 void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
                   const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
                   uint32_t* contacts);
-void launch_solve(hipStream_t st, const LaunchShape& ls, const float2* posIn, const float2* velIn,
-                  float2* posOut, float2* velOut, const float4* mats, const uint32_t* counts,
-                  const uint32_t* contacts, const CollideGrid& grid, const float2* fallback,
-                  float* pressures, int writePressure, float nextDriftDt, int insertNext,
-                  const CollideGrid& next);
+void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, float4* bodyOut,
+                  const float4* mats, const uint32_t* counts, const uint32_t* contacts,
+                  const CollideGrid& grid, const float2* fallback, float* pressures,
+                  int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next);
 
 // ---- P3M gravity ----
-void launch_bin(hipStream_t st, const LaunchShape& ls, const float2* pos, uint32_t* cellCount,
+void launch_bin(hipStream_t st, const LaunchShape& ls, const float4* body, uint32_t* cellCount,
                 uint32_t* homeOf);
 void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* homeOf,
                          uint32_t* cellCount, const uint32_t* cellPrefix, uint32_t* slotBody);
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
-                       const uint32_t* cellPrefix, const uint32_t* slotBody, const float2* pos,
+                       const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
                        const float4* mats, float4* cellBody, uint32_t* cellMass);
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
@@ -68,8 +66,8 @@ void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* wor
                         int twiddleLength);
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              float2* spec, const float2* twiddles, int twiddleLength);
-void launch_kick(hipStream_t st, const LaunchShape& ls, const float2* pos, float2* vel,
-                 float2* acc, const float4* mats, const float2* force, const uint32_t* cellPrefix,
-                 const uint32_t* cellMass, const float4* cellBody, int initial);
+void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
+                 const float2* force, const uint32_t* cellPrefix, const uint32_t* cellMass,
+                 const float4* cellBody, int initial);
 
 }  // namespace gs

@@ -23,13 +23,13 @@ __device__ __forceinline__ int home_cell(const SimDesc& s, float px, float py) {
 // S/build_grid.comp:71-74: the home cell of every body.
 __global__ void __launch_bounds__(WG) k_bin(const SimDesc* __restrict__ sims,
                                             const BodyBlock* __restrict__ blocks,
-                                            const float2* __restrict__ pos, uint32_t* cellCount,
+                                            const float4* __restrict__ body, uint32_t* cellCount,
                                             uint32_t* __restrict__ homeOf) {
-    BodyBlock b = blocks[blockIdx.x];
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n || !s.mutual) return;
-    float2 p = pos[g];
+    float4 p = body[g];
     int c = home_cell(s, p.x, p.y);
     homeOf[g] = (uint32_t)c;
     atomicAdd(&cellCount[c], 1u);
@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
                                                      uint32_t* cellCount,
                                                      const uint32_t* __restrict__ cellPrefix,
                                                      uint32_t* __restrict__ slotBody) {
-    BodyBlock b = blocks[blockIdx.x];
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n || !s.mutual) return;
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
 __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __restrict__ homeOf,
                                                    const uint32_t* __restrict__ cellPrefix,
                                                    const uint32_t* __restrict__ slotBody,
-                                                   const float2* __restrict__ pos,
+                                                   const float4* __restrict__ body,
                                                    const float4* __restrict__ mats,
                                                    float4* __restrict__ cellBody,
                                                    uint32_t* __restrict__ cellMass) {
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
     int s0 = (int)cellPrefix[c], s1 = (int)cellPrefix[c + 1];
     int rank = 0;
     for (int e = s0; e < s1; e++) rank += slotBody[e] > g ? 1 : 0;
-    float2 p = pos[g];
+    float4 p = body[g];
     float m = mats[g].x;
     cellBody[s0 + rank] = make_float4(p.x, p.y, m, __uint_as_float(g));
     if (rank == 0) {
@@ -76,49 +76,57 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
     }
 }
 
-// S/kick.comp:66-80
-__device__ __forceinline__ void local_acceleration(const SimDesc& s, float sx, float sy, float tx,
-                                                   float ty, float mass, float& ax, float& ay) {
-    float dx = sx - tx, dy = sy - ty;
-    if (!s.bounded) {
-        dx -= floorf(dx + 0.5f);
-        dy -= floorf(dy + 0.5f);
-    }
-    float px = dx * s.aspect, py = dy * 1.0f;
-    float r2 = px * px + py * py;
-    float lw = expf(-r2 / s.twoSigma2);
-    float scale = s.G * mass * lw / (s.totalMass * (r2 > s.minDist2 ? r2 : s.minDist2));
-    ax += scale * px / s.aspect;
-    ay += scale * py / 1.0f;
-}
-
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 
+// Cells of a stencil row whose tables are loaded together (the default
+// 3-cell near-field radius in one batch; wider stencils take several).
+constexpr int NEAR_BATCH = 7;
+
+// Short-range term of one source (S/kick.comp:66-80) from its physical
+// displacement (px = dx * aspect, py = dy) and its Gaussian weight.
+__device__ __forceinline__ void near_term(const SimDesc& s, float px, float py, float weight,
+                                          float mass, float& ax, float& ay) {
+    float r2 = px * px + py * py;
+    float q = s.gOverM * mass * weight * __builtin_amdgcn_rcpf(fmaxf(r2, s.minDist2));
+    ax += q * px * s.invAspect;
+    ay += q * py;
+}
+
+__device__ __forceinline__ float min_image(const SimDesc& s, float d) {
+    return s.bounded ? d : d - floorf(d + 0.5f);
+}
+
 // S/kick.comp:105-156 (sleep handled by the caller's schedule).
+//
+// The mesh gather follows the reference's operations. The near field is the
+// reference's sum in the reference's order, evaluated on the FP32-tolerance
+// path: divisions by constants become reciprocal products, exp is the
+// hardware exp2, and the Gaussian of a cell monopole is the product of its
+// x and y factors, so a body takes 2(2r+1) exponentials for its monopoles
+// instead of (2r+1)^2.
 __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
                                              const BodyBlock* __restrict__ blocks,
-                                             const float2* __restrict__ pos,
-                                             float2* __restrict__ vel, float2* __restrict__ acc,
+                                             float4* __restrict__ body,
+                                             float2* __restrict__ acc,
                                              const float2* __restrict__ force,
                                              const uint32_t* __restrict__ cellPrefix,
                                              const uint32_t* __restrict__ cellMass,
                                              const float4* __restrict__ cellBody, int initial) {
-    BodyBlock b = blocks[blockIdx.x];
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
     float h = 0.5f * s.dt;
+    float4 p = body[g];
     if (!s.mutual) {
         if (!initial) {
-            float2 v = vel[g];
-            v.x += s.fieldX * h;
-            v.y += s.fieldY * h;
-            vel[g] = v;
+            p.z += s.fieldX * h;
+            p.w += s.fieldY * h;
+            body[g] = p;
         }
         acc[g] = make_float2(s.fieldX, s.fieldY);
         return;
     }
-    float2 p = pos[g];
     // S/kick.comp:82-103: CIC gather of the mesh force (x = Fx, y = Fy).
     float gx = p.x * (float)s.pw - 0.5f, gy = p.y * (float)s.ph - 0.5f;
     float flx = floorf(gx), fly = floorf(gy);
@@ -133,49 +141,65 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
     float ay = mixf(mixf(v00.y, v10.y, fx), mixf(v01.y, v11.y, fx), fy);
     ax += s.fieldX;
     ay += s.fieldY;
-    // Short-range complement over the (2rx+1) x (2ry+1) physical cells.
+    const float k2 = s.invTwoSigma2;
     int hx = (int)floorf(p.x * (float)s.pw), hy = (int)floorf(p.y * (float)s.ph);
     for (int dy = -s.ry; dy <= s.ry; dy++) {
         int cy = hy + dy;
         if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
         int wy = wrap_axis(cy, s.ph);
-        for (int dx = -s.rx; dx <= s.rx; dx++) {
-            int cx = hx + dx;
-            if (s.bounded && (cx < 0 || cx >= s.pw)) continue;
-            int wx = wrap_axis(cx, s.pw);
-            int c = s.physBase + wx + wy * s.pw;
-            int s0 = (int)cellPrefix[c], s1 = (int)cellPrefix[c + 1];
-            int count = s1 - s0;
-            bool home = dx == 0 && dy == 0;
-            if (!home && count > s.exactLimit) {
-                float cxn = ((float)wx + 0.5f) / (float)s.pw;
-                float cyn = ((float)wy + 0.5f) / (float)s.ph;
-                float agg = (float)cellMass[c] / 256.0f;
-                local_acceleration(s, cxn, cyn, p.x, p.y, agg, ax, ay);
-            } else {
-                int limit = home ? s.homeLimit : s.exactLimit;
-                int end = s0 + (count < limit ? count : limit);
-                for (int t = s0; t < end; t++) {
-                    float4 o = cellBody[t];
-                    if (__float_as_uint(o.w) != (uint32_t)g)
-                        local_acceleration(s, o.x, o.y, p.x, p.y, o.z, ax, ay);
+        float py = min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
+        float ey = __expf(-(py * py) * k2);
+        const int rowBase = s.physBase + wy * s.pw;
+        for (int dx0 = -s.rx; dx0 <= s.rx; dx0 += NEAR_BATCH) {
+            uint32_t first[NEAR_BATCH], after[NEAR_BATCH], cmass[NEAR_BATCH];
+#pragma unroll
+            for (int k = 0; k < NEAR_BATCH; k++) {
+                int dx = dx0 + k, cx = hx + dx;
+                bool on = dx <= s.rx && (!s.bounded || (cx >= 0 && cx < s.pw));
+                int c = rowBase + wrap_axis(cx, s.pw);
+                first[k] = on ? cellPrefix[c] : 0u;
+                after[k] = on ? cellPrefix[c + 1] : 0u;
+                cmass[k] = on ? cellMass[c] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < NEAR_BATCH; k++) {
+                int dx = dx0 + k, cx = hx + dx;
+                if (dx > s.rx || (s.bounded && (cx < 0 || cx >= s.pw))) continue;
+                int s0 = (int)first[k];
+                int count = (int)(after[k] - first[k]);
+                bool home = dx == 0 && dy == 0;
+                if (!home && count > s.exactLimit) {
+                    int wx = wrap_axis(cx, s.pw);
+                    float px = min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
+                    float w = __expf(-(px * px) * k2) * ey;
+                    near_term(s, px, py, w, (float)cmass[k] * (1.0f / 256.0f), ax, ay);
+                } else {
+                    int limit = home ? s.homeLimit : s.exactLimit;
+                    int stop = s0 + (count < limit ? count : limit);
+                    for (int t = s0; t < stop; t++) {
+                        float4 o = cellBody[t];
+                        if (__float_as_uint(o.w) == (uint32_t)g) continue;
+                        float ox = min_image(s, o.x - p.x) * s.aspect;
+                        float oy = min_image(s, o.y - p.y);
+                        float w = __expf(-(ox * ox + oy * oy) * k2);
+                        near_term(s, ox, oy, w, o.z, ax, ay);
+                    }
                 }
             }
         }
     }
     if (!initial) {
-        float2 v = vel[g];
-        v.x += ax * h;
-        v.y += ay * h;
-        vel[g] = v;
+        p.z += ax * h;
+        p.w += ay * h;
+        body[g] = p;
     }
     acc[g] = make_float2(ax, ay);
 }
 
-void launch_bin(hipStream_t st, const LaunchShape& ls, const float2* pos, uint32_t* cellCount,
+void launch_bin(hipStream_t st, const LaunchShape& ls, const float4* body, uint32_t* cellCount,
                 uint32_t* homeOf) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_bin, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, pos,
+    hipLaunchKernelGGL(k_bin, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body,
                        cellCount, homeOf);
 }
 
@@ -187,20 +211,19 @@ void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
 }
 
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
-                       const uint32_t* cellPrefix, const uint32_t* slotBody, const float2* pos,
+                       const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
                        const float4* mats, float4* cellBody, uint32_t* cellMass) {
     if (slots <= 0) return;
     hipLaunchKernelGGL(k_cell_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, homeOf,
-                       cellPrefix, slotBody, pos, mats, cellBody, cellMass);
+                       cellPrefix, slotBody, body, mats, cellBody, cellMass);
 }
 
-void launch_kick(hipStream_t st, const LaunchShape& ls, const float2* pos, float2* vel,
-                 float2* acc, const float4* mats, const float2* force, const uint32_t* cellPrefix,
-                 const uint32_t* cellMass, const float4* cellBody, int initial) {
-    (void)mats;
+void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
+                 const float2* force, const uint32_t* cellPrefix, const uint32_t* cellMass,
+                 const float4* cellBody, int initial) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_kick, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, pos, vel,
-                       acc, force, cellPrefix, cellMass, cellBody, initial);
+    hipLaunchKernelGGL(k_kick, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
+                       force, cellPrefix, cellMass, cellBody, initial);
 }
 
 }  // namespace gs

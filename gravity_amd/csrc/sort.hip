@@ -136,13 +136,13 @@ __device__ __forceinline__ uint32_t sort_key(float x, float y, int wb, int hb) {
 // S/spatial_sort_count.comp:8-12
 __global__ void __launch_bounds__(WG) k_sort_count(const SimDesc* __restrict__ sims,
                                                    const BodyBlock* __restrict__ blocks,
-                                                   const float2* __restrict__ pos,
+                                                   const float4* __restrict__ body,
                                                    uint32_t* tileCount, uint32_t* keyOf) {
-    BodyBlock b = blocks[blockIdx.x];
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
-    float2 p = pos[g];
+    float4 p = body[g];
     uint32_t t = (uint32_t)s.tileBase + sort_key(p.x, p.y, s.sortWb, s.sortHb);
     keyOf[g] = t;
     atomicAdd(&tileCount[t], 1u);
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(WG) k_sort_scatter(const SimDesc* __restrict__
                                                      uint32_t* tileCount,
                                                      const uint32_t* __restrict__ tilePrefix,
                                                      uint32_t* __restrict__ slotSrc) {
-    BodyBlock b = blocks[blockIdx.x];
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
@@ -169,11 +169,11 @@ __global__ void __launch_bounds__(WG) k_sort_scatter(const SimDesc* __restrict__
 __global__ void __launch_bounds__(WG) k_sort_canon(
     const SimDesc* __restrict__ sims, const BodyBlock* __restrict__ blocks,
     const uint32_t* __restrict__ keyOf, const uint32_t* __restrict__ tilePrefix,
-    const uint32_t* __restrict__ slotSrc, const float2* __restrict__ posIn,
-    const float2* __restrict__ velIn, const float2* __restrict__ accIn,
-    const float4* __restrict__ matIn, float2* __restrict__ posOut, float2* __restrict__ velOut,
-    float2* __restrict__ accOut, float4* __restrict__ matOut, uint32_t* __restrict__ dest) {
-    BodyBlock b = blocks[blockIdx.x];
+    const uint32_t* __restrict__ slotSrc, const float4* __restrict__ bodyIn,
+    const float2* __restrict__ accIn, const float4* __restrict__ matIn,
+    float4* __restrict__ bodyOut, float2* __restrict__ accOut, float4* __restrict__ matOut,
+    uint32_t* __restrict__ dest) {
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int d = b.first + (int)threadIdx.x;
     if (d - s.base >= s.n) return;
@@ -184,18 +184,17 @@ __global__ void __launch_bounds__(WG) k_sort_canon(
     int rank = 0;
     for (int e = s0; e < s1; e++) rank += slotSrc[e] < src ? 1 : 0;
     int fin = s0 + rank;
-    posOut[fin] = posIn[src];
-    velOut[fin] = velIn[src];
+    bodyOut[fin] = bodyIn[src];
     accOut[fin] = accIn[src];
     matOut[fin] = matIn[src];
     dest[src] = (uint32_t)(fin - s.base);
 }
 
-void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float2* pos,
+void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float4* body,
                        uint32_t* tileCount, uint32_t* keyOf) {
     if (ls.bodyBlocks == 0) return;
     hipLaunchKernelGGL(k_sort_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       pos, tileCount, keyOf);
+                       body, tileCount, keyOf);
 }
 
 void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
@@ -207,13 +206,12 @@ void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
 
 void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
                        const uint32_t* tilePrefix, const uint32_t* slotSrc,
-                       const float2* posIn, const float2* velIn, const float2* accIn,
-                       const float4* matIn, float2* posOut, float2* velOut, float2* accOut,
-                       float4* matOut, uint32_t* dest) {
+                       const float4* bodyIn, const float2* accIn, const float4* matIn,
+                       float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest) {
     if (ls.bodyBlocks == 0) return;
     hipLaunchKernelGGL(k_sort_canon, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       keyOf, tilePrefix, slotSrc, posIn, velIn, accIn, matIn, posOut, velOut,
-                       accOut, matOut, dest);
+                       keyOf, tilePrefix, slotSrc, bodyIn, accIn, matIn, bodyOut, accOut, matOut,
+                       dest);
 }
 
 }  // namespace gs

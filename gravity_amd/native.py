@@ -17,7 +17,7 @@ GS_OK, GS_EINVAL, GS_ESTATE = 0, -1, -2
 GS_BOUNDED, GS_MUTUAL_GRAVITY, GS_LIQUID = 1, 2, 3
 GS_BODIES_XYVV, GS_PRESSURES, GS_ACCEL, GS_MATERIALS = 0, 1, 2, 3
 GS_CONTACT_COUNTS, GS_SORT_DEST, GS_FORCE_MESH, GS_CONTACTS = 4, 5, 6, 7
-CONTACT_SLOTS = 32
+CONTACT_SLOTS = 16
 
 
 class GsConfig(ctypes.Structure):
