@@ -37,9 +37,15 @@ __device__ __forceinline__ void grid_insert(const SimDesc& s, int g, float px, f
 }
 
 // S/collision_crowd.glsl:2-15: the lower index goes first; squares summed
-// without contraction.
+// without contraction. Without the torus's minimum image the order cannot
+// matter: fl(a - b) = -fl(b - a) exactly, so the squares are the same.
 __device__ __forceinline__ float pair_distance_squared(int id, float px, float py, int other,
                                                        float ox, float oy, const SimDesc& s) {
+    if (s.bounded) {
+        float dx = px * s.vw - ox * s.vw;
+        float dy = py * s.vh - oy * s.vh;
+        return dx * dx + dy * dy;
+    }
     bool ownFirst = id < other;
     float ax = ownFirst ? px : ox, ay = ownFirst ? py : oy;
     float bx = ownFirst ? ox : px, by = ownFirst ? oy : py;
@@ -74,6 +80,12 @@ __device__ __forceinline__ void stencil_heads(const SimDesc& s, float px, float 
     }
 }
 
+// Visits every body of the stencil (the callers' results do not depend on
+// the order). The lists are walked one cell after another: a wave then pays
+// for the longest list of each cell over its lanes, fewer visits than
+// walking the nine lists in lockstep (the kernels are VALU-bound, and the
+// lockstep walk's registers cost occupancy). guardLimit bounds the nodes
+// visited.
 template <typename F>
 __device__ __forceinline__ void walk_stencil(const SimDesc& s, float px, float py,
                                              const GridBuf& grid, int guardLimit, F&& visit) {
@@ -266,7 +278,15 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
     uint32_t crowd = ownContacts > oc ? ownContacts : oc;
     uint32_t weightCount = crowd > 1u ? crowd : 1u;
     float pairWeight = weightCount < RECIP_TABLE ? c_recip.v[weightCount] : 1.0f / (float)weightCount;
-    float share = om.x / (ownMat.x + om.x);
+    // m / (m + m) is exactly 0.5 for every normal m, and sqrt(mu * mu) is
+    // exactly mu for mu = 0 and for every mu in [2^-63, 2^63) (both checked
+    // over all floats of those ranges): equal materials skip the division
+    // and the square root with identical results.
+    float share;
+    if (om.x == ownMat.x && ownMat.x >= 0x1p-126f && ownMat.x < 0x1p127f)
+        share = 0.5f;
+    else
+        share = om.x / (ownMat.x + om.x);
     float ovx = ob.z * s.vw, ovy = ob.w * s.vh;
     float rvx = vpx - ovx, rvy = vpy - ovy;
     float rvn = rvx * nx + rvy * ny;
@@ -279,7 +299,11 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
         acc.vcx -= nx * k;
         acc.vcy -= ny * k;
     }
-    float friction = sqrtf(ownMat.y * om.y);
+    float friction;
+    if (om.y == ownMat.y && (ownMat.y == 0.0f || (ownMat.y >= 0x1p-63f && ownMat.y < 0x1p63f)))
+        friction = ownMat.y;
+    else
+        friction = sqrtf(ownMat.y * om.y);
     if (friction > 0.0f) {
         float approach = -rvn > 0.0f ? -rvn : 0.0f;
         float fx, fy;
@@ -295,7 +319,10 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
     }
 }
 
-constexpr int SOLVE_BATCH = 8;  // contacts whose loads are issued together
+// Contacts whose loads are issued together. The batch loop is not
+// unrolled: four contacts' registers keep the kernel at 7 waves per SIMD,
+// which holds every wave of a 400k-body launch resident at once.
+constexpr int SOLVE_BATCH = 4;
 
 __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               const BodyBlock* __restrict__ blocks,
@@ -320,29 +347,30 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     uint32_t own = counts[g];
     int listed = own < (uint32_t)CONTACT_SLOTS ? (int)own : CONTACT_SLOTS;
     uint32_t last = 0u;
-    // All slot indices first, then the contacts' data a batch at a time, so
-    // the gathers of a batch are in flight together.
-    uint32_t id[CONTACT_SLOTS];
+    const float2* __restrict__ matXY = reinterpret_cast<const float2*>(mats);
+#pragma unroll 1
+    for (int i0 = 0; i0 < listed; i0 += SOLVE_BATCH) {
+        uint32_t id[SOLVE_BATCH];
 #pragma unroll
-    for (int k = 0; k < CONTACT_SLOTS; k++)
-        id[k] = k < listed ? contacts[(size_t)k * planeStride + g] : (uint32_t)g;
-#pragma unroll
-    for (int i0 = 0; i0 < CONTACT_SLOTS; i0 += SOLVE_BATCH) {
-        if (i0 >= listed) break;
-        float4 ob[SOLVE_BATCH], om[SOLVE_BATCH];
+        for (int q = 0; q < SOLVE_BATCH; q++)
+            id[q] = i0 + q < listed ? contacts[(size_t)(i0 + q) * planeStride + g] : (uint32_t)g;
+        float4 ob[SOLVE_BATCH];
+        float2 mf[SOLVE_BATCH];
         uint32_t oc[SOLVE_BATCH];
 #pragma unroll
         for (int q = 0; q < SOLVE_BATCH; q++) {
-            ob[q] = bodyIn[id[i0 + q]];
-            om[q] = mats[id[i0 + q]];
-            oc[q] = counts[id[i0 + q]];
+            ob[q] = bodyIn[id[q]];
+            mf[q] = matXY[2 * (size_t)id[q]];  // mass, friction
+            oc[q] = counts[id[q]];
         }
 #pragma unroll
         for (int q = 0; q < SOLVE_BATCH; q++) {
             if (i0 + q < listed) {
-                last = id[i0 + q];
-                accumulate_contact(s, g, (int)last, ob[q], om[q], oc[q], fallback, ppx, ppy, vpx,
-                                   vpy, ownMat, own, writePressure, acc);
+                last = id[q];
+                float solid = s.liquid ? mats[last].z : 1.0f;
+                accumulate_contact(s, g, (int)last, ob[q], make_float4(mf[q].x, mf[q].y, solid, 0.0f),
+                                   oc[q], fallback, ppx, ppy, vpx, vpy, ownMat, own, writePressure,
+                                   acc);
             }
         }
     }

@@ -155,7 +155,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     cellPrefix_ = dalloc<uint32_t>((size_t)totalPhys_ + 1);
     homeOf_ = dalloc<uint32_t>(S);
     slotBody_ = dalloc<uint32_t>(S);
-    cellMass_ = dalloc<uint32_t>((size_t)totalPhys_);
+    cellRec_ = dalloc<uint4>((size_t)totalPhys_);
     cellBody_ = dalloc<float4>(S);
     spec_ = dalloc<float2>((size_t)totalMesh_);
     green_ = dalloc<float2>((size_t)totalMesh_);
@@ -209,7 +209,7 @@ Engine::~Engine() {
                     grid_[0].heads, grid_[1].heads, grid_[0].nodes, grid_[1].nodes,
                     grid_[0].cellOf, grid_[1].cellOf, press_, counts_, contacts_, tileCount_,
                     tilePrefix_, keyOf_, slotSrc_, dest_, scanTmp_, cellCount_, cellPrefix_,
-                    homeOf_, slotBody_, cellMass_, cellBody_, spec_, green_, tw_, fallback_,
+                    homeOf_, slotBody_, cellRec_, cellBody_, spec_, green_, tw_, fallback_,
                     dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -536,11 +536,13 @@ void Engine::enqueue_force_and_kick(bool initial) {
     // K/GravityPipeline.kt:907-973 then :1213-1261
     if (anyMutual_) {
         timed("bin", [&] { launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_); });
-        timed("cell_scan", [&] { launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_); });
+        timed("cell_scan", [&] {
+            launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, cellRec_);
+        });
         timed("cell_scatter", [&] { launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_); });
         timed("cell_canon", [&] {
             launch_cell_canon(st_, gravitySlots_, homeOf_, cellPrefix_, slotBody_, body_[bodyPar_],
-                              mat_[auxPar_], cellBody_, cellMass_);
+                              mat_[auxPar_], cellBody_, cellRec_);
         });
         timed("fft_rows_forward", [&] {
             launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_);
@@ -548,9 +550,12 @@ void Engine::enqueue_force_and_kick(bool initial) {
         timed("fft_columns", [&] { launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_); });
         timed("fft_rows_inverse", [&] { launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_); });
     }
+    bool radius3 = true;
+    for (const auto& h : sims_)
+        if (h.mutual && (h.desc.rx != 3 || h.desc.ry != 3)) radius3 = false;
     timed("kick", [&] {
-        launch_kick(st_, shape_, body_[bodyPar_], acc_[auxPar_], spec_,
-                    cellPrefix_, cellMass_, cellBody_, initial ? 1 : 0);
+        launch_kick(st_, shape_, body_[bodyPar_], acc_[auxPar_], spec_, cellRec_, cellBody_,
+                    initial ? 1 : 0, radius3);
     });
 }
 

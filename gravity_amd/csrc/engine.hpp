@@ -107,7 +107,8 @@ private:
     uint32_t *slotSrc_ = nullptr, *dest_ = nullptr, *scanTmp_ = nullptr;
     // P3M
     uint32_t *cellCount_ = nullptr, *cellPrefix_ = nullptr, *homeOf_ = nullptr;
-    uint32_t *slotBody_ = nullptr, *cellMass_ = nullptr;
+    uint32_t* slotBody_ = nullptr;
+    uint4* cellRec_ = nullptr;  // per physical cell: first slot, count, mass / 256, 0
     float4* cellBody_ = nullptr;
     float2 *spec_ = nullptr, *green_ = nullptr, *tw_ = nullptr, *fallback_ = nullptr;
     std::vector<GreenKey> greenKeys_;

@@ -25,7 +25,10 @@ struct LaunchShape {
 struct LineWork { int sim; int first; int count; int pad; };
 
 // ---- scan: out[i] = sum of in[0..i) for i in [0, len]; totals is scratch ----
-void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals);
+// Exclusive scan of len values (out[len] = total). rec, if given, receives
+// {prefix, count} in the first half of each 16-byte element record.
+void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals,
+                 uint4* rec = nullptr);
 int scan_totals_size(int len);
 
 // ---- spatial sort ----
@@ -57,7 +60,7 @@ void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
                          uint32_t* cellCount, const uint32_t* cellPrefix, uint32_t* slotBody);
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
-                       const float4* mats, float4* cellBody, uint32_t* cellMass);
+                       const float4* mats, float4* cellBody, uint4* cellRec);
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
                              const float2* twiddles, int twiddleLength);
@@ -66,8 +69,10 @@ void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* wor
                         int twiddleLength);
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              float2* spec, const float2* twiddles, int twiddleLength);
+// cellRec: per physical cell {first slot, body count, mass in 1/256, 0}.
+// radius3: every simulation of the batch has a 3-cell near field (rx = ry = 3).
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
-                 const float2* force, const uint32_t* cellPrefix, const uint32_t* cellMass,
-                 const float4* cellBody, int initial);
+                 const float2* force, const uint4* cellRec, const float4* cellBody, int initial,
+                 bool radius3);
 
 }  // namespace gs

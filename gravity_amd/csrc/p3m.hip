@@ -51,14 +51,15 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
 }
 
 // Orders each cell by descending body index; the first body of a cell also
-// sums the cell mass in 1/256 units (S/build_grid.comp:75), exactly.
+// sums the cell mass in 1/256 units (S/build_grid.comp:75), exactly, into
+// the cell record (whose first half the cell scan wrote).
 __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __restrict__ homeOf,
                                                    const uint32_t* __restrict__ cellPrefix,
                                                    const uint32_t* __restrict__ slotBody,
                                                    const float4* __restrict__ body,
                                                    const float4* __restrict__ mats,
                                                    float4* __restrict__ cellBody,
-                                                   uint32_t* __restrict__ cellMass) {
+                                                   uint4* __restrict__ cellRec) {
     int d = (int)(blockIdx.x * WG + threadIdx.x);
     if (d >= slots) return;
     uint32_t g = slotBody[d];
@@ -72,46 +73,79 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
     if (rank == 0) {
         uint32_t sum = 0u;
         for (int e = s0; e < s1; e++) sum += (uint32_t)rintf(mats[slotBody[e]].x * 256.0f);
-        cellMass[c] = sum;
+        reinterpret_cast<uint32_t*>(cellRec)[4 * (size_t)c + 2] = sum;
     }
 }
 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 
-// Cells of a stencil row whose tables are loaded together (the default
-// 3-cell near-field radius in one batch; wider stencils take several).
-constexpr int NEAR_BATCH = 7;
-
-// Short-range term of one source (S/kick.comp:66-80) from its physical
-// displacement (px = dx * aspect, py = dy) and its Gaussian weight.
-__device__ __forceinline__ void near_term(const SimDesc& s, float px, float py, float weight,
-                                          float mass, float& ax, float& ay) {
-    float r2 = px * px + py * py;
-    float q = s.gOverM * mass * weight * __builtin_amdgcn_rcpf(fmaxf(r2, s.minDist2));
-    ax += q * px * s.invAspect;
-    ay += q * py;
-}
-
 __device__ __forceinline__ float min_image(const SimDesc& s, float d) {
     return s.bounded ? d : d - floorf(d + 0.5f);
 }
 
+// Short-range term of one source (S/kick.comp:66-80) from its physical
+// displacement (px = dx * aspect, py = dy), its squared length r2, its
+// Gaussian weight and its mass: G m w / (M max(r2, minDist2)) * (dx, dy).
+__device__ __forceinline__ void near_term(const SimDesc& s, float px, float py, float r2,
+                                          float weight, float mass, float& ax, float& ay) {
+    float q = s.gOverM * mass * weight * __builtin_amdgcn_rcpf(fmaxf(r2, s.minDist2));
+    ax += q * (px * s.invAspect);
+    ay += q * py;
+}
+
+// The bodies of one cell taken one by one (a home cell, or a cell with at
+// most exactNeighborBodyLimit bodies), skipping the body itself.
+__device__ __forceinline__ void exact_cell(const SimDesc& s, const float4* __restrict__ cellBody,
+                                           int first, int count, int g, float x, float y, float k2,
+                                           float& ax, float& ay) {
+    for (int t = first; t < first + count; t++) {
+        float4 o = cellBody[t];
+        if (__float_as_uint(o.w) == (uint32_t)g) continue;
+        float ox = min_image(s, o.x - x) * s.aspect;
+        float oy = min_image(s, o.y - y);
+        float r2 = ox * ox + oy * oy;
+        near_term(s, ox, oy, r2, __expf(-r2 * k2), o.z, ax, ay);
+    }
+}
+
+// S/kick.comp:82-103: CIC gather of the mesh force (x = Fx, y = Fy).
+__device__ __forceinline__ float2 mesh_force(const SimDesc& s, const float2* __restrict__ force,
+                                             float x, float y) {
+    float gx = x * (float)s.pw - 0.5f, gy = y * (float)s.ph - 0.5f;
+    float flx = floorf(gx), fly = floorf(gy);
+    int bx = (int)flx, by = (int)fly;
+    float fx = gx - flx, fy = gy - fly;
+    int x0 = wrap_axis(bx, s.mw), x1 = wrap_axis(bx + 1, s.mw);
+    int y0 = wrap_axis(by, s.mh), y1 = wrap_axis(by + 1, s.mh);
+    const float2* F = force + s.meshBase;
+    float2 v00 = F[x0 + y0 * s.mw], v10 = F[x1 + y0 * s.mw];
+    float2 v01 = F[x0 + y1 * s.mw], v11 = F[x1 + y1 * s.mw];
+    return make_float2(mixf(mixf(v00.x, v10.x, fx), mixf(v01.x, v11.x, fx), fy),
+                       mixf(mixf(v00.y, v10.y, fx), mixf(v01.y, v11.y, fx), fy));
+}
+
 // S/kick.comp:105-156 (sleep handled by the caller's schedule).
 //
-// The mesh gather follows the reference's operations. The near field is the
-// reference's sum in the reference's order, evaluated on the FP32-tolerance
-// path: divisions by constants become reciprocal products, exp is the
-// hardware exp2, and the Gaussian of a cell monopole is the product of its
-// x and y factors, so a body takes 2(2r+1) exponentials for its monopoles
-// instead of (2r+1)^2.
+// The near field is the reference's sum over the (2rx+1) x (2ry+1)
+// physical cells around the home cell: the home cell body by body (at most
+// maximumHomeBodyInteractions list entries, the body itself skipped), cells
+// with more than exactNeighborBodyLimit bodies as a monopole at the cell
+// centre, the others body by body. It is evaluated on the FP32-tolerance
+// path (contraction allowed, reciprocal products, hardware exp2), and a
+// monopole's Gaussian is the product of its column and row factors.
+//
+// R > 0 is a near field of rx = ry = R cells (the default configurations):
+// the column offsets and factors are computed once per body and each cell
+// is one 16-byte record load. R = 0 takes any radius, a row at a time.
+template <int R>
 __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
                                              const BodyBlock* __restrict__ blocks,
                                              float4* __restrict__ body,
                                              float2* __restrict__ acc,
                                              const float2* __restrict__ force,
-                                             const uint32_t* __restrict__ cellPrefix,
-                                             const uint32_t* __restrict__ cellMass,
+                                             const uint4* __restrict__ cellRec,
                                              const float4* __restrict__ cellBody, int initial) {
+#pragma clang fp contract(fast)
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
@@ -127,63 +161,77 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
         acc[g] = make_float2(s.fieldX, s.fieldY);
         return;
     }
-    // S/kick.comp:82-103: CIC gather of the mesh force (x = Fx, y = Fy).
-    float gx = p.x * (float)s.pw - 0.5f, gy = p.y * (float)s.ph - 0.5f;
-    float flx = floorf(gx), fly = floorf(gy);
-    int bx = (int)flx, by = (int)fly;
-    float fx = gx - flx, fy = gy - fly;
-    int x0 = wrap_axis(bx, s.mw), x1 = wrap_axis(bx + 1, s.mw);
-    int y0 = wrap_axis(by, s.mh), y1 = wrap_axis(by + 1, s.mh);
-    const float2* F = force + s.meshBase;
-    float2 v00 = F[x0 + y0 * s.mw], v10 = F[x1 + y0 * s.mw];
-    float2 v01 = F[x0 + y1 * s.mw], v11 = F[x1 + y1 * s.mw];
-    float ax = mixf(mixf(v00.x, v10.x, fx), mixf(v01.x, v11.x, fx), fy);
-    float ay = mixf(mixf(v00.y, v10.y, fx), mixf(v01.y, v11.y, fx), fy);
-    ax += s.fieldX;
-    ay += s.fieldY;
+    float2 mf = mesh_force(s, force, p.x, p.y);
+    float ax = mf.x + s.fieldX, ay = mf.y + s.fieldY;
     const float k2 = s.invTwoSigma2;
+    const float cellMassUnit = 1.0f / 256.0f;
     int hx = (int)floorf(p.x * (float)s.pw), hy = (int)floorf(p.y * (float)s.ph);
-    for (int dy = -s.ry; dy <= s.ry; dy++) {
-        int cy = hy + dy;
-        if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
-        int wy = wrap_axis(cy, s.ph);
-        float py = min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
-        float ey = __expf(-(py * py) * k2);
-        const int rowBase = s.physBase + wy * s.pw;
-        for (int dx0 = -s.rx; dx0 <= s.rx; dx0 += NEAR_BATCH) {
-            uint32_t first[NEAR_BATCH], after[NEAR_BATCH], cmass[NEAR_BATCH];
+    if constexpr (R > 0) {
+        constexpr int D = 2 * R + 1;
+        float cpx[D], cex[D];
+        int cwx[D];
+        uint32_t colOk = 0u;
 #pragma unroll
-            for (int k = 0; k < NEAR_BATCH; k++) {
-                int dx = dx0 + k, cx = hx + dx;
-                bool on = dx <= s.rx && (!s.bounded || (cx >= 0 && cx < s.pw));
-                int c = rowBase + wrap_axis(cx, s.pw);
-                first[k] = on ? cellPrefix[c] : 0u;
-                after[k] = on ? cellPrefix[c + 1] : 0u;
-                cmass[k] = on ? cellMass[c] : 0u;
+        for (int k = 0; k < D; k++) {
+            int cx = hx + k - R;
+            if (!s.bounded || (cx >= 0 && cx < s.pw)) colOk |= 1u << k;
+            int wx = wrap_axis(cx, s.pw);
+            cwx[k] = wx;
+            float px = min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
+            cpx[k] = px;
+            cex[k] = __expf(-(px * px) * k2);
+        }
+#pragma unroll 1
+        for (int j = 0; j < D; j++) {
+            int cy = hy + j - R;
+            if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
+            int wy = wrap_axis(cy, s.ph);
+            float py = min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
+            float py2 = py * py;
+            float ey = __expf(-py2 * k2);
+            const uint4* row = cellRec + (s.physBase + wy * s.pw);
+            uint4 rec[D];
+#pragma unroll
+            for (int k = 0; k < D; k++)
+                rec[k] = (colOk >> k) & 1u ? row[cwx[k]] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                int count = (int)rec[k].y;
+                bool home = j == R && k == R;
+                if (!home && count > s.exactLimit) {
+                    float px = cpx[k];
+                    near_term(s, px, py, px * px + py2, cex[k] * ey,
+                              (float)rec[k].z * cellMassUnit, ax, ay);
+                } else if (count > 0) {
+                    int limit = home ? s.homeLimit : s.exactLimit;
+                    exact_cell(s, cellBody, (int)rec[k].x, count < limit ? count : limit, g, p.x,
+                               p.y, k2, ax, ay);
+                }
             }
-#pragma unroll
-            for (int k = 0; k < NEAR_BATCH; k++) {
-                int dx = dx0 + k, cx = hx + dx;
-                if (dx > s.rx || (s.bounded && (cx < 0 || cx >= s.pw))) continue;
-                int s0 = (int)first[k];
-                int count = (int)(after[k] - first[k]);
+        }
+    } else {
+        for (int dy = -s.ry; dy <= s.ry; dy++) {
+            int cy = hy + dy;
+            if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
+            int wy = wrap_axis(cy, s.ph);
+            float py = min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
+            float ey = __expf(-(py * py) * k2);
+            const uint4* row = cellRec + (s.physBase + wy * s.pw);
+            for (int dx = -s.rx; dx <= s.rx; dx++) {
+                int cx = hx + dx;
+                if (s.bounded && (cx < 0 || cx >= s.pw)) continue;
+                int wx = wrap_axis(cx, s.pw);
+                uint4 rec = row[wx];
+                int count = (int)rec.y;
                 bool home = dx == 0 && dy == 0;
                 if (!home && count > s.exactLimit) {
-                    int wx = wrap_axis(cx, s.pw);
                     float px = min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
-                    float w = __expf(-(px * px) * k2) * ey;
-                    near_term(s, px, py, w, (float)cmass[k] * (1.0f / 256.0f), ax, ay);
-                } else {
+                    near_term(s, px, py, px * px + py * py, __expf(-(px * px) * k2) * ey,
+                              (float)rec.z * cellMassUnit, ax, ay);
+                } else if (count > 0) {
                     int limit = home ? s.homeLimit : s.exactLimit;
-                    int stop = s0 + (count < limit ? count : limit);
-                    for (int t = s0; t < stop; t++) {
-                        float4 o = cellBody[t];
-                        if (__float_as_uint(o.w) == (uint32_t)g) continue;
-                        float ox = min_image(s, o.x - p.x) * s.aspect;
-                        float oy = min_image(s, o.y - p.y);
-                        float w = __expf(-(ox * ox + oy * oy) * k2);
-                        near_term(s, ox, oy, w, o.z, ax, ay);
-                    }
+                    exact_cell(s, cellBody, (int)rec.x, count < limit ? count : limit, g, p.x, p.y,
+                               k2, ax, ay);
                 }
             }
         }
@@ -212,18 +260,22 @@ void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
 
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
-                       const float4* mats, float4* cellBody, uint32_t* cellMass) {
+                       const float4* mats, float4* cellBody, uint4* cellRec) {
     if (slots <= 0) return;
     hipLaunchKernelGGL(k_cell_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, homeOf,
-                       cellPrefix, slotBody, body, mats, cellBody, cellMass);
+                       cellPrefix, slotBody, body, mats, cellBody, cellRec);
 }
 
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
-                 const float2* force, const uint32_t* cellPrefix, const uint32_t* cellMass,
-                 const float4* cellBody, int initial) {
+                 const float2* force, const uint4* cellRec, const float4* cellBody, int initial,
+                 bool radius3) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_kick, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
-                       force, cellPrefix, cellMass, cellBody, initial);
+    if (radius3)
+        hipLaunchKernelGGL(k_kick<3>, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                           body, acc, force, cellRec, cellBody, initial);
+    else
+        hipLaunchKernelGGL(k_kick<0>, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                           body, acc, force, cellRec, cellBody, initial);
 }
 
 }  // namespace gs

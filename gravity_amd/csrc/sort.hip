@@ -91,23 +91,33 @@ __global__ void __launch_bounds__(WG) k_scan_totals(uint32_t* totals, int blocks
     }
 }
 
+// Adds the block offsets. With rec, also writes {prefix, count} into the
+// first half of each element's 16-byte record (the cell tables of the kick).
 __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int len,
-                                                 const uint32_t* __restrict__ totals) {
+                                                 const uint32_t* __restrict__ totals,
+                                                 const uint32_t* __restrict__ in,
+                                                 uint2* __restrict__ rec) {
     int base = (int)blockIdx.x * SCAN_BLOCK;
     uint32_t add = totals[blockIdx.x];
     for (int i = (int)threadIdx.x; i < SCAN_BLOCK; i += WG) {
         int idx = base + i;
-        if (idx <= len) out[idx] += add;
+        if (idx <= len) {
+            uint32_t v = out[idx] + add;
+            out[idx] = v;
+            if (rec != nullptr && idx < len) rec[2 * (size_t)idx] = make_uint2(v, in[idx]);
+        }
     }
 }
 
 int scan_totals_size(int len) { return (len + 1 + SCAN_BLOCK - 1) / SCAN_BLOCK; }
 
-void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals) {
+void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals,
+                 uint4* rec) {
     int blocks = scan_totals_size(len);
     hipLaunchKernelGGL(k_scan_local, dim3(blocks), dim3(WG), 0, st, in, out, len, totals);
     hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(WG), 0, st, totals, blocks);
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in,
+                       reinterpret_cast<uint2*>(rec));
 }
 
 // ---------------------------------------------------------------------------
