@@ -70,6 +70,90 @@ def units_per_launch(name, pipe, n):
     return n
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_galaxy400k.md")
+KERNEL_SYMBOL = {"kick": "k_kick<3>"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy of known size), or None."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    symbol = KERNEL_SYMBOL.get(kernel, "k_" + kernel)
+    for line in open(TRAFFIC_SUMMARY):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == symbol:
+            return float(cells[4])
+    return None
+
+
+ENSEMBLE_VIEWPORT = (1024, 1024)
+
+
+def ensemble_configs():
+    from gravity_amd import CollisionConfig, GravityConfig, GravityFieldConfig, ParticleConfig
+    from gravity_amd import scenarios
+    members = scenarios.ensemble(64, 100, *ENSEMBLE_VIEWPORT)
+    cfgs = [GravityConfig(particles=ParticleConfig(capacity=n, initialBodies=n, seed=100 + k),
+                          gravity=GravityFieldConfig(gridWidth=g, gridHeight=g, minimumGrid=16),
+                          collisions=CollisionConfig(maximumSubsteps=SUBSTEPS, sleep=False))
+            for k, (n, g, _, _) in enumerate(members)]
+    return members, cfgs
+
+
+def batched_leg(steps, warmup):
+    """The ensemble of 64 torus simulations stepped as one batch (one graph
+    launch per step) and, for comparison, as 64 plain pipelines stepped one
+    after another. Both rates are whole-ensemble body-steps/s."""
+    from gravity_amd import GravityBatch, GravityPipeline
+    members, cfgs = ensemble_configs()
+    total = sum(m[0] for m in members)
+    with GravityBatch(cfgs, [ENSEMBLE_VIEWPORT] * len(cfgs), [m[2] for m in members],
+                      bounded=[False] * len(cfgs)) as batch:
+        for k, (n, _, b, mat) in enumerate(members):
+            batch.upload(k, b, mat)
+            batch.setActive(k, n)
+        batch.initializeAcceleration()
+        for _ in range(warmup):
+            batch.step()
+        batch.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.step()
+        batch.synchronize()
+        t_batch = time.perf_counter() - t0
+    pipes = []
+    try:
+        for cfg, (n, _, b, mat) in zip(cfgs, members):
+            p = GravityPipeline(cfg, *ENSEMBLE_VIEWPORT, scenarioState=b, bounded=False)
+            p.upload(b, mat)
+            p.initializeAcceleration(n)
+            pipes.append(p)
+        for _ in range(warmup):
+            for p, m in zip(pipes, members):
+                p.step(m[0])
+        for p in pipes:
+            p.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for p, m in zip(pipes, members):
+                p.step(m[0])
+        for p in pipes:
+            p.synchronize()
+        t_plain = time.perf_counter() - t0
+    finally:
+        for p in pipes:
+            p.close()
+    rate = total * steps / t_batch
+    plain = total * steps / t_plain
+    return {"workload": "ensemble-64: 64 torus simulations on 1024x1024, N = floor(10^U(3,5)), "
+                        "grids 2^U{6..9} squared, Plummer spheres a = 50 px, seeds 100..163",
+            "sims": len(members), "bodies": total, "steps": steps, "warmup": warmup,
+            "value": rate, "unit": "body-steps/s", "ms_per_step": t_batch * 1e3 / steps,
+            "plain_loop": plain, "plain_ms_per_step": t_plain * 1e3 / steps,
+            "speedup_vs_plain": rate / plain}
+
+
 def cpu_baseline(bodies, mats, steps=2):
     """The oracle (single-threaded C restatement) on `steps` steps of the same workload."""
     from oracle import oracle as orc
@@ -92,6 +176,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=8)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=20,
+                    help="steps of the batched-ensemble leg (0 skips it)")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel eagerly instead of through captured graphs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -143,6 +229,12 @@ def main():
     finite = bool(np.isfinite(b).all())
     pipe.close()
 
+    batched = None
+    if args.batched_steps > 0:
+        print(f"[bench] batched ensemble: {args.batched_steps} steps, batch and plain loop",
+              file=sys.stderr, flush=True)
+        batched = batched_leg(args.batched_steps, max(2, args.warmup // 4))
+
     cpu = None
     if not args.no_cpu:
         print(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps", file=sys.stderr, flush=True)
@@ -168,10 +260,12 @@ def main():
                    "bodies": n, "grid": list(GRID), "substeps": SUBSTEPS, "bounded": True,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dominant),
                      "avg_launch_us": avg_s * 1e6, "bytes_per_launch": alg_bytes},
         "kernels_ms_per_step": {k: v[1] / args.profile_steps for k, v in per_kernel.items()},
         "finite": finite,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -88,6 +88,26 @@ int gs_create(const gs_config* cfg, const float* state, int n, gs_sim** out) {
     });
 }
 
+int gs_batch_create(const gs_config* cfgs, int count, const float* const* states,
+                    const int* n, gs_sim** out) {
+    if (!cfgs || !out) return fail(GS_EINVAL, "null argument");
+    if (count < 1) return fail(GS_EINVAL, "a batch holds at least one simulation");
+    *out = nullptr;
+    std::vector<gs_config> c(cfgs, cfgs + count);
+    std::vector<const float*> st((size_t)count, nullptr);
+    std::vector<int> k((size_t)count, 0);
+    for (int i = 0; i < count; i++) {
+        k[i] = n ? n[i] : 0;
+        st[i] = states ? states[i] : nullptr;
+        if (k[i] < 0) return fail(GS_EINVAL, "negative scenario size");
+        if (k[i] > 0 && !st[i]) return fail(GS_EINVAL, "scenario data is null");
+    }
+    return guarded([&] {
+        auto* e = new gs::Engine(c, st, k);
+        *out = new gs_sim{e};
+    });
+}
+
 void gs_destroy(gs_sim* sim) {
     if (!sim) return;
     delete sim->engine;
@@ -162,9 +182,48 @@ int gs_stop_bodies(gs_sim* sim, int active) {
     });
 }
 
+#define INDEX_OR_FAIL                                                          \
+    if (index < 0 || index >= e.sims()) return fail(GS_EINVAL, "no such simulation");
+
+int gs_batch_size(gs_sim* sim) {
+    SIM_OR_FAIL
+    return e.sims();
+}
+
+int gs_batch_set_active(gs_sim* sim, int index, int active) {
+    SIM_OR_FAIL
+    INDEX_OR_FAIL
+    return guarded([&] { e.set_active(index, active); });
+}
+
+int gs_batch_upload(gs_sim* sim, int index, int first, int count, const float* xyvv,
+                    const float* mats) {
+    SIM_OR_FAIL
+    INDEX_OR_FAIL
+    if (count > 0 && !xyvv) return fail(GS_EINVAL, "null bodies");
+    return guarded([&] { e.upload(index, first, count, xyvv, mats); });
+}
+
+int gs_batch_download(gs_sim* sim, int index, int what, int first, int count, void* dst) {
+    SIM_OR_FAIL
+    INDEX_OR_FAIL
+    if (count > 0 && !dst) return fail(GS_EINVAL, "null destination");
+    return guarded([&] { e.download(index, what, first, count, dst); });
+}
+
+int gs_batch_initialize_acceleration(gs_sim* sim) {
+    SIM_OR_FAIL
+    return guarded([&] { e.initialize_acceleration(); });
+}
+
+int gs_batch_step(gs_sim* sim) {
+    SIM_OR_FAIL
+    return guarded([&] { e.step(); });
+}
+
 int gs_wake(gs_sim* sim) {
     SIM_OR_FAIL
-    return GS_OK;
+    return guarded([&] { e.wake(0); });
 }
 
 int gs_debug_resolve_collisions(gs_sim* sim, int count, float kick, float drift, int pressure,
@@ -187,8 +246,10 @@ int gs_debug_sort(gs_sim* sim, int count) {
 
 int gs_debug_resolve_liquid(gs_sim* sim, int count) {
     SIM_OR_FAIL
-    (void)count;
-    return fail(GS_ESTATE, "liquid passes are not built in this library");
+    return guarded([&] {
+        e.set_active(0, count);
+        e.debug_liquid();
+    });
 }
 
 int gs_synchronize(gs_sim* sim) {

@@ -106,19 +106,41 @@ __device__ __forceinline__ void walk_stencil(const SimDesc& s, float px, float p
 }
 
 // ---------------------------------------------------------------------------
-// Drift (S/collision_cache_drift.comp:79-91) and insertion into the grid.
+// Drift (S/collision_cache_drift.comp:60-91) and insertion into the grid.
+// With the sleeping tiles on, the step's kick pass tells every body's state
+// for the step: a body outside the awake tiles (and not a fluid body) rests,
+// its velocity dropped, and no pass of the step moves it.
 __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__ sims,
                                                      const BodyBlock* __restrict__ blocks,
                                                      float4* __restrict__ body,
                                                      const float2* __restrict__ acc,
+                                                     const float4* __restrict__ mats,
                                                      float kickDt, float driftDt, int insert,
-                                                     GridBuf grid) {
+                                                     GridBuf grid, SleepBufs sleep) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
     float4 v = body[g];
-    if (kickDt != 0.0f || driftDt != 0.0f) {
+    bool awake = true;
+    if (s.sleepOn) {
+        const SleepState st = sleep.state[b.sim];
+        if (sleep_on(s, st)) {
+            if (kickDt != 0.0f) {
+                bool fluid = s.liquid && mats[g].z == 0.0f;
+                awake = fluid || sleep.tileAwake[sleep_tile(s, v.x, v.y)] == st.stamp;
+                sleep.bodyAwake[g] = awake ? st.stamp : 0u;
+                if (!awake) {
+                    v.z = 0.0f;
+                    v.w = 0.0f;
+                    body[g] = v;
+                }
+            } else {
+                awake = sleep.bodyAwake[g] == st.stamp;
+            }
+        }
+    }
+    if (awake && (kickDt != 0.0f || driftDt != 0.0f)) {
         if (kickDt != 0.0f) {
             float2 a = acc[g];
             float h = 0.5f * kickDt;
@@ -148,12 +170,17 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
                                               const float4* __restrict__ mats,
                                               uint32_t* __restrict__ counts,
                                               uint32_t* __restrict__ contacts, int planeStride,
-                                              int guardLimit) {
+                                              int guardLimit, SleepBufs sleep) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
     if (clearPrev) prev.heads[prev.cellOf[g]] = 0u;
+    // A sleeping body keeps its last count (S/collision_cache_count.comp:147-160).
+    if (s.sleepOn) {
+        const SleepState st = sleep.state[b.sim];
+        if (sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp) return;
+    }
     float4 own = grid.nodes[g];
     float px = own.x, py = own.y;
     bool ownFluid = s.liquid && mats[g].z == 0.0f;
@@ -246,7 +273,7 @@ __device__ __forceinline__ void wall_contact(const SimDesc& s, float distance, f
 
 // One contact of the Jacobi sum, with the other body's data already loaded.
 __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int other, float4 ob,
-                                                   float4 om, uint32_t oc,
+                                                   float4 om, uint32_t oc, bool otherSleeps,
                                                    const float2* __restrict__ fallback, float ppx,
                                                    float ppy, float vpx, float vpy, float4 ownMat,
                                                    uint32_t ownContacts, int writePressure,
@@ -282,8 +309,12 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
     // exactly mu for mu = 0 and for every mu in [2^-63, 2^63) (both checked
     // over all floats of those ranges): equal materials skip the division
     // and the square root with identical results.
+    // A sleeping partner stays where it is: this side takes the whole
+    // correction (S/collision_cache_solve.comp:141-142, 201).
     float share;
-    if (om.x == ownMat.x && ownMat.x >= 0x1p-126f && ownMat.x < 0x1p127f)
+    if (otherSleeps)
+        share = 1.0f;
+    else if (om.x == ownMat.x && ownMat.x >= 0x1p-126f && ownMat.x < 0x1p127f)
         share = 0.5f;
     else
         share = om.x / (ownMat.x + om.x);
@@ -334,12 +365,26 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               int planeStride, GridBuf grid, int guardLimit,
                                               const float2* __restrict__ fallback,
                                               float* __restrict__ pressures, int writePressure,
-                                              float nextDriftDt, int insertNext, GridBuf next) {
+                                              float nextDriftDt, int insertNext, GridBuf next,
+                                              SleepBufs sleep) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
     float4 me = bodyIn[g];
+    bool sleeping = false;
+    uint32_t stamp = 0u;
+    if (s.sleepOn) {
+        const SleepState st = sleep.state[b.sim];
+        sleeping = sleep_on(s, st);
+        stamp = st.stamp;
+    }
+    if (sleeping && sleep.bodyAwake[g] != stamp) {
+        // A sleeping body is carried over as it is (S/collision_cache_solve.comp:279-287).
+        bodyOut[g] = me;
+        if (insertNext) grid_insert(s, g, me.x, me.y, next);
+        return;
+    }
     float4 ownMat = mats[g];
     float ppx = me.x * s.vw, ppy = me.y * s.vh;
     float vpx = me.z * s.vw, vpy = me.w * s.vh;
@@ -368,9 +413,10 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
             if (i0 + q < listed) {
                 last = id[q];
                 float solid = s.liquid ? mats[last].z : 1.0f;
+                bool otherSleeps = sleeping && sleep.bodyAwake[last] != stamp;
                 accumulate_contact(s, g, (int)last, ob[q], make_float4(mf[q].x, mf[q].y, solid, 0.0f),
-                                   oc[q], fallback, ppx, ppy, vpx, vpy, ownMat, own, writePressure,
-                                   acc);
+                                   oc[q], otherSleeps, fallback, ppx, ppy, vpx, vpy, ownMat, own,
+                                   writePressure, acc);
             }
         }
     }
@@ -389,8 +435,10 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
             });
             if (best == 0xffffffffu) break;
             last = best;
-            accumulate_contact(s, g, (int)best, bodyIn[best], mats[best], counts[best], fallback,
-                               ppx, ppy, vpx, vpy, ownMat, own, writePressure, acc);
+            bool otherSleeps = sleeping && sleep.bodyAwake[best] != stamp;
+            accumulate_contact(s, g, (int)best, bodyIn[best], mats[best], counts[best],
+                               otherSleeps, fallback, ppx, ppy, vpx, vpy, ownMat, own,
+                               writePressure, acc);
         }
     }
     if (s.bounded) {
@@ -417,7 +465,23 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     }
     o.z = vpx / s.vw;
     o.w = vpy / s.vh;
-    if (writePressure && !(s.liquid && ownMat.z == 0.0f)) pressures[g] = acc.pressure;
+    bool fluid = s.liquid && ownMat.z == 0.0f;
+    if (writePressure && s.sleepOn) {
+        // S/collision_cache_solve.comp:449-471: a body that moved this step,
+        // or that nothing holds, keeps its tile awake for the next one.
+        bool held = own > 0u || (s.bounded && (ppx < s.holdLo || ppy < s.holdLo ||
+                                               ppx > s.holdHiX || ppy > s.holdHiY));
+        if (fluid) held = true;
+        float dx = vpx * s.dt, dy = vpy * s.dt;
+        float d2 = dx * dx + dy * dy;
+        float m2 = acc.pcx * acc.pcx + acc.pcy * acc.pcy;
+        if (!held || d2 >= s.sleepDisp2 || m2 > s.settling2) {
+            sleep.tileActive[sleep_tile(s, o.x, o.y)] = stamp + 1u;
+            if (d2 >= s.tileMark2)
+                sleep.tileActive[sleep_tile(s, o.x + dx / s.vw, o.y + dy / s.vh)] = stamp + 1u;
+        }
+    }
+    if (writePressure && !fluid) pressures[g] = acc.pressure;
     // The next pass's drift (S/collision_cache_drift.comp:83-91; only the
     // first pass of a step kicks, so there is none here).
     if (nextDriftDt != 0.0f) {
@@ -459,10 +523,11 @@ void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b) {
 }
 
 void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
-                         float kickDt, float driftDt, int insert, const CollideGrid& grid) {
+                         const float4* mats, float kickDt, float driftDt, int insert,
+                         const CollideGrid& grid) {
     if (ls.bodyBlocks == 0) return;
     hipLaunchKernelGGL(k_drift_insert, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       body, acc, kickDt, driftDt, insert, grid_of(grid));
+                       body, acc, mats, kickDt, driftDt, insert, grid_of(grid), ls.sleep);
 }
 
 void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
@@ -471,7 +536,7 @@ void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid
     if (ls.bodyBlocks == 0) return;
     hipLaunchKernelGGL(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        grid_of(grid), grid_of(prev), clearPrev, mats, counts, contacts,
-                       ls.planeStride, ls.guardLimit);
+                       ls.planeStride, ls.guardLimit, ls.sleep);
 }
 
 void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, float4* bodyOut,
@@ -482,7 +547,7 @@ void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, f
     hipLaunchKernelGGL(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, bodyIn,
                        bodyOut, mats, counts, contacts, ls.planeStride, grid_of(grid),
                        ls.guardLimit, fallback, pressures, writePressure, nextDriftDt, insertNext,
-                       grid_of(next));
+                       grid_of(next), ls.sleep);
 }
 
 }  // namespace gs

@@ -74,7 +74,57 @@ struct SimDesc {
     float invPw, invPh; // 1 / pw, 1 / ph
     float gOverM;       // G / totalMass
     int pad2, pad3, pad4;
+    // Position-based liquid (K/LiquidKernel.kt, K/LiquidSolver.kt:64-101),
+    // px units; each float is the uniform the shaders receive.
+    float lh, lh2;          // kernel radius, and its square as the shaders form it
+    float lpoly6, lspiky;   // 4 / (pi h^8), -30 / (pi h^5)
+    float lrest, lrelax;    // rest density, relaxation
+    float lrowStep;         // d sqrt(3) / 2, the wall rows' spacing
+    float lwallScale;       // poly6 * 32 / 35 / d
+    float lpressureScale;   // h / dt
+    float lviscosity;       // viscosity / rest density
+    float lradius;          // d / 2, where the walls stop a body
+    int lgw, lgh;           // liquid grid, cells >= h
+    int liqCellBase;        // first liquid cell of this simulation
+    int lviscous;           // viscosity > 0: the XSPH pass runs
+    int lpad1;
+    // Sleeping tiles (S/sleep_layout.glsl, K/SleepTiles.kt:37-107)
+    int sleepOn;            // CollisionConfig.sleep with the contact passes on
+    int tilesX, tilesY;     // 32 px tiles, rounded up
+    int sleepTileBase;      // first tile of this simulation
+    float sleepDisp2;       // (sleepSpeedPixels * dt)^2
+    float holdLo;           // d/2 + 0.05 d: a body this close to a wall is held
+    float holdHiX, holdHiY; // viewport - d/2 - 0.05 d
+    float settling2;        // (0.005 d)^2: a pushed body is not at rest
+    float tileMark2;        // (16 px)^2: a faster body also wakes where it heads
+    int spad0, spad1;
 };
+
+// Per-simulation sleep state, advanced on the device at the end of a step
+// (K/SleepTiles.kt endStep): the step stamp (never 0) and the steps left in
+// which every body stays awake.
+struct SleepState { uint32_t stamp; int wakeSteps; int pad0, pad1; };
+constexpr int SLEEP_WAKE_STEPS = 2;  // K/SleepTiles.kt WAKE_STEPS
+
+struct SleepBufs {
+    SleepState* state;     // per simulation
+    uint32_t* bodyAwake;   // per body: the stamp of the step it is awake in
+    uint32_t* tileAwake;   // per tile: the stamp of the step it is awake in
+    uint32_t* tileActive;  // per tile: the stamp of the step it was marked for
+};
+
+// uSleep of the current step: tiles on and the wake steps over.
+__device__ __forceinline__ bool sleep_on(const SimDesc& s, const SleepState& st) {
+    return s.sleepOn && st.wakeSteps == 0;
+}
+
+// S/sleep_layout.glsl sleepTile, as a global tile index.
+__device__ __forceinline__ int sleep_tile(const SimDesc& s, float x, float y) {
+    float fx = floorf(x * (float)s.tilesX), fy = floorf(y * (float)s.tilesY);
+    int tx = fx >= 0.0f ? (fx < (float)(s.tilesX - 1) ? (int)fx : s.tilesX - 1) : 0;
+    int ty = fy >= 0.0f ? (fy < (float)(s.tilesY - 1) ? (int)fy : s.tilesY - 1) : 0;
+    return s.sleepTileBase + tx + ty * s.tilesX;
+}
 
 // Global work tables (built on the host whenever the batch shape changes).
 struct BodyBlock { int sim; int first; };

@@ -88,8 +88,12 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
             c.spatial_sort_interval != cfgs[0].spatial_sort_interval)
             throw ArgumentError("a batch shares dt, substeps and sort interval");
     }
+    for (const auto& c : cfgs)
+        for (const auto& o : cfgs)
+            if (c.liquid && o.liquid && c.liquid_iterations != o.liquid_iterations)
+                throw ArgumentError("liquid simulations of a batch share their iteration count");
     sims_.resize(cfgs.size());
-    int slot = 0, cells = 0, tiles = 0, phys = 0, mesh = 0;
+    int slot = 0, cells = 0, tiles = 0, phys = 0, mesh = 0, liq = 0, sleepTiles = 0;
     for (size_t i = 0; i < cfgs.size(); i++) {
         HostSim& h = sims_[i];
         h.cfg = cfgs[i];
@@ -120,6 +124,17 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
         h.phys = h.cfg.grid_width * h.cfg.grid_height;
         h.desc.physBase = phys;
         phys += h.phys;
+        // K/LiquidSolver.kt:45-46: liquid cells no smaller than the kernel radius
+        float lh = h.cfg.contact_diameter_pixels * h.cfg.kernel_radius_diameters;
+        h.desc.lgw = std::max(1, (int)(vw / lh));
+        h.desc.lgh = std::max(1, (int)(vh / lh));
+        h.desc.liqCellBase = liq;
+        liq += h.desc.lgw * h.desc.lgh;
+        // K/SleepTiles.kt:45-46: 32 px tiles, rounded up
+        h.desc.tilesX = (h.cfg.viewport_width + 31) / 32;
+        h.desc.tilesY = (h.cfg.viewport_height + 31) / 32;
+        h.desc.sleepTileBase = sleepTiles;
+        sleepTiles += h.desc.tilesX * h.desc.tilesY;
         bool wallsOk = 2 * std::max(h.cfg.grid_width, h.cfg.grid_height) <= 2048;
         h.meshMax = h.phys * (wallsOk ? 4 : 1);
         h.desc.meshBase = mesh;
@@ -130,6 +145,8 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     totalTiles_ = tiles;
     totalPhys_ = phys;
     totalMesh_ = mesh;
+    totalLiq_ = liq;
+    totalSleepTiles_ = sleepTiles;
 
     check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "stream");
     size_t S = (size_t)totalSlots_;
@@ -149,7 +166,8 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     keyOf_ = dalloc<uint32_t>(S);
     slotSrc_ = dalloc<uint32_t>(S);
     dest_ = dalloc<uint32_t>(S);
-    int scanMax = std::max(scan_totals_size(totalTiles_), scan_totals_size(totalPhys_));
+    int scanMax = std::max({scan_totals_size(totalTiles_), scan_totals_size(totalPhys_),
+                            scan_totals_size(totalLiq_)});
     scanTmp_ = dalloc<uint32_t>((size_t)scanMax);
     cellCount_ = dalloc<uint32_t>((size_t)totalPhys_ + 1);
     cellPrefix_ = dalloc<uint32_t>((size_t)totalPhys_ + 1);
@@ -157,6 +175,33 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     slotBody_ = dalloc<uint32_t>(S);
     cellRec_ = dalloc<uint4>((size_t)totalPhys_);
     cellBody_ = dalloc<float4>(S);
+    liqCount_ = dalloc<uint32_t>((size_t)totalLiq_ + 1);
+    liqPrefix_ = dalloc<uint32_t>((size_t)totalLiq_ + 1);
+    liqCellOf_ = dalloc<uint32_t>(S);
+    liqUnordered_ = dalloc<uint32_t>(S);
+    liqItems_ = dalloc<uint32_t>(S);
+    lambdas_ = dalloc<float>(S);
+    sleepState_ = dalloc<SleepState>(sims_.size());
+    bodyAwake_ = dalloc<uint32_t>(S);
+    tileAwake_ = dalloc<uint32_t>((size_t)std::max(totalSleepTiles_, 1));
+    tileActive_ = dalloc<uint32_t>((size_t)std::max(totalSleepTiles_, 1));
+    {
+        // K/SleepTiles.kt:56-60: stamp 1 (0 is never a stamp, so cleared
+        // tiles sleep), everything awake for the first steps.
+        std::vector<SleepState> st(sims_.size(), SleepState{1u, SLEEP_WAKE_STEPS, 0, 0});
+        check(hipMemcpy(sleepState_, st.data(), st.size() * sizeof(SleepState),
+                        hipMemcpyHostToDevice), "sleep state");
+        std::vector<BodyBlock> tb;
+        for (int i = 0; i < (int)sims_.size(); i++) {
+            int tiles = sims_[i].desc.tilesX * sims_[i].desc.tilesY;
+            for (int t = 0; t < tiles; t += WG) tb.push_back({i, t});
+        }
+        tileBlocks_ = (int)tb.size();
+        dTileBlocks_ = dalloc<BodyBlock>(std::max<size_t>(tb.size(), 1));
+        if (!tb.empty())
+            check(hipMemcpy(dTileBlocks_, tb.data(), tb.size() * sizeof(BodyBlock),
+                            hipMemcpyHostToDevice), "tile blocks");
+    }
     spec_ = dalloc<float2>((size_t)totalMesh_);
     green_ = dalloc<float2>((size_t)totalMesh_);
     tw_ = dalloc<float2>((size_t)twLen_);
@@ -210,6 +255,8 @@ Engine::~Engine() {
                     grid_[0].cellOf, grid_[1].cellOf, press_, counts_, contacts_, tileCount_,
                     tilePrefix_, keyOf_, slotSrc_, dest_, scanTmp_, cellCount_, cellPrefix_,
                     homeOf_, slotBody_, cellRec_, cellBody_, spec_, green_, tw_, fallback_,
+                    liqCount_, liqPrefix_, liqCellOf_, liqUnordered_, liqItems_, lambdas_,
+                    sleepState_, bodyAwake_, tileAwake_, tileActive_, dTileBlocks_,
                     dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -273,6 +320,32 @@ void Engine::fill_desc(HostSim& h) {
     d.invPw = 1.0f / (float)d.pw;
     d.invPh = 1.0f / (float)d.ph;
     d.gOverM = (float)((double)d.G / (double)d.totalMass);
+    // K/LiquidSolver.kt:64-101: the liquid passes' uniforms
+    LiquidConstants lk = liquid_constants(c.contact_diameter_pixels, c.kernel_radius_diameters);
+    d.lh = lk.h;
+    d.lh2 = lk.h * lk.h;
+    d.lpoly6 = lk.poly6;
+    d.lspiky = lk.spiky;
+    d.lrest = lk.rest;
+    d.lrelax = lk.relax;
+    d.lrowStep = c.contact_diameter_pixels * std::sqrt(3.0f) * 0.5f;
+    d.lwallScale = lk.poly6 * 32.0f / 35.0f / c.contact_diameter_pixels;
+    d.lpressureScale = lk.h / c.time_step;
+    d.lviscosity = c.liquid_viscosity / lk.rest;
+    d.lviscous = c.liquid_viscosity > 0.0f ? 1 : 0;
+    d.lradius = 0.5f * c.contact_diameter_pixels;
+    // K/SleepTiles.kt:51, 71-72 and S/collision_cache_solve.comp:452-466
+    d.sleepOn = c.sleep && c.collisions_enabled ? 1 : 0;
+    float disp = c.sleep_speed_pixels * c.time_step;
+    d.sleepDisp2 = disp * disp;
+    float radius = 0.5f * c.contact_diameter_pixels, slack = 0.05f * c.contact_diameter_pixels;
+    d.holdLo = radius + slack;
+    d.holdHiX = d.vw - radius - slack;
+    d.holdHiY = d.vh - radius - slack;
+    float settling = 0.005f * c.contact_diameter_pixels;
+    d.settling2 = settling * settling;
+    float mark = 0.5f * 32.0f;
+    d.tileMark2 = mark * mark;
 }
 
 void Engine::push_descs() {
@@ -304,12 +377,19 @@ void Engine::ensure_green(int s) {
 void Engine::rebuild_tables() {
     std::vector<BodyBlock> blocks;
     std::vector<LineWork> rowsF, rowsI, cols;
-    anyMutual_ = anyLiquid_ = anyCollide_ = false;
+    anyMutual_ = anyLiquid_ = anyCollide_ = anySleep_ = false;
     gravitySlots_ = 0;
+    liquidSlots_ = 0;
+    anyViscous_ = false;
     for (int s = 0; s < (int)sims_.size(); s++) {
         const HostSim& h = sims_[s];
         for (int i = 0; i < h.n; i += WG) blocks.push_back({s, h.base + i});
         anyLiquid_ |= h.liquid != 0;
+        anySleep_ |= h.desc.sleepOn != 0;
+        if (h.liquid) {
+            liquidSlots_ += h.n;
+            anyViscous_ |= h.cfg.liquid_viscosity > 0.0f;
+        }
         anyCollide_ |= h.cfg.collisions_enabled != 0;
         if (!h.mutual) continue;
         anyMutual_ = true;
@@ -341,6 +421,7 @@ void Engine::rebuild_tables() {
     shape_.bodyBlocks = bodyBlocks_;
     shape_.planeStride = totalSlots_;
     shape_.guardLimit = totalSlots_ + 1;
+    shape_.sleep = SleepBufs{sleepState_, bodyAwake_, tileAwake_, tileActive_};
     invalidate_graphs();
 }
 
@@ -550,12 +631,9 @@ void Engine::enqueue_force_and_kick(bool initial) {
         timed("fft_columns", [&] { launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_); });
         timed("fft_rows_inverse", [&] { launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_); });
     }
-    bool radius3 = true;
-    for (const auto& h : sims_)
-        if (h.mutual && (h.desc.rx != 3 || h.desc.ry != 3)) radius3 = false;
     timed("kick", [&] {
         launch_kick(st_, shape_, body_[bodyPar_], acc_[auxPar_], spec_, cellRec_, cellBody_,
-                    initial ? 1 : 0, radius3);
+                    initial ? 1 : 0);
     });
 }
 
@@ -563,7 +641,8 @@ void Engine::enqueue_force_and_kick(bool initial) {
 void Engine::enqueue_pass(float kickDt, float driftDt, bool writePressure) {
     int g = gridPar_;
     timed("drift_insert", [&] {
-        launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], kickDt, driftDt, 1, grid_[g]);
+        launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_], kickDt,
+                            driftDt, 1, grid_[g]);
     });
     timed("count", [&] {
         launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
@@ -586,7 +665,8 @@ void Engine::enqueue_step(bool sortDue) {
     const int S = h0.substeps;
     if (anyCollide_) {
         timed("drift_insert", [&] {
-            launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], dt, sub, 1, grid_[gridPar_]);
+            launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_], dt, sub, 1,
+                                grid_[gridPar_]);
         });
         for (int k = 0; k < S; k++) {
             int g = gridPar_;
@@ -605,8 +685,10 @@ void Engine::enqueue_step(bool sortDue) {
             gridPar_ ^= 1;
         }
         if (anyLiquid_) {
+            enqueue_liquid();
             timed("insert", [&] {
-                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], 0.0f, 0.0f, 1, grid_[gridPar_]);
+                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_], 0.0f,
+                                    0.0f, 1, grid_[gridPar_]);
             });
         }
         enqueue_force_and_kick(false);
@@ -622,16 +704,67 @@ void Engine::enqueue_step(bool sortDue) {
         });
         bodyPar_ ^= 1;
         gridPar_ ^= 1;
+        // K/GravityPipeline.kt:487-490: the tiles awake in the next step.
+        if (anySleep_) {
+            timed("sleep_tiles", [&] {
+                launch_sleep_advance(st_, dSims_, (int)sims_.size(), dTileBlocks_, tileBlocks_,
+                                     shape_.sleep);
+            });
+        }
     } else {
         // Gravity-only configurations: kick-drift-kick with plain drifts.
         for (int k = 0; k < S; k++) {
             timed("drift", [&] {
-                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_],
+                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_],
                                     k == 0 ? dt : 0.0f, sub, 0, grid_[gridPar_]);
             });
         }
         enqueue_force_and_kick(false);
     }
+}
+
+// K/LiquidSolver.kt:107-173: start cells, `iterations` x (lambda, apply),
+// then the viscosity pass; the bodies ping-pong at every apply.
+void Engine::enqueue_liquid() {
+    timed("liquid_bin", [&] {
+        launch_liquid_bin(st_, shape_, body_[bodyPar_], liqCount_, liqCellOf_);
+    });
+    timed("liquid_scan", [&] { launch_scan(st_, liqCount_, liqPrefix_, totalLiq_, scanTmp_); });
+    timed("liquid_scatter", [&] {
+        launch_liquid_scatter(st_, shape_, liqCellOf_, liqCount_, liqPrefix_, liqUnordered_);
+    });
+    timed("liquid_canon", [&] {
+        launch_liquid_canon(st_, liquidSlots_, liqCellOf_, liqPrefix_, liqUnordered_, liqItems_);
+    });
+    int iterations = 0;
+    for (const auto& h : sims_)
+        if (h.liquid) iterations = std::max(iterations, h.cfg.liquid_iterations);
+    for (int it = 0; it < iterations; it++) {
+        timed("liquid_lambda", [&] {
+            launch_liquid_lambda(st_, shape_, body_[bodyPar_], mat_[auxPar_], liqCellOf_,
+                                 liqPrefix_, liqItems_, fallback_, lambdas_, press_,
+                                 it == iterations - 1 ? 1 : 0);
+        });
+        timed("liquid_apply", [&] {
+            launch_liquid_apply(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1], mat_[auxPar_],
+                                liqCellOf_, liqPrefix_, liqItems_, fallback_, lambdas_);
+        });
+        bodyPar_ ^= 1;
+    }
+    if (anyViscous_) {
+        timed("liquid_viscosity", [&] {
+            launch_liquid_viscosity(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
+                                    mat_[auxPar_], liqCellOf_, liqPrefix_, liqItems_);
+        });
+        bodyPar_ ^= 1;
+    }
+}
+
+void Engine::debug_liquid() {
+    normalize();
+    if (bodyBlocks_ == 0 || !anyLiquid_) return;
+    enqueue_liquid();
+    check(hipGetLastError(), "liquid");
 }
 
 void Engine::initialize_acceleration() {
@@ -640,8 +773,23 @@ void Engine::initialize_acceleration() {
     check(hipGetLastError(), "initialize acceleration");
 }
 
+void Engine::wake(int s) {
+    launch_sleep_wake(st_, shape_.sleep, s, SLEEP_WAKE_STEPS);
+    check(hipGetLastError(), "wake");
+}
+
 void Engine::step() {
     normalize();
+    // K/GravityPipeline.kt:446-452: new bodies, walls, field, gravity or
+    // liquid wake everything for a while (nothing has marked the tiles).
+    for (int i = 0; i < (int)sims_.size(); i++) {
+        HostSim& h = sims_[i];
+        SleepScene scene{h.n, h.bounded, h.fieldX, h.fieldY, h.mutual, h.liquid};
+        if (!(scene == h.scene)) {
+            h.scene = scene;
+            if (h.desc.sleepOn) wake(i);
+        }
+    }
     const HostSim& h0 = sims_[0];
     int interval = h0.cfg.spatial_sort_interval;
     bool sortDue = false;

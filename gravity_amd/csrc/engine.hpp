@@ -20,6 +20,17 @@ namespace gs {
 struct ArgumentError : std::runtime_error { using std::runtime_error::runtime_error; };
 struct StateError : std::runtime_error { using std::runtime_error::runtime_error; };
 
+// What wakes the sleeping tiles when it changes between steps.
+struct SleepScene {
+    int n = -1, bounded = 0;
+    float fieldX = 0.0f, fieldY = 0.0f;
+    int mutual = 0, liquid = 0;
+    bool operator==(const SleepScene& o) const {
+        return n == o.n && bounded == o.bounded && fieldX == o.fieldX && fieldY == o.fieldY &&
+               mutual == o.mutual && liquid == o.liquid;
+    }
+};
+
 struct HostSim {
     gs_config cfg;
     int n = 0;          // active bodies
@@ -34,6 +45,7 @@ struct HostSim {
     int phys = 0;                         // physical P3M cells
     int meshMax = 0;                      // mesh nodes reserved (walls double)
     SimDesc desc{};
+    SleepScene scene;                     // of the last step
 };
 
 class Engine {
@@ -63,8 +75,10 @@ public:
     void initialize_acceleration();
     void step();
     void stop_bodies(int s);
+    void wake(int s);
     void debug_collide(float kickDt, float driftDt, bool writePressure);
     void debug_sort();
+    void debug_liquid();
 
     // Eager steps with an event pair around every launch.
     std::vector<std::tuple<std::string, int, double>> profile(int steps);
@@ -82,6 +96,7 @@ private:
     void enqueue_sort();
     void enqueue_force_and_kick(bool initial);
     void enqueue_pass(float kickDt, float driftDt, bool writePressure);
+    void enqueue_liquid();
     void clear_grids();
     void invalidate_graphs();
     template <typename F> void timed(const char* name, F&& f);
@@ -89,7 +104,8 @@ private:
     std::vector<HostSim> sims_;
     hipStream_t st_ = nullptr;
     int totalSlots_ = 0;      // body slots (sum of 256-rounded capacities)
-    int totalCells_ = 0, totalTiles_ = 0, totalPhys_ = 0, totalMesh_ = 0;
+    int totalCells_ = 0, totalTiles_ = 0, totalPhys_ = 0, totalMesh_ = 0, totalLiq_ = 0;
+    int totalSleepTiles_ = 0;
     int twLen_ = 2048;
     bool useGraphs_ = true;
 
@@ -110,14 +126,24 @@ private:
     uint32_t* slotBody_ = nullptr;
     uint4* cellRec_ = nullptr;  // per physical cell: first slot, count, mass / 256, 0
     float4* cellBody_ = nullptr;
+    // liquid: bodies counting-sorted by their start cell, ascending per cell
+    uint32_t *liqCount_ = nullptr, *liqPrefix_ = nullptr, *liqCellOf_ = nullptr;
+    uint32_t *liqUnordered_ = nullptr, *liqItems_ = nullptr;
+    float* lambdas_ = nullptr;
+    // sleeping tiles
+    SleepState* sleepState_ = nullptr;
+    uint32_t *bodyAwake_ = nullptr, *tileAwake_ = nullptr, *tileActive_ = nullptr;
+    BodyBlock* dTileBlocks_ = nullptr;
+    int tileBlocks_ = 0;
     float2 *spec_ = nullptr, *green_ = nullptr, *tw_ = nullptr, *fallback_ = nullptr;
     std::vector<GreenKey> greenKeys_;
     // tables
     SimDesc* dSims_ = nullptr;
     BodyBlock* dBlocks_ = nullptr;
     LineWork *dRowsF_ = nullptr, *dCols_ = nullptr, *dRowsI_ = nullptr;
-    int bodyBlocks_ = 0, rowsF_ = 0, cols_ = 0, rowsI_ = 0, gravitySlots_ = 0;
-    bool anyMutual_ = false, anyLiquid_ = false, anyCollide_ = false;
+    int bodyBlocks_ = 0, rowsF_ = 0, cols_ = 0, rowsI_ = 0, gravitySlots_ = 0, liquidSlots_ = 0;
+    bool anyMutual_ = false, anyLiquid_ = false, anyCollide_ = false, anyViscous_ = false;
+    bool anySleep_ = false;
     LaunchShape shape_{};
 
     // graphs keyed by (bodyPar, auxPar, gridPar, sortDue), with the parities

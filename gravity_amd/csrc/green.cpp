@@ -114,4 +114,47 @@ void fallback_normals(std::vector<float>& out) {
     }
 }
 
+LiquidConstants liquid_constants(float diameter, float radiusDiameters) {
+    // K/LiquidKernel.kt:14-60: 2D poly6 and spiky coefficients at the kernel
+    // radius, the rest density as the poly6 sum over the hexagonal packing of
+    // bodies one diameter apart (the body itself included), and the
+    // relaxation as 1% of that packing's squared spiky-gradient sum.
+    LiquidConstants k{};
+    const float h = diameter * radiusDiameters;
+    const double hd = (double)h;
+    const double h5 = hd * hd * hd * hd * hd;
+    k.h = h;
+    k.poly6 = (float)(4.0 / (M_PI * (h5 * hd * hd * hd)));
+    k.spiky = (float)(-30.0 / (M_PI * h5));
+    const double d = (double)diameter;
+    const double rowStep = d * std::sqrt(3.0) * 0.5;
+    const int rows = (int)std::ceil(hd / rowStep) + 1;
+    const int columns = (int)std::ceil(hd / d) + 1;
+    auto each = [&](auto&& f) {
+        for (int r = -rows; r <= rows; r++) {
+            double shift = (r & 1) == 0 ? 0.0 : 0.5 * d;
+            for (int c = -columns; c <= columns; c++) f(c * d + shift, r * rowStep);
+        }
+    };
+    double rest = 0.0;
+    each([&](double x, double y) {
+        double q = hd * (double)h - (x * x + y * y);
+        rest += q > 0.0 ? (double)k.poly6 * q * q * q : 0.0;
+    });
+    k.rest = (float)rest;
+    double relax = 0.0;
+    each([&](double x, double y) {
+        double r = std::sqrt(x * x + y * y);
+        double g = 0.0;
+        if (r > 0.0 && r < hd) {
+            double q = hd - r;
+            g = -(double)k.spiky * q * q;
+        }
+        g /= (double)k.rest;
+        relax += g * g;
+    });
+    k.relax = (float)(0.01 * relax);
+    return k;
+}
+
 }  // namespace gs

@@ -1,5 +1,5 @@
-// Host-built tables of the step: force-kernel spectrum, FFT twiddles and
-// the coincident-pair normals.
+// Host-built tables of the step: force-kernel spectrum, FFT twiddles, the
+// coincident-pair normals and the liquid kernel constants.
 #pragma once
 
 #include <vector>
@@ -21,5 +21,15 @@ void green_spectrum(const GreenKey& key, std::vector<float>& out);
 void fft_twiddles(int length, std::vector<float>& out);
 // cos/sin of the 65536 fallback angles, interleaved.
 void fallback_normals(std::vector<float>& out);
+
+// Position-based-fluid kernel constants in pixels (K/LiquidKernel.kt).
+struct LiquidConstants {
+    float h;       // kernel radius
+    float poly6;   // 4 / (pi h^8)
+    float spiky;   // -30 / (pi h^5)
+    float rest;    // rest density of the hexagonal packing
+    float relax;   // constraint relaxation
+};
+LiquidConstants liquid_constants(float diameter, float radiusDiameters);
 
 }  // namespace gs

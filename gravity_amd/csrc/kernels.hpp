@@ -19,6 +19,7 @@ struct LaunchShape {
     int bodyBlocks;      // workgroups over all active bodies
     int planeStride;     // body slots of the batch (stride of the contact planes)
     int guardLimit;      // bound on any linked-list walk
+    SleepBufs sleep;     // sleeping tiles of the batch
 };
 
 // Mesh work item: {simulation, first line, lines}.
@@ -44,7 +45,8 @@ void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* ke
 // ---- collisions ----
 void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b);
 void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
-                         float kickDt, float driftDt, int insert, const CollideGrid& grid);
+                         const float4* mats, float kickDt, float driftDt, int insert,
+                         const CollideGrid& grid);
 void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
                   const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
                   uint32_t* contacts);
@@ -70,9 +72,30 @@ void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* wor
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              float2* spec, const float2* twiddles, int twiddleLength);
 // cellRec: per physical cell {first slot, body count, mass in 1/256, 0}.
-// radius3: every simulation of the batch has a 3-cell near field (rx = ry = 3).
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
-                 const float2* force, const uint4* cellRec, const float4* cellBody, int initial,
-                 bool radius3);
+                 const float2* force, const uint4* cellRec, const float4* cellBody, int initial);
+
+// ---- sleeping tiles: wake the next step's tiles, then advance the stamps ----
+void launch_sleep_advance(hipStream_t st, const SimDesc* sims, int simCount,
+                          const BodyBlock* tileBlocks, int tileBlockCount, const SleepBufs& sleep);
+void launch_sleep_wake(hipStream_t st, const SleepBufs& sleep, int sim, int steps);
+
+// ---- position-based liquid ----
+void launch_liquid_bin(hipStream_t st, const LaunchShape& ls, const float4* body,
+                       uint32_t* liqCount, uint32_t* cellOf);
+void launch_liquid_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* cellOf,
+                           uint32_t* liqCount, const uint32_t* prefix, uint32_t* unordered);
+void launch_liquid_canon(hipStream_t st, int slots, const uint32_t* cellOf,
+                         const uint32_t* prefix, const uint32_t* unordered, uint32_t* items);
+void launch_liquid_lambda(hipStream_t st, const LaunchShape& ls, const float4* body,
+                          const float4* mats, const uint32_t* cellOf, const uint32_t* prefix,
+                          const uint32_t* items, const float2* fallback, float* lambdas,
+                          float* pressures, int writePressure);
+void launch_liquid_apply(hipStream_t st, const LaunchShape& ls, const float4* body, float4* out,
+                         const float4* mats, const uint32_t* cellOf, const uint32_t* prefix,
+                         const uint32_t* items, const float2* fallback, const float* lambdas);
+void launch_liquid_viscosity(hipStream_t st, const LaunchShape& ls, const float4* body,
+                             float4* out, const float4* mats, const uint32_t* cellOf,
+                             const uint32_t* prefix, const uint32_t* items);
 
 }  // namespace gs

@@ -88,6 +88,7 @@ __device__ __forceinline__ float min_image(const SimDesc& s, float d) {
 // Gaussian weight and its mass: G m w / (M max(r2, minDist2)) * (dx, dy).
 __device__ __forceinline__ void near_term(const SimDesc& s, float px, float py, float r2,
                                           float weight, float mass, float& ax, float& ay) {
+#pragma clang fp contract(fast)
     float q = s.gOverM * mass * weight * __builtin_amdgcn_rcpf(fmaxf(r2, s.minDist2));
     ax += q * (px * s.invAspect);
     ay += q * py;
@@ -98,6 +99,7 @@ __device__ __forceinline__ void near_term(const SimDesc& s, float px, float py, 
 __device__ __forceinline__ void exact_cell(const SimDesc& s, const float4* __restrict__ cellBody,
                                            int first, int count, int g, float x, float y, float k2,
                                            float& ax, float& ay) {
+#pragma clang fp contract(fast)
     for (int t = first; t < first + count; t++) {
         float4 o = cellBody[t];
         if (__float_as_uint(o.w) == (uint32_t)g) continue;
@@ -124,32 +126,127 @@ __device__ __forceinline__ float2 mesh_force(const SimDesc& s, const float2* __r
                        mixf(mixf(v00.y, v10.y, fx), mixf(v01.y, v11.y, fx), fy));
 }
 
-// S/kick.comp:105-156 (sleep handled by the caller's schedule).
-//
-// The near field is the reference's sum over the (2rx+1) x (2ry+1)
+// The kick's near field (S/kick.comp:118-151) is the reference's sum over
+// the (2rx+1) x (2ry+1)
 // physical cells around the home cell: the home cell body by body (at most
 // maximumHomeBodyInteractions list entries, the body itself skipped), cells
 // with more than exactNeighborBodyLimit bodies as a monopole at the cell
-// centre, the others body by body. It is evaluated on the FP32-tolerance
-// path (contraction allowed, reciprocal products, hardware exp2), and a
-// monopole's Gaussian is the product of its column and row factors.
-//
-// R > 0 is a near field of rx = ry = R cells (the default configurations):
-// the column offsets and factors are computed once per body and each cell
-// is one 16-byte record load. R = 0 takes any radius, a row at a time.
+// centre, the others body by body. The near field is evaluated on the
+// FP32-tolerance path (contraction allowed, reciprocal products, hardware
+// exp2), and a monopole's Gaussian is the product of its column and row
+// factors; the mesh gather, the field and the half kick round as written.
+
+// Near field of rx = ry = R cells: the column offsets and Gaussian factors
+// of a body are computed once, and each row's records are loaded together.
 template <int R>
+__device__ __forceinline__ void near_field_fixed(const SimDesc& s,
+                                                 const uint4* __restrict__ cellRec,
+                                                 const float4* __restrict__ cellBody, int g,
+                                                 float x, float y, float& ax, float& ay) {
+#pragma clang fp contract(fast)
+    const float k2 = s.invTwoSigma2;
+    const float cellMassUnit = 1.0f / 256.0f;
+    int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
+    constexpr int D = 2 * R + 1;
+    float cpx[D], cex[D];
+    int cwx[D];
+    uint32_t colOk = 0u;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        int cx = hx + k - R;
+        if (!s.bounded || (cx >= 0 && cx < s.pw)) colOk |= 1u << k;
+        int wx = wrap_axis(cx, s.pw);
+        cwx[k] = wx;
+        float px = min_image(s, ((float)wx + 0.5f) * s.invPw - x) * s.aspect;
+        cpx[k] = px;
+        cex[k] = __expf(-(px * px) * k2);
+    }
+#pragma unroll 1
+    for (int j = 0; j < D; j++) {
+        int cy = hy + j - R;
+        if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
+        int wy = wrap_axis(cy, s.ph);
+        float py = min_image(s, ((float)wy + 0.5f) * s.invPh - y);
+        float py2 = py * py;
+        float ey = __expf(-py2 * k2);
+        const uint4* row = cellRec + (s.physBase + wy * s.pw);
+        uint4 rec[D];
+#pragma unroll
+        for (int k = 0; k < D; k++)
+            rec[k] = (colOk >> k) & 1u ? row[cwx[k]] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            int count = (int)rec[k].y;
+            bool home = j == R && k == R;
+            if (!home && count > s.exactLimit) {
+                float px = cpx[k];
+                near_term(s, px, py, px * px + py2, cex[k] * ey, (float)rec[k].z * cellMassUnit,
+                          ax, ay);
+            } else if (count > 0) {
+                int limit = home ? s.homeLimit : s.exactLimit;
+                exact_cell(s, cellBody, (int)rec[k].x, count < limit ? count : limit, g, x, y, k2,
+                           ax, ay);
+            }
+        }
+    }
+}
+
+// Near field of any radius, a cell at a time.
+__device__ __forceinline__ void near_field_any(const SimDesc& s, const uint4* __restrict__ cellRec,
+                                               const float4* __restrict__ cellBody, int g,
+                                               float x, float y, float& ax, float& ay) {
+#pragma clang fp contract(fast)
+    const float k2 = s.invTwoSigma2;
+    const float cellMassUnit = 1.0f / 256.0f;
+    int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
+    for (int dy = -s.ry; dy <= s.ry; dy++) {
+        int cy = hy + dy;
+        if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
+        int wy = wrap_axis(cy, s.ph);
+        float py = min_image(s, ((float)wy + 0.5f) * s.invPh - y);
+        float py2 = py * py;
+        float ey = __expf(-py2 * k2);
+        const uint4* row = cellRec + (s.physBase + wy * s.pw);
+        for (int dx = -s.rx; dx <= s.rx; dx++) {
+            int cx = hx + dx;
+            if (s.bounded && (cx < 0 || cx >= s.pw)) continue;
+            int wx = wrap_axis(cx, s.pw);
+            uint4 rec = row[wx];
+            int count = (int)rec.y;
+            bool home = dx == 0 && dy == 0;
+            if (!home && count > s.exactLimit) {
+                float px = min_image(s, ((float)wx + 0.5f) * s.invPw - x) * s.aspect;
+                near_term(s, px, py, px * px + py2, __expf(-(px * px) * k2) * ey,
+                          (float)rec.z * cellMassUnit, ax, ay);
+            } else if (count > 0) {
+                int limit = home ? s.homeLimit : s.exactLimit;
+                exact_cell(s, cellBody, (int)rec.x, count < limit ? count : limit, g, x, y, k2,
+                           ax, ay);
+            }
+        }
+    }
+}
+
+// S/kick.comp:105-156. A simulation with the default 3-cell near field
+// takes near_field_fixed<3>, any other near_field_any; the choice is per
+// simulation, so a simulation kicks the same alone or in a batch.
 __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
                                              const BodyBlock* __restrict__ blocks,
                                              float4* __restrict__ body,
                                              float2* __restrict__ acc,
                                              const float2* __restrict__ force,
                                              const uint4* __restrict__ cellRec,
-                                             const float4* __restrict__ cellBody, int initial) {
-#pragma clang fp contract(fast)
+                                             const float4* __restrict__ cellBody, int initial,
+                                             SleepBufs sleep) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
+    // S/kick.comp:109-110: a sleeping body is not kicked.
+    if (!initial && s.sleepOn) {
+        const SleepState st = sleep.state[b.sim];
+        if (sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp) return;
+    }
     float h = 0.5f * s.dt;
     float4 p = body[g];
     if (!s.mutual) {
@@ -163,79 +260,10 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
     }
     float2 mf = mesh_force(s, force, p.x, p.y);
     float ax = mf.x + s.fieldX, ay = mf.y + s.fieldY;
-    const float k2 = s.invTwoSigma2;
-    const float cellMassUnit = 1.0f / 256.0f;
-    int hx = (int)floorf(p.x * (float)s.pw), hy = (int)floorf(p.y * (float)s.ph);
-    if constexpr (R > 0) {
-        constexpr int D = 2 * R + 1;
-        float cpx[D], cex[D];
-        int cwx[D];
-        uint32_t colOk = 0u;
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            int cx = hx + k - R;
-            if (!s.bounded || (cx >= 0 && cx < s.pw)) colOk |= 1u << k;
-            int wx = wrap_axis(cx, s.pw);
-            cwx[k] = wx;
-            float px = min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
-            cpx[k] = px;
-            cex[k] = __expf(-(px * px) * k2);
-        }
-#pragma unroll 1
-        for (int j = 0; j < D; j++) {
-            int cy = hy + j - R;
-            if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
-            int wy = wrap_axis(cy, s.ph);
-            float py = min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
-            float py2 = py * py;
-            float ey = __expf(-py2 * k2);
-            const uint4* row = cellRec + (s.physBase + wy * s.pw);
-            uint4 rec[D];
-#pragma unroll
-            for (int k = 0; k < D; k++)
-                rec[k] = (colOk >> k) & 1u ? row[cwx[k]] : make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                int count = (int)rec[k].y;
-                bool home = j == R && k == R;
-                if (!home && count > s.exactLimit) {
-                    float px = cpx[k];
-                    near_term(s, px, py, px * px + py2, cex[k] * ey,
-                              (float)rec[k].z * cellMassUnit, ax, ay);
-                } else if (count > 0) {
-                    int limit = home ? s.homeLimit : s.exactLimit;
-                    exact_cell(s, cellBody, (int)rec[k].x, count < limit ? count : limit, g, p.x,
-                               p.y, k2, ax, ay);
-                }
-            }
-        }
-    } else {
-        for (int dy = -s.ry; dy <= s.ry; dy++) {
-            int cy = hy + dy;
-            if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
-            int wy = wrap_axis(cy, s.ph);
-            float py = min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
-            float ey = __expf(-(py * py) * k2);
-            const uint4* row = cellRec + (s.physBase + wy * s.pw);
-            for (int dx = -s.rx; dx <= s.rx; dx++) {
-                int cx = hx + dx;
-                if (s.bounded && (cx < 0 || cx >= s.pw)) continue;
-                int wx = wrap_axis(cx, s.pw);
-                uint4 rec = row[wx];
-                int count = (int)rec.y;
-                bool home = dx == 0 && dy == 0;
-                if (!home && count > s.exactLimit) {
-                    float px = min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
-                    near_term(s, px, py, px * px + py * py, __expf(-(px * px) * k2) * ey,
-                              (float)rec.z * cellMassUnit, ax, ay);
-                } else if (count > 0) {
-                    int limit = home ? s.homeLimit : s.exactLimit;
-                    exact_cell(s, cellBody, (int)rec.x, count < limit ? count : limit, g, p.x, p.y,
-                               k2, ax, ay);
-                }
-            }
-        }
-    }
+    if (s.rx == 3 && s.ry == 3)
+        near_field_fixed<3>(s, cellRec, cellBody, g, p.x, p.y, ax, ay);
+    else
+        near_field_any(s, cellRec, cellBody, g, p.x, p.y, ax, ay);
     if (!initial) {
         p.z += ax * h;
         p.w += ay * h;
@@ -267,15 +295,10 @@ void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
 }
 
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
-                 const float2* force, const uint4* cellRec, const float4* cellBody, int initial,
-                 bool radius3) {
+                 const float2* force, const uint4* cellRec, const float4* cellBody, int initial) {
     if (ls.bodyBlocks == 0) return;
-    if (radius3)
-        hipLaunchKernelGGL(k_kick<3>, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                           body, acc, force, cellRec, cellBody, initial);
-    else
-        hipLaunchKernelGGL(k_kick<0>, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                           body, acc, force, cellRec, cellBody, initial);
+    hipLaunchKernelGGL(k_kick, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
+                       force, cellRec, cellBody, initial, ls.sleep);
 }
 
 }  // namespace gs

@@ -90,6 +90,14 @@ SIGNATURES = {
     "gs_query": (_I, [_P, ctypes.POINTER(_I)]),
     "gs_profile_steps": (_I, [_P, _I, ctypes.POINTER(GsKernelStat), _I, ctypes.POINTER(_I)]),
     "gs_set_graphs": (_I, [_P, _I]),
+    "gs_batch_create": (_I, [ctypes.POINTER(GsConfig), _I, ctypes.POINTER(_P), ctypes.POINTER(_I),
+                             ctypes.POINTER(_P)]),
+    "gs_batch_size": (_I, [_P]),
+    "gs_batch_set_active": (_I, [_P, _I, _I]),
+    "gs_batch_upload": (_I, [_P, _I, _I, _I, _P, _P]),
+    "gs_batch_download": (_I, [_P, _I, _I, _I, _I, _P]),
+    "gs_batch_initialize_acceleration": (_I, [_P]),
+    "gs_batch_step": (_I, [_P]),
     "gs_last_error": (ctypes.c_char_p, []),
 }
 

@@ -268,6 +268,11 @@ class GravityPipeline:
     def sortBodies(self, activeBodies: int) -> None:
         native.check(self._lib.gs_debug_sort(self._h, int(activeBodies)))
 
+    def resolveLiquid(self, activeBodies: int) -> None:
+        """The liquid solve of a step on its own (the reflection hook
+        resolveLiquid, T/CollisionGpuSupport.kt:68-90)."""
+        native.check(self._lib.gs_debug_resolve_liquid(self._h, int(activeBodies)))
+
     def setGraphs(self, enabled: bool) -> None:
         native.check(self._lib.gs_set_graphs(self._h, int(bool(enabled))))
 

@@ -141,3 +141,19 @@ def targets(n: int, seed: int) -> np.ndarray:
     """Seeded uniform target positions (normalised) of the adjoint loss."""
     rng = np.random.default_rng(seed)
     return rng.uniform(0.0, 1.0, (n, 2)).astype(np.float32)
+
+
+def ensemble(count: int = 64, first_seed: int = 100, width: int = 1024, height: int = 1024):
+    """The batched-ensemble workload: `count` independent torus simulations,
+    seeds first_seed.., each with N = floor(10^U(3, 5)) bodies in a Plummer
+    sphere (a = 50 px, virial velocities) and a square P3M grid of
+    2^U{6..9} cells a side. Returns (N, grid, bodies, materials) per member."""
+    out = []
+    for k in range(count):
+        seed = first_seed + k
+        rng = np.random.default_rng(seed)
+        n = int(math.floor(10.0 ** rng.uniform(3.0, 5.0)))
+        grid = int(2 ** rng.integers(6, 10))
+        bodies, mats = plummer(n, width, height, a_px=50.0, seed=seed)
+        out.append((n, grid, bodies, mats))
+    return out

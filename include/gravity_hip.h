@@ -168,6 +168,29 @@ int gs_profile_steps(gs_sim *sim, int steps, gs_kernel_stat *out, int capacity, 
  * every kernel eagerly. */
 int gs_set_graphs(gs_sim *sim, int enabled);
 
+/* Batched ensembles. No reference counterpart: the reference runs one
+ * simulation per GravityPipeline. A batch handle holds `count` independent
+ * simulations, each with its own configuration, viewport, body count and
+ * P3M grid (they share time_step, maximum_substeps and
+ * spatial_sort_interval), and steps all of them with one launch sequence
+ * (one captured HIP graph): every kernel walks a table of the batch's body
+ * blocks, mesh lines or tiles. Simulation i holds scenarios[i] (nullable)
+ * of n_scenario[i] bodies. gs_* calls without an index act on simulation 0;
+ * gs_step / gs_initialize_acceleration set simulation 0's active count and
+ * advance the whole batch. */
+int gs_batch_create(const gs_config *cfgs, int count, const float *const *scenarios,
+                    const int *n_scenario, gs_sim **out);
+/* Number of simulations in the handle (a plain handle holds 1). */
+int gs_batch_size(gs_sim *sim);
+/* Active bodies of simulation `index` from the next step on. */
+int gs_batch_set_active(gs_sim *sim, int index, int active);
+int gs_batch_upload(gs_sim *sim, int index, int first, int count, const float *xyvv,
+                    const float *materials);
+int gs_batch_download(gs_sim *sim, int index, int what, int first, int count, void *dst);
+/* initializeAcceleration and step of every simulation, with their active counts. */
+int gs_batch_initialize_acceleration(gs_sim *sim);
+int gs_batch_step(gs_sim *sim);
+
 /* Message of the last failure on this thread, or "" (Kotlin exception text). */
 const char *gs_last_error(void);
 

@@ -275,8 +275,8 @@ static void wall_contact(const orc_params *p, float distance, float nx, float ny
 static void accumulate_contact(const orc_params *p, uint32_t id, uint32_t other, const float *in,
                                const float *materials, const uint32_t *counts, float ppx, float ppy,
                                float vpx, float vpy, uint32_t own_contacts, float eff_dt,
-                               float *pcx, float *pcy, float *vcx, float *vcy, float *pressure,
-                               int write_pressure) {
+                               int other_sleeps, float *pcx, float *pcy, float *vcx, float *vcy,
+                               float *pressure, int write_pressure) {
     float vw = (float)p->width, vh = (float)p->height;
     if (!hard_pair(p, materials, (int)id, (int)other)) return;
     const float *o = in + 4 * other;
@@ -299,7 +299,9 @@ static void accumulate_contact(const orc_params *p, uint32_t id, uint32_t other,
     uint32_t crowd = own_contacts > oc ? own_contacts : oc;
     float pair_weight = 1.0f / (float)(crowd > 1u ? crowd : 1u);
     float m_own = materials[4 * id], m_other = materials[4 * other];
-    float share = m_other / (m_own + m_other);
+    /* A sleeping body stays where it is: this side takes the whole correction
+     * (S/collision_cache_solve.comp:141-142,201). */
+    float share = other_sleeps ? 1.0f : m_other / (m_own + m_other);
     float ovx = o[2] * vw, ovy = o[3] * vh;
     float rvx = vpx - ovx, rvy = vpy - ovy;
     float rvn = rvx * nx + rvy * ny;
@@ -328,22 +330,55 @@ static void accumulate_contact(const orc_params *p, uint32_t id, uint32_t other,
     }
 }
 
-/* S/collision_cache_solve.comp:274-478 (non-RIGID, sleep off). */
-void orc_solve(const orc_params *p, int n, const float *in, float *out, const float *materials,
-               const uint32_t *counts, const uint32_t *lists, int cap, float *pressures,
-               int write_pressure) {
+/* Sleeping tiles of a simulation (S/sleep_layout.glsl, K/SleepTiles.kt:37-107). */
+typedef struct {
+    int sleeping;             /* uSleep: enabled and past the wake steps */
+    int enabled;              /* the final pass marks tiles */
+    uint32_t stamp;
+    int tiles_x, tiles_y;
+    uint32_t *body_awake, *tile_awake, *tile_active;
+    float disp2;              /* (sleepSpeedPixels * dt)^2 */
+} sleep_ctx;
+
+#define SLEEP_TILE_PIXELS 32
+#define SLEEP_WAKE_STEPS 2
+
+/* S/sleep_layout.glsl sleepTile */
+static int sleep_tile(const sleep_ctx *sl, float x, float y) {
+    float fx = floorf(x * (float)sl->tiles_x), fy = floorf(y * (float)sl->tiles_y);
+    int tx = fx >= 0.0f ? (fx < (float)(sl->tiles_x - 1) ? (int)fx : sl->tiles_x - 1) : 0;
+    int ty = fy >= 0.0f ? (fy < (float)(sl->tiles_y - 1) ? (int)fy : sl->tiles_y - 1) : 0;
+    return tx + ty * sl->tiles_x;
+}
+
+static int sleep_awake(const sleep_ctx *sl, int i) {
+    return !sl || !sl->sleeping || sl->body_awake[i] == sl->stamp;
+}
+
+/* S/collision_cache_solve.comp:274-478 (non-RIGID); sl (nullable) adds the
+ * sleeping tiles: sleeping bodies are carried over, a sleeping partner takes
+ * no share, and the final pass marks the tiles of bodies not at rest. */
+static void solve_impl(const orc_params *p, int n, const float *in, float *out,
+                       const float *materials, const uint32_t *counts, const uint32_t *lists,
+                       int cap, float *pressures, int write_pressure, const sleep_ctx *sl) {
     float vw = (float)p->width, vh = (float)p->height;
     float eff_dt = p->dt / (float)p->substeps;
     for (int i = 0; i < n; i++) {
         const float *b = in + 4 * i;
+        if (!sleep_awake(sl, i)) {
+            memcpy(out + 4 * i, b, 16);
+            continue;
+        }
         float ppx = b[0] * vw, ppy = b[1] * vh;
         float vpx = b[2] * vw, vpy = b[3] * vh;
         float pcx = 0, pcy = 0, vcx = 0, vcy = 0, pressure = 0;
         uint32_t own = counts[i];
-        for (uint32_t k = 0; k < own && (int)k < cap; k++)
-            accumulate_contact(p, (uint32_t)i, lists[(size_t)i * cap + k], in, materials, counts, ppx,
-                               ppy, vpx, vpy, own, eff_dt, &pcx, &pcy, &vcx, &vcy, &pressure,
-                               write_pressure);
+        for (uint32_t k = 0; k < own && (int)k < cap; k++) {
+            uint32_t j = lists[(size_t)i * cap + k];
+            accumulate_contact(p, (uint32_t)i, j, in, materials, counts, ppx, ppy, vpx, vpy, own,
+                               eff_dt, !sleep_awake(sl, (int)j), &pcx, &pcy, &vcx, &vcy,
+                               &pressure, write_pressure);
+        }
         if (p->bounded) {
             float weight = 1.0f / (float)(own + 1u);
             float fr = materials[4 * i + 1];
@@ -368,30 +403,100 @@ void orc_solve(const orc_params *p, int n, const float *in, float *out, const fl
         o[2] = vpx / vw;
         o[3] = vpy / vh;
         int fluid = p->liquid && materials[4 * i + 2] == 0.0f;
+        if (write_pressure && sl && sl->enabled) {
+            /* S/collision_cache_solve.comp:449-471: a body that moved this
+             * step, or that nothing holds, keeps its tile awake. */
+            float radius = 0.5f * p->diameter, slack = 0.05f * p->diameter;
+            int held = own > 0u || (p->bounded && (ppx < radius + slack || ppy < radius + slack ||
+                                                   ppx > vw - radius - slack ||
+                                                   ppy > vh - radius - slack));
+            if (fluid) held = 1;
+            float dx = vpx * p->dt, dy = vpy * p->dt;
+            float settling = 0.005f * p->diameter;
+            float d2 = dx * dx + dy * dy, m2 = pcx * pcx + pcy * pcy;
+            if (!held || d2 >= sl->disp2 || m2 > settling * settling) {
+                sl->tile_active[sleep_tile(sl, o[0], o[1])] = sl->stamp + 1u;
+                float mark = 0.5f * (float)SLEEP_TILE_PIXELS;
+                if (d2 >= mark * mark)
+                    sl->tile_active[sleep_tile(sl, o[0] + dx / vw, o[1] + dy / vh)] = sl->stamp + 1u;
+            }
+        }
         if (write_pressure && !fluid) pressures[i] = pressure;
     }
 }
 
-#define ORC_CAP 4096
-void orc_collide_pass(const orc_params *p, int n, float *bodies, const float *accel,
-                      const float *materials, float *pressures, float kick_dt, float drift_dt,
-                      int write_pressure, uint32_t *counts_out) {
-    orc_drift(p, n, bodies, accel, kick_dt, drift_dt);
+void orc_solve(const orc_params *p, int n, const float *in, float *out, const float *materials,
+               const uint32_t *counts, const uint32_t *lists, int cap, float *pressures,
+               int write_pressure) {
+    solve_impl(p, n, in, out, materials, counts, lists, cap, pressures, write_pressure, NULL);
+}
+
+/* One pass: drift, contacts, solve. counts (nullable) persists between
+ * passes: a sleeping body keeps its last count there
+ * (S/collision_cache_count.comp:147-160). sl (nullable): sleeping tiles. */
+static void collide_pass_impl(const orc_params *p, int n, float *bodies, const float *accel,
+                              const float *materials, float *pressures, float kick_dt,
+                              float drift_dt, int write_pressure, uint32_t *counts_out,
+                              uint32_t *persist, const sleep_ctx *sl) {
+    if (sl && sl->sleeping) {
+        /* S/collision_cache_drift.comp:60-91: the kick pass tells every
+         * body's state for the step; a sleeping body rests, its velocity
+         * dropped, and neither kicks nor drifts. */
+        for (int i = 0; i < n; i++) {
+            float *b = bodies + 4 * i;
+            int awake;
+            if (kick_dt != 0.0f) {
+                int fluid = p->liquid && materials[4 * i + 2] == 0.0f;
+                awake = fluid || sl->tile_awake[sleep_tile(sl, b[0], b[1])] == sl->stamp;
+                sl->body_awake[i] = awake ? sl->stamp : 0u;
+                if (!awake) { b[2] = 0.0f; b[3] = 0.0f; }
+            } else {
+                awake = sl->body_awake[i] == sl->stamp;
+            }
+            if (!awake || (kick_dt == 0.0f && drift_dt == 0.0f)) continue;
+            if (kick_dt != 0.0f) {
+                float h = 0.5f * kick_dt;
+                b[2] += accel[2 * i] * h;
+                b[3] += accel[2 * i + 1] * h;
+            }
+            b[0] += b[2] * drift_dt;
+            b[1] += b[3] * drift_dt;
+            if (p->bounded) {
+                b[0] = clampf(b[0], 0.0f, MAXIMUM_UNIT);
+                b[1] = clampf(b[1], 0.0f, MAXIMUM_UNIT);
+            } else {
+                b[0] = fractf(b[0]);
+                b[1] = fractf(b[1]);
+            }
+        }
+    } else {
+        orc_drift(p, n, bodies, accel, kick_dt, drift_dt);
+    }
     if (!p->collide) return;
     uint32_t *counts = malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
     orc_contacts(p, n, bodies, materials, counts, NULL, 0);
-    size_t total = 0;
-    for (int i = 0; i < n; i++) total += counts[i];
     int cap = 1;
     for (int i = 0; i < n; i++) if ((int)counts[i] > cap) cap = (int)counts[i];
     uint32_t *lists = malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1) * (size_t)cap);
     orc_contacts(p, n, bodies, materials, counts, lists, cap);
+    if (persist) {
+        for (int i = 0; i < n; i++) {
+            if (sleep_awake(sl, i)) persist[i] = counts[i];
+            else counts[i] = persist[i];
+        }
+    }
     float *out = malloc((size_t)(n > 0 ? n : 1) * 16);
-    orc_solve(p, n, bodies, out, materials, counts, lists, cap, pressures, write_pressure);
+    solve_impl(p, n, bodies, out, materials, counts, lists, cap, pressures, write_pressure, sl);
     memcpy(bodies, out, (size_t)n * 16);
     if (counts_out) memcpy(counts_out, counts, sizeof(uint32_t) * (size_t)n);
-    (void)total;
     free(out); free(lists); free(counts);
+}
+
+void orc_collide_pass(const orc_params *p, int n, float *bodies, const float *accel,
+                      const float *materials, float *pressures, float kick_dt, float drift_dt,
+                      int write_pressure, uint32_t *counts_out) {
+    collide_pass_impl(p, n, bodies, accel, materials, pressures, kick_dt, drift_dt,
+                      write_pressure, counts_out, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -600,6 +705,9 @@ struct orc_sim {
     long steps;
     uint32_t *last_dest;
     int kernel_valid;
+    uint32_t *counts;         /* contact counts, kept between passes and steps */
+    sleep_ctx sleep;
+    int wake_steps;
 };
 
 static void sim_init_kernel(orc_sim *s) {
@@ -637,7 +745,55 @@ orc_sim *orc_sim_create(const orc_params *p, int n, const float *bodies, const f
     s->last_dest = malloc(4 * nn);
     for (int i = 0; i < n; i++) s->last_dest[i] = (uint32_t)i;
     s->total_mass = (float)(n > 1 ? n : 1);
+    s->counts = calloc(nn, 4);
+    /* K/SleepTiles.kt:37-60: 32 px tiles, stamp 1, everything awake for
+     * the first two steps. */
+    s->sleep.enabled = p->sleep && p->collide;
+    s->sleep.stamp = 1u;
+    s->sleep.tiles_x = (p->width + SLEEP_TILE_PIXELS - 1) / SLEEP_TILE_PIXELS;
+    s->sleep.tiles_y = (p->height + SLEEP_TILE_PIXELS - 1) / SLEEP_TILE_PIXELS;
+    size_t tiles = (size_t)s->sleep.tiles_x * s->sleep.tiles_y;
+    s->sleep.tile_awake = calloc(tiles, 4);
+    s->sleep.tile_active = calloc(tiles, 4);
+    s->sleep.body_awake = calloc(nn, 4);
+    float disp = p->sleep_speed * p->dt;
+    s->sleep.disp2 = disp * disp;
+    s->wake_steps = SLEEP_WAKE_STEPS;
     return s;
+}
+
+void orc_sim_wake(orc_sim *s) { s->wake_steps = SLEEP_WAKE_STEPS; }
+uint32_t orc_sim_sleep_stamp(const orc_sim *s) { return s->sleep.stamp; }
+void orc_sim_body_awake(const orc_sim *s, uint32_t *awake) {
+    memcpy(awake, s->sleep.body_awake, (size_t)s->n * 4);
+}
+
+/* S/sleep_tiles.comp:10-28 then K/SleepTiles.kt:91-100 endStep: the tiles
+ * awake in the next step are those marked active or next to one. */
+static void sleep_advance(orc_sim *s) {
+    sleep_ctx *sl = &s->sleep;
+    if (!sl->enabled) return;
+    uint32_t next = sl->stamp + 1u;
+    int tx = sl->tiles_x, ty = sl->tiles_y;
+    for (int index = 0; index < tx * ty; index++) {
+        int x0 = index % tx, y0 = index / tx, awake = 0;
+        for (int dy = -1; dy <= 1; dy++) {
+            for (int dx = -1; dx <= 1; dx++) {
+                int x = x0 + dx, y = y0 + dy;
+                if (s->p.bounded) {
+                    if (x < 0 || y < 0 || x >= tx || y >= ty) continue;
+                } else {
+                    x = (x + tx) % tx;
+                    y = (y + ty) % ty;
+                }
+                if (sl->tile_active[x + y * tx] == next) awake = 1;
+            }
+        }
+        if (awake) sl->tile_awake[index] = next;
+    }
+    sl->stamp++;
+    if (sl->stamp == 0u) sl->stamp = 1u;
+    if (s->wake_steps > 0) s->wake_steps--;
 }
 
 void orc_sim_destroy(orc_sim *s) {
@@ -645,7 +801,8 @@ void orc_sim_destroy(orc_sim *s) {
     free(s->bodies); free(s->accel); free(s->mats); free(s->press);
     free(s->tw); free(s->kx); free(s->ky); free(s->fft); free(s->fx); free(s->fy);
     free(s->density); free(s->cell_count); free(s->cell_start); free(s->cell_mass);
-    free(s->cell_list); free(s->last_dest); free(s);
+    free(s->cell_list); free(s->last_dest); free(s->counts);
+    free(s->sleep.tile_awake); free(s->sleep.tile_active); free(s->sleep.body_awake); free(s);
 }
 
 void orc_sim_set_bodies(orc_sim *s, const float *bodies) { memcpy(s->bodies, bodies, (size_t)s->n * 16); }
@@ -753,6 +910,8 @@ void orc_sim_kick(orc_sim *s, int initial) {
     orc_neighbor_radius(&s->p, &rx, &ry);
     for (int i = 0; i < s->n; i++) {
         float *b = s->bodies + 4 * i;
+        /* S/kick.comp:109-110: a sleeping body is not kicked. */
+        if (!initial && !sleep_awake(&s->sleep, i)) continue;
         if (!s->p.mutual_gravity) {
             if (!initial) { b[2] += fieldx * half; b[3] += fieldy * half; }
             s->accel[2 * i] = fieldx; s->accel[2 * i + 1] = fieldy;
@@ -807,15 +966,19 @@ void orc_sim_step(orc_sim *s) {
         orc_spatial_sort(s->n, s->p.width, s->p.height, s->bodies, s->accel, s->mats, s->last_dest);
     else if (s->p.sort_interval <= 0)
         s->steps++;
+    /* Sleep needs the contact passes: their final pass marks the tiles. */
+    s->sleep.sleeping = s->p.sleep && s->p.collide && s->wake_steps == 0;
     float sub = s->p.dt / (float)s->p.substeps;
     for (int k = 0; k < s->p.substeps; k++)
-        orc_collide_pass(&s->p, s->n, s->bodies, s->accel, s->mats, s->press,
-                         k == 0 ? s->p.dt : 0.0f, sub, 0, NULL);
+        collide_pass_impl(&s->p, s->n, s->bodies, s->accel, s->mats, s->press,
+                          k == 0 ? s->p.dt : 0.0f, sub, 0, NULL, s->counts, &s->sleep);
     if (s->p.liquid) orc_sim_resolve_liquid(s);
     if (s->p.mutual_gravity) orc_sim_build_force(s);
     orc_sim_kick(s, 0);
     if (s->p.collide)
-        orc_collide_pass(&s->p, s->n, s->bodies, s->accel, s->mats, s->press, 0.0f, 0.0f, 1, NULL);
+        collide_pass_impl(&s->p, s->n, s->bodies, s->accel, s->mats, s->press, 0.0f, 0.0f, 1,
+                          NULL, s->counts, &s->sleep);
+    sleep_advance(s);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -924,7 +1087,12 @@ static void lq_delta(const liquid_k *k, const float *a, const float *b, float *d
 
 /* K/LiquidSolver.kt:107-173: grid of the step's start positions, 12 Jacobi
  * (lambda, apply) iterations, XSPH viscosity. The neighbours of a body are
- * the bodies of the 3x3 cells around its start cell, in ascending index. */
+ * the bodies of the 3x3 cells around its start cell (S/liquid_kernel.glsl:
+ * "every pass searches around the cell a body had when the grid was
+ * built"). The reference sums them in the order of its atomically built
+ * lists; the canonical order here is the stencil's cell order (rows, then
+ * columns, from -1 to 1), ascending body index within a cell, a cell met
+ * twice on a grid narrower than three cells taken once. */
 void orc_sim_resolve_liquid(orc_sim *s) {
     const orc_params *p = &s->p;
     liquid_k k;
@@ -952,7 +1120,7 @@ void orc_sim_resolve_liquid(orc_sim *s) {
     memcpy(fill, start, sizeof(int) * (size_t)gw * gh);
     for (int i = 0; i < n; i++) items[fill[cell[i]]++] = i;
     free(fill);
-    /* neighbour candidates per body, ascending */
+    /* neighbour candidates per body, in the canonical order */
     int *nstart = malloc(sizeof(int) * ((size_t)n + 1));
     int cap = 64, used = 0;
     int *nb = malloc(sizeof(int) * (size_t)cap);
@@ -975,7 +1143,6 @@ void orc_sim_resolve_liquid(orc_sim *s) {
                 nb[used++] = j;
             }
         }
-        qsort(nb + first, (size_t)(used - first), sizeof(int), cmp_u32);
     }
     nstart[n] = used;
     float *pos = malloc((size_t)(n > 0 ? n : 1) * 8), *lam = malloc((size_t)(n > 0 ? n : 1) * 4);

@@ -52,6 +52,9 @@ typedef struct orc_params {
     float kernel_radius_diameters;
     int liquid_iterations;
     float viscosity;
+    /* sleeping tiles (CollisionConfig.sleep, sleepSpeedPixels) */
+    int sleep;
+    float sleep_speed;        /* px/s */
 } orc_params;
 
 /* ---- a: spatial sort ---------------------------------------------------- */
@@ -119,6 +122,12 @@ void orc_sim_density_spectrum(const orc_sim *s, float *spectrum);
 void orc_sim_kick(orc_sim *s, int initial);
 void orc_sim_initialize_acceleration(orc_sim *s);
 void orc_sim_step(orc_sim *s);
+/* GravityPipeline.wake(): every body awake for the next two steps. */
+void orc_sim_wake(orc_sim *s);
+/* Step stamp of the sleeping tiles, and the body-awake stamps of the last
+ * step (a body was awake in it when its stamp equals the stamp - 1). */
+uint32_t orc_sim_sleep_stamp(const orc_sim *s);
+void orc_sim_body_awake(const orc_sim *s, uint32_t *awake);
 /* Sort at the next step (as the reference's step counter would). */
 long orc_sim_steps(const orc_sim *s);
 void orc_sim_last_sort_dest(const orc_sim *s, uint32_t *dest);

@@ -26,6 +26,7 @@ class OrcParams(ctypes.Structure):
         ("window_floor", ctypes.c_float), ("field_x", ctypes.c_float), ("field_y", ctypes.c_float),
         ("sort_interval", ctypes.c_int), ("kernel_radius_diameters", ctypes.c_float),
         ("liquid_iterations", ctypes.c_int), ("viscosity", ctypes.c_float),
+        ("sleep", ctypes.c_int), ("sleep_speed", ctypes.c_float),
     ]
 
 
@@ -78,6 +79,9 @@ def load():
             "orc_sim_kick": (None, [P, I]),
             "orc_sim_initialize_acceleration": (None, [P]),
             "orc_sim_step": (None, [P]),
+            "orc_sim_wake": (None, [P]),
+            "orc_sim_sleep_stamp": (U, [P]),
+            "orc_sim_body_awake": (None, [P, P]),
             "orc_sim_steps": (ctypes.c_long, [P]),
             "orc_sim_last_sort_dest": (None, [P, P]),
             "orc_liquid_constants": (None, [F, F, P, P, P, P, P]),
@@ -98,7 +102,8 @@ def params(width, height, *, bounded=False, mutual_gravity=True, liquid=False, c
            dt=1.0 / 120.0, diameter=1.12838, restitution=0.0, coincident_eps=1e-12, grid=(512, 256),
            gravity=0.035, softening=0.85, near_sigma=3.0, near_min=1, near_max=6, exact_limit=2,
            home_limit=8, min_dist_frac=0.05, window_floor=0.08, field=(0.0, 0.0), sort_interval=2,
-           kernel_radius=2.0, liquid_iterations=12, viscosity=0.1) -> OrcParams:
+           kernel_radius=2.0, liquid_iterations=12, viscosity=0.1, sleep=False,
+           sleep_speed=6.0) -> OrcParams:
     p = OrcParams()
     p.width, p.height, p.bounded = width, height, int(bounded)
     p.mutual_gravity, p.liquid, p.collide = int(mutual_gravity), int(liquid), int(collide)
@@ -111,6 +116,7 @@ def params(width, height, *, bounded=False, mutual_gravity=True, liquid=False, c
     p.field_x, p.field_y = field
     p.sort_interval = sort_interval
     p.kernel_radius_diameters, p.liquid_iterations, p.viscosity = kernel_radius, liquid_iterations, viscosity
+    p.sleep, p.sleep_speed = int(sleep), sleep_speed
     return p
 
 
@@ -265,6 +271,18 @@ class Sim:
         d = np.empty(self.n, np.uint32)
         load().orc_sim_last_sort_dest(self._h, _p(d))
         return d
+
+    def wake(self):
+        load().orc_sim_wake(self._h)
+
+    def sleep_stamp(self) -> int:
+        return int(load().orc_sim_sleep_stamp(self._h))
+
+    def body_awake(self):
+        """Whether each body was awake in the last step."""
+        a = np.empty(self.n, np.uint32)
+        load().orc_sim_body_awake(self._h, _p(a))
+        return a == np.uint32(self.sleep_stamp() - 1)
 
     def resolve_liquid(self):
         load().orc_sim_resolve_liquid(self._h)

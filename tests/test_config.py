@@ -121,3 +121,12 @@ def test_create_rejects_bad_arguments_before_touching_the_gpu():
     c = native.defaults()
     assert lib.gs_create(ctypes.byref(c), None, -1, ctypes.byref(h)) == native.GS_EINVAL
     assert lib.gs_step(None, 1) == native.GS_EINVAL
+
+
+def test_batch_rejects_mixed_time_steps_before_touching_the_gpu():
+    from gravity_amd import CollisionConfig, GravityBatch, GravityConfig, ParticleConfig
+    a = GravityConfig(particles=ParticleConfig(capacity=100, initialBodies=1))
+    b = GravityConfig(particles=ParticleConfig(capacity=100, initialBodies=1),
+                      collisions=CollisionConfig(timeStep=1.0 / 60.0))
+    with pytest.raises(ValueError, match="shares dt"):
+        GravityBatch([a, b], [(256, 256), (256, 256)])
