@@ -154,6 +154,59 @@ def batched_leg(steps, warmup):
             "speedup_vs_plain": rate / plain}
 
 
+def adjoint_leg(steps=64, every=8, warmup=5):
+    """The adjoint workload: 100,000 bodies uniform on a 1440x720 torus, P3M
+    grid 512x256, gravity only; 64 leapfrog steps recorded with a checkpoint
+    every 8, then the VJP of the mean squared distance of the final positions
+    to seeded targets with respect to the initial velocities (recomputing
+    each checkpoint interval). The cost ratio is the VJP's time per step over
+    a plain forward step's."""
+    from gravity_amd import CollisionConfig, GravityConfig, GravityFieldConfig, GravityPipeline
+    from gravity_amd import ParticleConfig, scenarios
+    n, W, H = 100_000, 1440, 720
+    bodies, mats = scenarios.uniform(n, W, H, seed=1)
+    targets = scenarios.targets(n, 12)
+    cfg = GravityConfig(particles=ParticleConfig(capacity=n, initialBodies=n, seed=1),
+                        gravity=GravityFieldConfig(gridWidth=512, gridHeight=256),
+                        collisions=CollisionConfig(maximumSubsteps=1), collisionsEnabled=False)
+    with GravityPipeline(cfg, W, H, scenarioState=bodies) as pipe:
+        pipe.upload(bodies, mats)
+        pipe.initializeAcceleration(n)
+        for _ in range(warmup):
+            pipe.step(n)
+        pipe.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pipe.step(n)
+        pipe.synchronize()
+        t_fwd = (time.perf_counter() - t0) / steps
+        pipe.upload(bodies, mats)
+        pipe.initializeAcceleration(n)
+        pipe.synchronize()
+        t0 = time.perf_counter()
+        pipe.adjointRecord(n, steps, every)
+        pipe.synchronize()
+        t_rec = time.perf_counter() - t0
+        final = pipe.bodies(n)
+        where = pipe.adjointOrder(n)
+        view = np.array([W, H], np.float64)
+        d = (final[where, :2].astype(np.float64) - targets) * view
+        d -= np.floor(d / view + 0.5) * view
+        dldx = np.zeros((n, 2), np.float32)
+        dldx[where] = (2.0 * d * view / n).astype(np.float32)
+        t0 = time.perf_counter()
+        _, dv0 = pipe.adjointVjp(dldx, n)
+        t_vjp = time.perf_counter() - t0
+    return {"workload": "adjoint-100k: 100,000 bodies uniform on a 1440x720 torus, grid 512x256, "
+                        "gravity only, 64 steps, checkpoint every 8, loss mean |x_T - target|^2",
+            "bodies": n, "steps": steps, "checkpoint_every": every,
+            "forward_ms_per_step": t_fwd * 1e3, "record_ms_per_step": t_rec * 1e3 / steps,
+            "vjp_ms_per_step": t_vjp * 1e3 / steps,
+            "cost_vs_forward": (t_vjp / steps) / t_fwd,
+            "value": n * steps / (t_rec + t_vjp), "unit": "body-steps/s (record + VJP)",
+            "gradient_finite": bool(np.isfinite(dv0).all())}
+
+
 def cpu_baseline(bodies, mats, steps=2):
     """The oracle (single-threaded C restatement) on `steps` steps of the same workload."""
     from oracle import oracle as orc
@@ -178,6 +231,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=20,
                     help="steps of the batched-ensemble leg (0 skips it)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel eagerly instead of through captured graphs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -235,6 +289,11 @@ def main():
               file=sys.stderr, flush=True)
         batched = batched_leg(args.batched_steps, max(2, args.warmup // 4))
 
+    adjoint = None
+    if not args.no_adjoint:
+        print("[bench] adjoint: 64 steps recorded, VJP", file=sys.stderr, flush=True)
+        adjoint = adjoint_leg()
+
     cpu = None
     if not args.no_cpu:
         print(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps", file=sys.stderr, flush=True)
@@ -266,6 +325,7 @@ def main():
         "kernels_ms_per_step": {k: v[1] / args.profile_steps for k, v in per_kernel.items()},
         "finite": finite,
         "batched": batched,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -18,8 +18,8 @@ PACKAGE = Path(__file__).resolve().parent
 CSRC = PACKAGE / "csrc"
 BUILD = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libgravity_hip.so"
-SOURCES = ["sort.hip", "collide.hip", "p3m.hip", "fft.hip", "liquid.hip", "sleep.hip", "engine.hip",
-           "green.cpp", "capi.cpp"]
+SOURCES = ["sort.hip", "collide.hip", "p3m.hip", "fft.hip", "liquid.hip", "sleep.hip", "adjoint.hip",
+           "engine.hip", "green.cpp", "capi.cpp"]
 ARCH = os.environ.get("GRAVITY_OFFLOAD_ARCH", "gfx950")
 
 COMMON_FLAGS = [

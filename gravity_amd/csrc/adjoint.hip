@@ -1,0 +1,259 @@
+// Adjoint of the gravity step: vector-Jacobian products of the P3M
+// acceleration field, for the reversed leapfrog of Engine::adjoint_vjp.
+//
+// The forward leapfrog step n (drift-only passes, no collisions) is
+//   vh = v_n + dt/2 A(x_n),  x_{n+1} = x_n + dt vh,  v_{n+1} = vh + dt/2 A(x_{n+1}),
+// with A(x_i) = sum over mesh nodes of W(x_i - node) F(node) (CIC gather of
+// F = K * rho, rho = sum_k m_k W(x_k - node)) plus the near field of the
+// kick (S/kick.comp:66-80, 118-151). Going backwards, with w = vbar + dt xbar
+// (the adjoint of vh), each step needs g = J_A(x_n)^T w once: the two
+// half-kicks that read A(x_n) both see w. The terms of J_A^T w:
+//   - the gather's own-position derivative: grad of W(x_i - node) against
+//     w_i . F(node);
+//   - the mesh: w spread with the CIC weights (transposed gather), the
+//     transposed convolution K^T = -K (the force kernels are odd), i.e. the
+//     forward FFT pipeline applied to wx - i wy, minus the real part of its
+//     result is rhobar; each body then takes m_i grad W(x_i - node) against
+//     rhobar (the deposit's derivative);
+//   - the near field: the derivative of every term with respect to the
+//     body's own position, and, for every body k that sees body i exactly,
+//     the derivative of k's term with respect to i's position (gathered, so
+//     no atomics). The near-field topology (cell membership, monopoles,
+//     the home-cell cap) is frozen: it is piecewise constant in x.
+// The fixed-point mass deposit is differentiated as the CIC deposit it
+// rounds. All of it is FP32 with contraction (a tolerance path).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gs {
+
+__device__ __forceinline__ float adj_min_image(const SimDesc& s, float d) {
+    return s.bounded ? d : d - floorf(d + 0.5f);
+}
+
+// d(w . A)/d(px, py) for one near-field term A = C m phi(r2) (px/aspect, py),
+// phi = exp(-r2 / 2 sigma^2) / max(r2, minDist2), px = dx * aspect, py = dy.
+__device__ __forceinline__ void near_term_vjp(const SimDesc& s, float px, float py, float mass,
+                                              float wx, float wy, float& dpx, float& dpy) {
+#pragma clang fp contract(fast)
+    const float k2 = s.invTwoSigma2;
+    float r2 = px * px + py * py;
+    float e = __expf(-r2 * k2);
+    bool clamped = !(r2 > s.minDist2);
+    float q = __builtin_amdgcn_rcpf(clamped ? s.minDist2 : r2);
+    float phi = e * q;
+    float dphi = -k2 * phi - (clamped ? 0.0f : phi * q);
+    float sw = wx * px * s.invAspect + wy * py;
+    float cm = s.gOverM * mass;
+    dpx = cm * (2.0f * dphi * px * sw + phi * wx * s.invAspect);
+    dpy = cm * (2.0f * dphi * py * sw + phi * wy);
+}
+
+__device__ __forceinline__ float2 adjoint_w(const float2* __restrict__ xbar,
+                                            const float2* __restrict__ vbar, int i, float dt) {
+    float2 a = xbar[i], b = vbar[i];
+    return make_float2(b.x + dt * a.x, b.y + dt * a.y);
+}
+
+// Per body: the gather's own-position term and the near-field terms of
+// J_A^T w; also the value the transposed gather spreads, (wx, -wy).
+__global__ void __launch_bounds__(WG) k_adj_direct(const SimDesc* __restrict__ sims,
+                                                   const BodyBlock* __restrict__ blocks,
+                                                   const float4* __restrict__ body,
+                                                   const float4* __restrict__ mats,
+                                                   const float2* __restrict__ force,
+                                                   const uint4* __restrict__ cellRec,
+                                                   const float4* __restrict__ cellBody,
+                                                   const float2* __restrict__ xbar,
+                                                   const float2* __restrict__ vbar, float dt,
+                                                   float2* __restrict__ gbar,
+                                                   float2* __restrict__ spread) {
+#pragma clang fp contract(fast)
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[b.sim];
+    int g = b.first + (int)threadIdx.x;
+    if (g - s.base >= s.n) return;
+    float4 p = body[g];
+    float mass = mats[g].x;
+    float2 w = adjoint_w(xbar, vbar, g, dt);
+    spread[g] = make_float2(w.x, -w.y);
+    float gx = 0.0f, gy = 0.0f;
+    // CIC gather: A = sum of bilinear weights times F; its gradient in x.
+    if (s.mutual) {
+        float fgx = p.x * (float)s.pw - 0.5f, fgy = p.y * (float)s.ph - 0.5f;
+        float flx = floorf(fgx), fly = floorf(fgy);
+        int bx = (int)flx, by = (int)fly;
+        float fx = fgx - flx, fy = fgy - fly;
+        int x0 = wrap_axis(bx, s.mw), x1 = wrap_axis(bx + 1, s.mw);
+        int y0 = wrap_axis(by, s.mh), y1 = wrap_axis(by + 1, s.mh);
+        const float2* F = force + s.meshBase;
+        float2 v00 = F[x0 + y0 * s.mw], v10 = F[x1 + y0 * s.mw];
+        float2 v01 = F[x0 + y1 * s.mw], v11 = F[x1 + y1 * s.mw];
+        float d00 = w.x * v00.x + w.y * v00.y, d10 = w.x * v10.x + w.y * v10.y;
+        float d01 = w.x * v01.x + w.y * v01.y, d11 = w.x * v11.x + w.y * v11.y;
+        gx += (float)s.pw * ((d10 - d00) * (1.0f - fy) + (d11 - d01) * fy);
+        gy += (float)s.ph * ((d01 - d00) * (1.0f - fx) + (d11 - d10) * fx);
+    }
+    if (s.mutual) {
+        const float cellMassUnit = 1.0f / 256.0f;
+        int hx = (int)floorf(p.x * (float)s.pw), hy = (int)floorf(p.y * (float)s.ph);
+        int home = s.physBase + wrap_axis(hx, s.pw) + wrap_axis(hy, s.ph) * s.pw;
+        uint4 own = cellRec[home];
+        int myCount = (int)own.y, myRank = 0;
+        for (int t = (int)own.x; t < (int)own.x + myCount; t++)
+            if (__float_as_uint(cellBody[t].w) == (uint32_t)g) myRank = t - (int)own.x;
+        for (int dy = -s.ry; dy <= s.ry; dy++) {
+            int cy = hy + dy;
+            if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
+            int wy = wrap_axis(cy, s.ph);
+            float cpy = adj_min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
+            const uint4* row = cellRec + (s.physBase + wy * s.pw);
+            for (int dx = -s.rx; dx <= s.rx; dx++) {
+                int cx = hx + dx;
+                if (s.bounded && (cx < 0 || cx >= s.pw)) continue;
+                int wx = wrap_axis(cx, s.pw);
+                uint4 rec = row[wx];
+                int count = (int)rec.y;
+                if (count == 0) continue;
+                bool isHome = dx == 0 && dy == 0;
+                float dpx, dpy;
+                // this body's own terms (S/kick.comp:118-151), against its position
+                if (!isHome && count > s.exactLimit) {
+                    float cpx = adj_min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
+                    near_term_vjp(s, cpx, cpy, (float)rec.z * cellMassUnit, w.x, w.y, dpx, dpy);
+                    gx -= s.aspect * dpx;
+                    gy -= dpy;
+                } else {
+                    int limit = isHome ? s.homeLimit : s.exactLimit;
+                    int stop = (int)rec.x + (count < limit ? count : limit);
+                    for (int t = (int)rec.x; t < stop; t++) {
+                        float4 o = cellBody[t];
+                        if (__float_as_uint(o.w) == (uint32_t)g) continue;
+                        float ox = adj_min_image(s, o.x - p.x) * s.aspect;
+                        float oy = adj_min_image(s, o.y - p.y);
+                        near_term_vjp(s, ox, oy, o.z, w.x, w.y, dpx, dpy);
+                        gx -= s.aspect * dpx;
+                        gy -= dpy;
+                    }
+                }
+                // the terms of the bodies of this cell that see this body exactly
+                bool seen = isHome ? myRank < s.homeLimit : myCount <= s.exactLimit;
+                if (!seen) continue;
+                for (int t = (int)rec.x; t < (int)rec.x + count; t++) {
+                    float4 o = cellBody[t];
+                    int k = (int)__float_as_uint(o.w);
+                    if (k == g) continue;
+                    float2 wk = adjoint_w(xbar, vbar, k, dt);
+                    float ox = adj_min_image(s, p.x - o.x) * s.aspect;
+                    float oy = adj_min_image(s, p.y - o.y);
+                    near_term_vjp(s, ox, oy, mass, wk.x, wk.y, dpx, dpy);
+                    gx += s.aspect * dpx;
+                    gy += dpy;
+                }
+            }
+        }
+    }
+    gbar[g] = make_float2(gx, gy);
+}
+
+// The deposit's derivative against rhobar = -Re(result of the transposed
+// convolution), then the step's update: xbar += coef * g, vbar = w.
+__global__ void __launch_bounds__(WG) k_adj_finish(const SimDesc* __restrict__ sims,
+                                                   const BodyBlock* __restrict__ blocks,
+                                                   const float4* __restrict__ body,
+                                                   const float4* __restrict__ mats,
+                                                   const float2* __restrict__ result,
+                                                   const float2* __restrict__ gbar,
+                                                   float2* __restrict__ xbar,
+                                                   float2* __restrict__ vbar, float dt,
+                                                   float coef) {
+#pragma clang fp contract(fast)
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[b.sim];
+    int g = b.first + (int)threadIdx.x;
+    if (g - s.base >= s.n) return;
+    float4 p = body[g];
+    float2 gsum = gbar[g];
+    if (s.mutual) {
+        float m = mats[g].x;
+        float fgx = p.x * (float)s.pw - 0.5f, fgy = p.y * (float)s.ph - 0.5f;
+        float flx = floorf(fgx), fly = floorf(fgy);
+        int bx = (int)flx, by = (int)fly;
+        float fx = fgx - flx, fy = fgy - fly;
+        int x0 = wrap_axis(bx, s.mw), x1 = wrap_axis(bx + 1, s.mw);
+        int y0 = wrap_axis(by, s.mh), y1 = wrap_axis(by + 1, s.mh);
+        const float2* R = result + s.meshBase;
+        float r00 = -R[x0 + y0 * s.mw].x, r10 = -R[x1 + y0 * s.mw].x;
+        float r01 = -R[x0 + y1 * s.mw].x, r11 = -R[x1 + y1 * s.mw].x;
+        gsum.x += m * (float)s.pw * ((r10 - r00) * (1.0f - fy) + (r11 - r01) * fy);
+        gsum.y += m * (float)s.ph * ((r01 - r00) * (1.0f - fx) + (r11 - r10) * fx);
+    }
+    float2 w = adjoint_w(xbar, vbar, g, dt);
+    float2 xb = xbar[g];
+    xbar[g] = make_float2(xb.x + coef * gsum.x, xb.y + coef * gsum.y);
+    vbar[g] = w;
+}
+
+// Undoes a spatial sort on the adjoint state: the value of old body i is
+// the sorted value at dest[i].
+__global__ void __launch_bounds__(WG) k_adj_unsort(const SimDesc* __restrict__ sims,
+                                                   const BodyBlock* __restrict__ blocks,
+                                                   const uint32_t* __restrict__ dest,
+                                                   const float2* __restrict__ xin,
+                                                   const float2* __restrict__ vin,
+                                                   float2* __restrict__ xout,
+                                                   float2* __restrict__ vout) {
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[b.sim];
+    int g = b.first + (int)threadIdx.x;
+    if (g - s.base >= s.n) return;
+    uint32_t d = dest[g];
+    xout[g] = xin[d];
+    vout[g] = vin[d];
+}
+
+// Composes a sort into the running permutation: where[i] is the index now
+// holding the body that started at index i.
+__global__ void __launch_bounds__(WG) k_adj_compose(const SimDesc* __restrict__ sims,
+                                                    const BodyBlock* __restrict__ blocks,
+                                                    const uint32_t* __restrict__ dest,
+                                                    uint32_t* __restrict__ where) {
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[b.sim];
+    int g = b.first + (int)threadIdx.x;
+    if (g - s.base >= s.n) return;
+    where[g] = dest[where[g]];
+}
+
+void launch_adj_direct(hipStream_t st, const LaunchShape& ls, const float4* body,
+                       const float4* mats, const float2* force, const uint4* cellRec,
+                       const float4* cellBody, const float2* xbar, const float2* vbar, float dt,
+                       float2* gbar, float2* spread) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_adj_direct, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                       body, mats, force, cellRec, cellBody, xbar, vbar, dt, gbar, spread);
+}
+
+void launch_adj_finish(hipStream_t st, const LaunchShape& ls, const float4* body,
+                       const float4* mats, const float2* result, const float2* gbar, float2* xbar,
+                       float2* vbar, float dt, float coef) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_adj_finish, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                       body, mats, result, gbar, xbar, vbar, dt, coef);
+}
+
+void launch_adj_unsort(hipStream_t st, const LaunchShape& ls, const uint32_t* dest,
+                       const float2* xin, const float2* vin, float2* xout, float2* vout) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_adj_unsort, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                       dest, xin, vin, xout, vout);
+}
+
+void launch_adj_compose(hipStream_t st, const LaunchShape& ls, const uint32_t* dest,
+                        uint32_t* where) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_adj_compose, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+                       dest, where);
+}
+
+}  // namespace gs

@@ -1,6 +1,7 @@
 // extern "C" boundary (include/gravity_hip.h): argument checks with the
 // reference's messages, exceptions mapped to return codes, a thread-local
 // error string.
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -219,6 +220,68 @@ int gs_batch_initialize_acceleration(gs_sim* sim) {
 int gs_batch_step(gs_sim* sim) {
     SIM_OR_FAIL
     return guarded([&] { e.step(); });
+}
+
+int gs_adjoint_record(gs_sim* sim, int active, int steps, int checkpoint_every) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        e.set_active(0, active);
+        e.adjoint_record(steps, checkpoint_every);
+    });
+}
+
+int gs_adjoint_vjp(gs_sim* sim, const float* dL_dx, float* dL_dx0, float* dL_dv0) {
+    SIM_OR_FAIL
+    if (!dL_dx) return fail(GS_EINVAL, "null gradient");
+    return guarded([&] { e.adjoint_vjp(dL_dx, dL_dx0, dL_dv0); });
+}
+
+int gs_adjoint_order(gs_sim* sim, uint32_t* where) {
+    SIM_OR_FAIL
+    if (!where) return fail(GS_EINVAL, "null destination");
+    return guarded([&] { e.adjoint_order(where); });
+}
+
+int gs_vjp_initial_velocity(gs_sim* sim, int active, int steps, int checkpoint_every,
+                            const float* targets, float* loss, float* dL_dv0) {
+    SIM_OR_FAIL
+    if (!targets || !dL_dv0) return fail(GS_EINVAL, "null argument");
+    return guarded([&] {
+        e.set_active(0, active);
+        e.adjoint_record(steps, checkpoint_every);
+        const gs::HostSim& h = e.sim(0);
+        const size_t n = (size_t)h.n;
+        std::vector<float> xyvv(4 * n), grad(2 * n, 0.0f);
+        std::vector<uint32_t> where(n);
+        e.download(0, GS_BODIES_XYVV, 0, (int)n, xyvv.data());
+        e.adjoint_order(where.data());
+        // L = mean |minimum image((x_T - target) * viewport)|^2, px^2
+        const double W = h.cfg.viewport_width, H = h.cfg.viewport_height;
+        double sum = 0.0;
+        for (size_t i = 0; i < n; i++) {
+            size_t f = where[i];
+            double dx = ((double)xyvv[4 * f] - targets[2 * i]) * W;
+            double dy = ((double)xyvv[4 * f + 1] - targets[2 * i + 1]) * H;
+            if (!h.bounded) {
+                dx -= std::floor(dx / W + 0.5) * W;
+                dy -= std::floor(dy / H + 0.5) * H;
+            }
+            sum += dx * dx + dy * dy;
+            grad[2 * f] = (float)(2.0 * dx * W / (double)n);
+            grad[2 * f + 1] = (float)(2.0 * dy * H / (double)n);
+        }
+        if (loss) *loss = n ? (float)(sum / (double)n) : 0.0f;
+        e.adjoint_vjp(grad.data(), nullptr, dL_dv0);
+    });
+}
+
+int gs_debug_acceleration_vjp(gs_sim* sim, int count, const float* w, float* out) {
+    SIM_OR_FAIL
+    if (!w || !out) return fail(GS_EINVAL, "null argument");
+    return guarded([&] {
+        e.set_active(0, count);
+        e.debug_accel_vjp(w, out);
+    });
 }
 
 int gs_wake(gs_sim* sim) {

@@ -36,6 +36,10 @@ __device__ __forceinline__ void grid_insert(const SimDesc& s, int g, float px, f
     grid.cellOf[g] = c;
 }
 
+__device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c) {
+    return max(min(a, b), min(max(a, b), c));
+}
+
 // S/collision_crowd.glsl:2-15: the lower index goes first; squares summed
 // without contraction. Without the torus's minimum image the order cannot
 // matter: fl(a - b) = -fl(b - a) exactly, so the squares are the same.
@@ -194,14 +198,13 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
         float d2 = pair_distance_squared(g, px, py, other, ox, oy, s);
         if (!(d2 < s.diameter2)) return;
         n++;
-        // Insertion keeping the CONTACT_SLOTS smallest indices, ascending.
+        // Insertion keeping the CONTACT_SLOTS smallest indices, ascending:
+        // with s sorted, the new slot k is the median of s[k-1], u, s[k]
+        // (one v_med3_u32 each, all independent of one another).
         uint32_t u = (uint32_t)other;
 #pragma unroll
-        for (int k = 0; k < CONTACT_SLOTS; k++) {
-            uint32_t lo = min(u, slot[k]);
-            u = max(u, slot[k]);
-            slot[k] = lo;
-        }
+        for (int k = CONTACT_SLOTS - 1; k > 0; k--) slot[k] = med3u(slot[k - 1], u, slot[k]);
+        slot[0] = min(u, slot[0]);
     });
     counts[g] = n;
 #pragma unroll

@@ -246,6 +246,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
 
 Engine::~Engine() {
     invalidate_graphs();
+    adjoint_free();
     for (auto& e : events_) {
         hipEventDestroy(e.second.first);
         hipEventDestroy(e.second.second);
@@ -615,6 +616,11 @@ void Engine::enqueue_sort() {
 
 void Engine::enqueue_force_and_kick(bool initial) {
     // K/GravityPipeline.kt:907-973 then :1213-1261
+    enqueue_force();
+    enqueue_kick(initial);
+}
+
+void Engine::enqueue_force() {
     if (anyMutual_) {
         timed("bin", [&] { launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_); });
         timed("cell_scan", [&] {
@@ -631,6 +637,9 @@ void Engine::enqueue_force_and_kick(bool initial) {
         timed("fft_columns", [&] { launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_); });
         timed("fft_rows_inverse", [&] { launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_); });
     }
+}
+
+void Engine::enqueue_kick(bool initial) {
     timed("kick", [&] {
         launch_kick(st_, shape_, body_[bodyPar_], acc_[auxPar_], spec_, cellRec_, cellBody_,
                     initial ? 1 : 0);
@@ -765,6 +774,228 @@ void Engine::debug_liquid() {
     if (bodyBlocks_ == 0 || !anyLiquid_) return;
     enqueue_liquid();
     check(hipGetLastError(), "liquid");
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the gravity step (adjoint.hip): checkpoints every `every`
+// steps while recording; the VJP recomputes one segment at a time from its
+// checkpoint, keeping the force build of every step of the segment (body
+// positions and masses, force mesh, cell tables, sort permutation), and
+// runs the reversed leapfrog through it.
+
+void Engine::adjoint_free() {
+    for (void* p : adjAllocs_) hipFree(p);
+    adjAllocs_.clear();
+    adjCk_.clear();
+    adjSlot_.clear();
+    adjX_ = adjV_ = adjX2_ = adjV2_ = adjG_ = adjSpread_ = nullptr;
+    adjWhere_ = nullptr;
+    adjFinal_ = AdjCheckpoint{};
+}
+
+void Engine::adjoint_save_slot(int j) {
+    const HostSim& h = sims_[0];
+    AdjSlot& sl = adjSlot_[j];
+    size_t n = (size_t)h.cap;
+    auto copy = [&](void* dst, const void* src, size_t bytes) {
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st_), "adjoint slot");
+    };
+    copy(sl.body, body_[bodyPar_] + h.base, n * sizeof(float4));
+    copy(sl.mats, mat_[auxPar_] + h.base, n * sizeof(float4));
+    copy(sl.force, spec_ + h.desc.meshBase, (size_t)h.meshMax * sizeof(float2));
+    copy(sl.rec, cellRec_ + h.desc.physBase, (size_t)h.phys * sizeof(uint4));
+    copy(sl.prefix, cellPrefix_ + h.desc.physBase, ((size_t)h.phys + 1) * sizeof(uint32_t));
+    copy(sl.cellBody, cellBody_, (size_t)gravitySlots_ * sizeof(float4));
+}
+
+void Engine::adjoint_record(int steps, int every) {
+    if (sims_.size() != 1) throw ArgumentError("the adjoint runs on a single simulation");
+    const HostSim& h = sims_[0];
+    if (h.cfg.collisions_enabled) throw ArgumentError("the adjoint differentiates the gravity step without contact passes");
+    if (!h.mutual) throw ArgumentError("the adjoint needs mutual gravity");
+    if (h.liquid) throw ArgumentError("the adjoint does not differentiate the liquid");
+    if (steps < 1) throw ArgumentError("steps must be positive");
+    if (every < 1) throw ArgumentError("checkpoint interval must be positive");
+    normalize();
+    adjoint_free();
+    size_t n = (size_t)h.cap;
+    auto alloc = [&](size_t bytes) {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) throw StateError("hipMalloc failed");
+        adjAllocs_.push_back(p);
+        return p;
+    };
+    int segments = (steps + every - 1) / every;
+    adjCk_.resize((size_t)segments);
+    for (auto& c : adjCk_) {
+        c.body = (float4*)alloc(n * sizeof(float4));
+        c.acc = (float2*)alloc(n * sizeof(float2));
+        c.mats = (float4*)alloc(n * sizeof(float4));
+    }
+    adjSlot_.resize((size_t)every);
+    for (auto& sl : adjSlot_) {
+        sl.body = (float4*)alloc(n * sizeof(float4));
+        sl.mats = (float4*)alloc(n * sizeof(float4));
+        sl.force = (float2*)alloc((size_t)h.meshMax * sizeof(float2));
+        sl.rec = (uint4*)alloc((size_t)h.phys * sizeof(uint4));
+        sl.prefix = (uint32_t*)alloc(((size_t)h.phys + 1) * sizeof(uint32_t));
+        sl.cellBody = (float4*)alloc((size_t)std::max(gravitySlots_, 1) * sizeof(float4));
+        sl.dest = (uint32_t*)alloc(n * sizeof(uint32_t));
+    }
+    adjFinal_.body = (float4*)alloc(n * sizeof(float4));
+    adjFinal_.acc = (float2*)alloc(n * sizeof(float2));
+    adjFinal_.mats = (float4*)alloc(n * sizeof(float4));
+    for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_})
+        *p = (float2*)alloc(n * sizeof(float2));
+    adjWhere_ = (uint32_t*)alloc(n * sizeof(uint32_t));
+    std::vector<uint32_t> iota(n);
+    for (size_t i = 0; i < n; i++) iota[i] = (uint32_t)i;
+    check(hipMemcpy(adjWhere_, iota.data(), n * 4, hipMemcpyHostToDevice), "adjoint order");
+    adjSortDue_.assign((size_t)steps, 0);
+    adjSteps_ = steps;
+    adjEvery_ = every;
+    int interval = h.cfg.spatial_sort_interval;
+    for (int k = 0; k < steps; k++) {
+        if (k % every == 0) {
+            AdjCheckpoint& c = adjCk_[(size_t)(k / every)];
+            check(hipMemcpyAsync(c.body, body_[bodyPar_] + h.base, n * sizeof(float4),
+                                 hipMemcpyDeviceToDevice, st_), "checkpoint");
+            check(hipMemcpyAsync(c.acc, acc_[auxPar_] + h.base, n * sizeof(float2),
+                                 hipMemcpyDeviceToDevice, st_), "checkpoint");
+            check(hipMemcpyAsync(c.mats, mat_[auxPar_] + h.base, n * sizeof(float4),
+                                 hipMemcpyDeviceToDevice, st_), "checkpoint");
+            c.steps = sims_[0].steps;
+        }
+        bool sortDue = interval > 0 && (sims_[0].steps % interval) == 0;
+        adjSortDue_[(size_t)k] = sortDue ? 1 : 0;
+        step();
+        if (sortDue) launch_adj_compose(st_, shape_, dest_, adjWhere_);
+    }
+    check(hipGetLastError(), "adjoint record");
+}
+
+void Engine::debug_accel_vjp(const float* w, float* out) {
+    if (sims_.size() != 1 || !sims_[0].mutual) throw ArgumentError("one simulation with mutual gravity");
+    normalize();
+    const HostSim& h = sims_[0];
+    const size_t cap = (size_t)h.cap, n = (size_t)h.n;
+    if (!adjX_) {
+        for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_}) {
+            void* q = nullptr;
+            if (hipMalloc(&q, std::max<size_t>(cap * sizeof(float2), 16)) != hipSuccess)
+                throw StateError("hipMalloc failed");
+            adjAllocs_.push_back(q);
+            *p = (float2*)q;
+        }
+    }
+    check(hipMemsetAsync(adjX_, 0, cap * sizeof(float2), st_), "vjp");
+    check(hipMemsetAsync(adjV_, 0, cap * sizeof(float2), st_), "vjp");
+    check(hipMemcpyAsync(adjV_, w, n * sizeof(float2), hipMemcpyHostToDevice, st_), "vjp");
+    enqueue_force();
+    launch_adj_direct(st_, shape_, body_[bodyPar_], mat_[auxPar_], spec_, cellRec_, cellBody_,
+                      adjX_, adjV_, 0.0f, adjG_, adjSpread_);
+    launch_fft_rows_forward_values(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, adjSpread_,
+                                   spec_, tw_, twLen_);
+    launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_);
+    launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_);
+    launch_adj_finish(st_, shape_, body_[bodyPar_], mat_[auxPar_], spec_, adjG_, adjX_, adjV_, 0.0f,
+                      1.0f);
+    check(hipMemcpyAsync(out, adjX_, n * sizeof(float2), hipMemcpyDeviceToHost, st_), "vjp");
+    check(hipStreamSynchronize(st_), "vjp");
+}
+
+void Engine::adjoint_order(uint32_t* where) {
+    if (!adjWhere_) throw StateError("no adjoint recording");
+    check(hipMemcpyAsync(where, adjWhere_, (size_t)sims_[0].n * 4, hipMemcpyDeviceToHost, st_),
+          "adjoint order");
+    check(hipStreamSynchronize(st_), "adjoint order");
+}
+
+void Engine::adjoint_vjp(const float* dLdx, float* dLdx0, float* dLdv0) {
+    if (adjCk_.empty()) throw StateError("no adjoint recording");
+    HostSim& h = sims_[0];
+    const size_t n = (size_t)h.n, cap = (size_t)h.cap;
+    const float dt = h.cfg.time_step;
+    const int S = h.substeps;
+    const float sub = h.desc.subDt;
+    // the final state, put back afterwards
+    const int bp = bodyPar_, ap = auxPar_, gp = gridPar_;
+    const long long finalSteps = h.steps;
+    check(hipMemcpyAsync(adjFinal_.body, body_[bodyPar_] + h.base, cap * sizeof(float4),
+                         hipMemcpyDeviceToDevice, st_), "adjoint");
+    check(hipMemcpyAsync(adjFinal_.acc, acc_[auxPar_] + h.base, cap * sizeof(float2),
+                         hipMemcpyDeviceToDevice, st_), "adjoint");
+    check(hipMemcpyAsync(adjFinal_.mats, mat_[auxPar_] + h.base, cap * sizeof(float4),
+                         hipMemcpyDeviceToDevice, st_), "adjoint");
+    check(hipMemsetAsync(adjX_, 0, cap * sizeof(float2), st_), "adjoint");
+    check(hipMemsetAsync(adjV_, 0, cap * sizeof(float2), st_), "adjoint");
+    check(hipMemcpyAsync(adjX_, dLdx, n * sizeof(float2), hipMemcpyHostToDevice, st_), "adjoint");
+    // The adjoint arrays are indexed like the bodies (sim 0 starts at 0).
+    for (int seg = (int)adjCk_.size() - 1; seg >= 0; seg--) {
+        const int c = seg * adjEvery_;
+        const int k = std::min(adjEvery_, adjSteps_ - c);
+        const AdjCheckpoint& ck = adjCk_[(size_t)seg];
+        check(hipMemcpyAsync(body_[bodyPar_] + h.base, ck.body, cap * sizeof(float4),
+                             hipMemcpyDeviceToDevice, st_), "adjoint restore");
+        check(hipMemcpyAsync(acc_[auxPar_] + h.base, ck.acc, cap * sizeof(float2),
+                             hipMemcpyDeviceToDevice, st_), "adjoint restore");
+        check(hipMemcpyAsync(mat_[auxPar_] + h.base, ck.mats, cap * sizeof(float4),
+                             hipMemcpyDeviceToDevice, st_), "adjoint restore");
+        // recompute the segment, keeping each step's force build
+        enqueue_force();
+        adjoint_save_slot(0);
+        for (int j = 0; j < k; j++) {
+            AdjSlot& sl = adjSlot_[(size_t)j];
+            sl.sorted = adjSortDue_[(size_t)(c + j)] != 0;
+            if (sl.sorted) {
+                enqueue_sort();
+                check(hipMemcpyAsync(sl.dest, dest_ + h.base, cap * sizeof(uint32_t),
+                                     hipMemcpyDeviceToDevice, st_), "adjoint slot");
+            }
+            if (j + 1 == k) break;  // the last step's build is not needed here
+            for (int q = 0; q < S; q++)
+                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_],
+                                    q == 0 ? dt : 0.0f, sub, 0, grid_[gridPar_]);
+            enqueue_force();
+            adjoint_save_slot(j + 1);
+            enqueue_kick(false);
+        }
+        // reversed leapfrog through the segment
+        for (int j = k - 1; j >= 0; j--) {
+            const AdjSlot& sl = adjSlot_[(size_t)j];
+            const int step = c + j;
+            if (sl.sorted) {
+                launch_adj_unsort(st_, shape_, sl.dest, adjX_, adjV_, adjX2_, adjV2_);
+                std::swap(adjX_, adjX2_);
+                std::swap(adjV_, adjV2_);
+            }
+            launch_adj_direct(st_, shape_, sl.body - h.base, sl.mats - h.base,
+                              sl.force - h.desc.meshBase, sl.rec - h.desc.physBase,
+                              sl.cellBody, adjX_, adjV_, dt, adjG_, adjSpread_);
+            launch_fft_rows_forward_values(st_, dSims_, dRowsF_, rowsF_, sl.prefix - h.desc.physBase,
+                                           sl.cellBody, adjSpread_, spec_, tw_, twLen_);
+            launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_);
+            launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_);
+            launch_adj_finish(st_, shape_, sl.body - h.base, sl.mats - h.base, spec_, adjG_, adjX_,
+                              adjV_, dt, step == 0 ? 0.5f * dt : dt);
+        }
+    }
+    check(hipGetLastError(), "adjoint");
+    if (dLdx0)
+        check(hipMemcpyAsync(dLdx0, adjX_, n * sizeof(float2), hipMemcpyDeviceToHost, st_), "adjoint");
+    if (dLdv0)
+        check(hipMemcpyAsync(dLdv0, adjV_, n * sizeof(float2), hipMemcpyDeviceToHost, st_), "adjoint");
+    bodyPar_ = bp;
+    auxPar_ = ap;
+    gridPar_ = gp;
+    h.steps = finalSteps;
+    check(hipMemcpyAsync(body_[bodyPar_] + h.base, adjFinal_.body, cap * sizeof(float4),
+                         hipMemcpyDeviceToDevice, st_), "adjoint");
+    check(hipMemcpyAsync(acc_[auxPar_] + h.base, adjFinal_.acc, cap * sizeof(float2),
+                         hipMemcpyDeviceToDevice, st_), "adjoint");
+    check(hipMemcpyAsync(mat_[auxPar_] + h.base, adjFinal_.mats, cap * sizeof(float4),
+                         hipMemcpyDeviceToDevice, st_), "adjoint");
+    check(hipStreamSynchronize(st_), "adjoint");
 }
 
 void Engine::initialize_acceleration() {

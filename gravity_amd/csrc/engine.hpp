@@ -76,6 +76,16 @@ public:
     void step();
     void stop_bodies(int s);
     void wake(int s);
+
+    // Adjoint of the gravity step (one simulation, drift-only passes):
+    // `steps` steps like step(), keeping the state every `every` steps, then
+    // the vector-Jacobian product of those steps by checkpointed recomputation.
+    void adjoint_record(int steps, int every);
+    void adjoint_vjp(const float* dLdx, float* dLdx0, float* dLdv0);
+    // where[i]: the index now holding the body that started the recording at i
+    void adjoint_order(uint32_t* where);
+    // J_A(x)^T w of the acceleration field at the current positions
+    void debug_accel_vjp(const float* w, float* out);
     void debug_collide(float kickDt, float driftDt, bool writePressure);
     void debug_sort();
     void debug_liquid();
@@ -95,6 +105,8 @@ private:
     void enqueue_step(bool sortDue);
     void enqueue_sort();
     void enqueue_force_and_kick(bool initial);
+    void enqueue_force();
+    void enqueue_kick(bool initial);
     void enqueue_pass(float kickDt, float driftDt, bool writePressure);
     void enqueue_liquid();
     void clear_grids();
@@ -150,6 +162,35 @@ private:
     // the captured schedule ends in
     struct CapturedStep { hipGraphExec_t exec; int bodyPar, auxPar, gridPar; };
     std::map<int, CapturedStep> graphs_;
+
+    // adjoint
+    struct AdjCheckpoint {
+        float4* body = nullptr;
+        float2* acc = nullptr;
+        float4* mats = nullptr;
+        long long steps = 0;
+    };
+    struct AdjSlot {
+        float4* body = nullptr;
+        float4* mats = nullptr;
+        float2* force = nullptr;
+        uint4* rec = nullptr;
+        uint32_t* prefix = nullptr;
+        float4* cellBody = nullptr;
+        uint32_t* dest = nullptr;
+        bool sorted = false;
+    };
+    std::vector<AdjCheckpoint> adjCk_;
+    std::vector<AdjSlot> adjSlot_;
+    std::vector<char> adjSortDue_;
+    int adjSteps_ = 0, adjEvery_ = 0;
+    float2 *adjX_ = nullptr, *adjV_ = nullptr, *adjX2_ = nullptr, *adjV2_ = nullptr;
+    float2 *adjG_ = nullptr, *adjSpread_ = nullptr;
+    uint32_t* adjWhere_ = nullptr;
+    AdjCheckpoint adjFinal_;
+    std::vector<void*> adjAllocs_;
+    void adjoint_free();
+    void adjoint_save_slot(int j);
 
     // profiling
     bool profiling_ = false;

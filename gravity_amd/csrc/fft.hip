@@ -270,6 +270,73 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_inverse(const SimDesc* __re
     }
 }
 
+// Adjoint of the force gather (the transposed CIC interpolation): every body
+// spreads its float2 value over its four CIC nodes (no mass), with float LDS
+// atomics, then the rows are transformed forward as in k_fft_rows_forward.
+__global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward_values(
+    const SimDesc* __restrict__ sims, const LineWork* __restrict__ work,
+    const uint32_t* __restrict__ cellPrefix, const float4* __restrict__ cellBody,
+    const float2* __restrict__ value, float2* __restrict__ spec, const float2* __restrict__ tw,
+    int twLen) {
+    __shared__ float2 data[ROW_LDS];
+    LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[w.sim];
+    const int L = s.mw;
+    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) data[i] = make_float2(0.0f, 0.0f);
+    __syncthreads();
+    for (int l = 0; l < w.count; l++) {
+        int y = mesh_row(s, w.first + l);
+        int r0, r1;
+        if (s.bounded) {
+            if (y == s.mh - 1) { r0 = 0; r1 = 0; }
+            else { r0 = y - 1 < 0 ? 0 : y - 1; r1 = y + 1 > s.ph - 1 ? s.ph - 1 : y + 1; }
+        } else if (s.ph < 3) {
+            r0 = 0; r1 = s.ph - 1;
+        } else {
+            r0 = y - 1; r1 = y + 1;
+        }
+        for (int rr = r0; rr <= r1; rr++) {
+            int r = s.bounded || s.ph < 3 ? rr : wrap_axis(rr, s.ph);
+            int c0 = s.physBase + r * s.pw;
+            int t0 = (int)cellPrefix[c0], t1 = (int)cellPrefix[c0 + s.pw];
+            for (int t = t0 + (int)threadIdx.x; t < t1; t += ROW_NT) {
+                float4 bd = cellBody[t];
+                float2 v = value[__float_as_uint(bd.w)];
+                float gx = bd.x * (float)s.pw - 0.5f, gy = bd.y * (float)s.ph - 0.5f;
+                float flx = floorf(gx), fly = floorf(gy);
+                int bx = (int)flx, by = (int)fly;
+                float fx = gx - flx, fy = gy - fly;
+                float wgt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy,
+                                fx * fy};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    int my = wrap_axis(by + (k >> 1), s.mh);
+                    if (my != y) continue;
+                    int mx = wrap_axis(bx + (k & 1), s.mw);
+                    atomicAdd(&data[l * L + mx].x, wgt[k] * v.x);
+                    atomicAdd(&data[l * L + mx].y, wgt[k] * v.y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    lds_fft<-1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, tw, twLen);
+    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
+        int l = i / L, x = i - l * L;
+        int y = mesh_row(s, w.first + l);
+        spec[s.meshBase + y * s.mw + x] = data[i];
+    }
+}
+
+void launch_fft_rows_forward_values(hipStream_t st, const SimDesc* sims, const LineWork* work,
+                                    int items, const uint32_t* cellPrefix, const float4* cellBody,
+                                    const float2* value, float2* spec, const float2* twiddles,
+                                    int twiddleLength) {
+    if (items == 0) return;
+    hipLaunchKernelGGL(k_fft_rows_forward_values, dim3(items), dim3(ROW_NT), 0, st, sims, work,
+                       cellPrefix, cellBody, value, spec, twiddles, twiddleLength);
+}
+
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
                              const float2* twiddles, int twiddleLength) {

@@ -80,6 +80,23 @@ void launch_sleep_advance(hipStream_t st, const SimDesc* sims, int simCount,
                           const BodyBlock* tileBlocks, int tileBlockCount, const SleepBufs& sleep);
 void launch_sleep_wake(hipStream_t st, const SleepBufs& sleep, int sim, int steps);
 
+// ---- adjoint of the gravity step ----
+void launch_fft_rows_forward_values(hipStream_t st, const SimDesc* sims, const LineWork* work,
+                                    int items, const uint32_t* cellPrefix, const float4* cellBody,
+                                    const float2* value, float2* spec, const float2* twiddles,
+                                    int twiddleLength);
+void launch_adj_direct(hipStream_t st, const LaunchShape& ls, const float4* body,
+                       const float4* mats, const float2* force, const uint4* cellRec,
+                       const float4* cellBody, const float2* xbar, const float2* vbar, float dt,
+                       float2* gbar, float2* spread);
+void launch_adj_finish(hipStream_t st, const LaunchShape& ls, const float4* body,
+                       const float4* mats, const float2* result, const float2* gbar, float2* xbar,
+                       float2* vbar, float dt, float coef);
+void launch_adj_unsort(hipStream_t st, const LaunchShape& ls, const uint32_t* dest,
+                       const float2* xin, const float2* vin, float2* xout, float2* vout);
+void launch_adj_compose(hipStream_t st, const LaunchShape& ls, const uint32_t* dest,
+                        uint32_t* where);
+
 // ---- position-based liquid ----
 void launch_liquid_bin(hipStream_t st, const LaunchShape& ls, const float4* body,
                        uint32_t* liqCount, uint32_t* cellOf);

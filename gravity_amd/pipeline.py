@@ -268,6 +268,45 @@ class GravityPipeline:
     def sortBodies(self, activeBodies: int) -> None:
         native.check(self._lib.gs_debug_sort(self._h, int(activeBodies)))
 
+    # -- adjoint (no reference counterpart) ---------------------------------
+    def adjointRecord(self, activeBodies: int, steps: int, checkpointEvery: int = 8) -> None:
+        """`steps` steps like step(), keeping a checkpoint every checkpointEvery."""
+        native.check(self._lib.gs_adjoint_record(self._h, int(activeBodies), int(steps),
+                                                 int(checkpointEvery)))
+
+    def adjointVjp(self, dLdx: np.ndarray, count: int) -> tuple[np.ndarray, np.ndarray]:
+        """(dL/dx0, dL/dv0) of the recorded steps for dL/dx at their end."""
+        g = np.ascontiguousarray(dLdx, dtype=np.float32).reshape(-1)
+        x0 = np.empty(2 * count, np.float32)
+        v0 = np.empty(2 * count, np.float32)
+        native.check(self._lib.gs_adjoint_vjp(self._h, native.ptr(g), native.ptr(x0), native.ptr(v0)))
+        return x0.reshape(count, 2), v0.reshape(count, 2)
+
+    def adjointOrder(self, count: int) -> np.ndarray:
+        out = np.empty(count, np.uint32)
+        native.check(self._lib.gs_adjoint_order(self._h, native.ptr(out)))
+        return out
+
+    def vjpInitialVelocity(self, activeBodies: int, steps: int, checkpointEvery: int,
+                           targets: np.ndarray) -> tuple[float, np.ndarray]:
+        """Loss mean |x_T - target|^2 (px^2, minimum image) and its gradient
+        with respect to the initial normalised velocities."""
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        grad = np.empty(2 * activeBodies, np.float32)
+        loss = ctypes.c_float()
+        native.check(self._lib.gs_vjp_initial_velocity(self._h, int(activeBodies), int(steps),
+                                                       int(checkpointEvery), native.ptr(t),
+                                                       ctypes.byref(loss), native.ptr(grad)))
+        return float(loss.value), grad.reshape(activeBodies, 2)
+
+    def accelerationVjp(self, activeBodies: int, w: np.ndarray) -> np.ndarray:
+        """J_A(x)^T w at the current positions (test hook of the adjoint)."""
+        wv = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+        out = np.empty(2 * activeBodies, np.float32)
+        native.check(self._lib.gs_debug_acceleration_vjp(self._h, int(activeBodies), native.ptr(wv),
+                                                         native.ptr(out)))
+        return out.reshape(activeBodies, 2)
+
     def resolveLiquid(self, activeBodies: int) -> None:
         """The liquid solve of a step on its own (the reflection hook
         resolveLiquid, T/CollisionGpuSupport.kt:68-90)."""

@@ -191,6 +191,35 @@ int gs_batch_download(gs_sim *sim, int index, int what, int first, int count, vo
 int gs_batch_initialize_acceleration(gs_sim *sim);
 int gs_batch_step(gs_sim *sim);
 
+/* Adjoint of the gravity step. No reference counterpart: the reference has
+ * no derivatives. For a single simulation without contact passes
+ * (collisions_enabled = 0), with mutual gravity, torus or walls (a wall's
+ * clamp is differentiated as the identity).
+ *
+ * gs_adjoint_record: `steps` steps exactly like gs_step from the current
+ * state (its acceleration initialised), keeping the state every
+ * checkpoint_every steps. gs_adjoint_vjp: the vector-Jacobian product of
+ * those steps by reversed leapfrog, recomputing one checkpoint interval at a
+ * time: given dL/dx_T (float[2 * n], normalised units, the final body
+ * order) it writes dL/dx_0 and dL/dv_0 (float[2 * n] each, nullable), with
+ * respect to the normalised positions and velocities of the recording's
+ * start, in its body order; the handle keeps the final state. The spatial
+ * sorts of the recording are undone on the way back. gs_adjoint_order:
+ * where[i] = the index that now holds the body that started at index i. */
+int gs_adjoint_record(gs_sim *sim, int active, int steps, int checkpoint_every);
+int gs_adjoint_vjp(gs_sim *sim, const float *dL_dx, float *dL_dx0, float *dL_dv0);
+int gs_adjoint_order(gs_sim *sim, uint32_t *where);
+/* Record + loss L = mean |minimum image((x_T - target) * viewport)|^2 (px^2,
+ * targets float[2 * n] normalised, in the start order) + VJP: writes L and
+ * dL/dv_0 (float[2 * n], normalised units, start order). */
+int gs_vjp_initial_velocity(gs_sim *sim, int active, int steps, int checkpoint_every,
+                            const float *targets, float *loss, float *dL_dv0);
+
+/* Test hook of the adjoint: out = J_A(x)^T w, the transposed Jacobian of the
+ * accelerations (force build + kick, normalised units) at the current
+ * positions of `count` bodies, applied to w (float[2 * count] each). */
+int gs_debug_acceleration_vjp(gs_sim *sim, int count, const float *w, float *out);
+
 /* Message of the last failure on this thread, or "" (Kotlin exception text). */
 const char *gs_last_error(void);
 

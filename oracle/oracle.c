@@ -1257,3 +1257,223 @@ void orc_sim_resolve_liquid(orc_sim *s) {
     free(pos); free(lam); free(out); free(pout); free(nb); free(nstart);
     free(cell); free(start); free(items);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Reference for the adjoint: the gravity-only leapfrog in FP64             */
+/* (K/GravityPipeline.kt:445-494 with drift-only passes, buildForce :907-973, */
+/* applyKick :1213-1261). The same P3M force as the FP32 path (Gaussian-split */
+/* kernel, CIC deposit and gather, the same deconvolution, the near field of */
+/* S/kick.comp with the same topology rules), evaluated in double with the  */
+/* CIC deposit unrounded, so its central finite differences give the FP64   */
+/* gradient the HIP adjoint is checked against. No spatial sort: it only    */
+/* permutes the bodies.                                                     */
+
+typedef struct {
+    int mw, mh, pw, ph, n, rx, ry;
+    double aspect, G, total_mass, sigma, md2;
+    double *tw, *kx, *ky, *rho, *fx, *fy;      /* complex meshes, FP64 spectra of the kernel */
+    int *cell_start, *cell_list;
+    double *cell_mass;
+} grav64;
+
+static void g64_init(grav64 *g, const orc_params *p, int n) {
+    g->pw = p->grid_w; g->ph = p->grid_h;
+    g->mw = p->bounded ? 2 * g->pw : g->pw;
+    g->mh = p->bounded ? 2 * g->ph : g->ph;
+    g->n = n;
+    g->aspect = (double)p->width / (double)p->height;
+    g->G = p->gravity;
+    g->total_mass = n > 1 ? n : 1;
+    g->sigma = (double)p->softening_cells / (double)g->ph;
+    double cw = g->aspect / g->pw, ch = 1.0 / g->ph;
+    double md = p->min_dist_frac * (cw < ch ? cw : ch);
+    g->md2 = md * md;
+    orc_neighbor_radius(p, &g->rx, &g->ry);
+    size_t m = (size_t)g->mw * g->mh;
+    int max_axis = g->mw > g->mh ? g->mw : g->mh;
+    g->tw = malloc(sizeof(double) * 4 * (size_t)max_axis);
+    orc_twiddles_f64(max_axis, g->tw);
+    g->kx = calloc(2 * m, sizeof(double)); g->ky = calloc(2 * m, sizeof(double));
+    g->rho = malloc(sizeof(double) * 2 * m);
+    g->fx = malloc(sizeof(double) * 2 * m); g->fy = malloc(sizeof(double) * 2 * m);
+    const double PI = 3.141592653589793238463;
+    for (size_t id = 0; id < m; id++) {
+        int rx = (int)(id % g->mw), ry = (int)(id / g->mw);
+        int ox = rx <= g->mw / 2 ? rx : rx - g->mw;
+        int oy = ry <= g->mh / 2 ? ry : ry - g->mh;
+        double dx = ox * g->aspect / g->pw, dy = (double)oy / g->ph;
+        double r2 = dx * dx + dy * dy;
+        if (r2 == 0.0) continue;
+        double lw = 1.0 - exp(-r2 / (2.0 * g->sigma * g->sigma));
+        double dirx = rx * 2 == g->mw ? 0.0 : dx, diry = ry * 2 == g->mh ? 0.0 : dy;
+        double scale = -g->G * lw / (g->total_mass * r2);
+        g->kx[2 * id] = scale * dirx / g->aspect;
+        g->ky[2 * id] = scale * diry;
+    }
+    orc_fft2d_f64(g->kx, g->mw, g->mh, g->tw, 0);
+    orc_fft2d_f64(g->ky, g->mw, g->mh, g->tw, 0);
+    /* S/convolve.comp:24-47 folded into the spectra */
+    for (size_t id = 0; id < m; id++) {
+        int rx = (int)(id % g->mw), ry = (int)(id / g->mw);
+        int ikx = rx <= g->mw / 2 ? rx : rx - g->mw, iky = ry <= g->mh / 2 ? ry : ry - g->mh;
+        double ax = PI * ikx / g->mw, ay = PI * iky / g->mh;
+        double sx = fabs(ax) < 1e-12 ? 1.0 : sin(ax) / ax, sy = fabs(ay) < 1e-12 ? 1.0 : sin(ay) / ay;
+        double aw = sx * sx * sy * sy, p2 = aw * aw;
+        double s = 1.0 / (p2 > p->window_floor ? p2 : p->window_floor) / (double)m;
+        g->kx[2 * id] *= s; g->kx[2 * id + 1] *= s;
+        g->ky[2 * id] *= s; g->ky[2 * id + 1] *= s;
+    }
+    size_t cells = (size_t)g->pw * g->ph;
+    g->cell_start = malloc(sizeof(int) * (cells + 1));
+    g->cell_list = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    g->cell_mass = malloc(sizeof(double) * cells);
+}
+
+static void g64_free(grav64 *g) {
+    free(g->tw); free(g->kx); free(g->ky); free(g->rho); free(g->fx); free(g->fy);
+    free(g->cell_start); free(g->cell_list); free(g->cell_mass);
+}
+
+static double g64_image(const orc_params *p, double d) { return p->bounded ? d : d - floor(d + 0.5); }
+
+static void g64_local(const grav64 *g, const orc_params *p, double sx, double sy, double tx,
+                      double ty, double mass, double *ax, double *ay) {
+    double px = g64_image(p, sx - tx) * g->aspect, py = g64_image(p, sy - ty);
+    double r2 = px * px + py * py;
+    double lw = exp(-r2 / (2.0 * g->sigma * g->sigma));
+    double scale = g->G * mass * lw / (g->total_mass * (r2 > g->md2 ? r2 : g->md2));
+    *ax += scale * px / g->aspect;
+    *ay += scale * py;
+}
+
+/* Accelerations of every body at positions xy (normalised), masses m. */
+static void g64_accel(grav64 *g, const orc_params *p, const double *xy, const float *mats,
+                      double *acc) {
+    size_t m = (size_t)g->mw * g->mh;
+    for (size_t i = 0; i < 2 * m; i++) g->rho[i] = 0.0;
+    size_t cells = (size_t)g->pw * g->ph;
+    int *count = calloc(cells, sizeof(int)), *home = malloc(sizeof(int) * (size_t)(g->n > 0 ? g->n : 1));
+    for (size_t c = 0; c < cells; c++) g->cell_mass[c] = 0.0;
+    for (int i = 0; i < g->n; i++) {
+        double x = xy[2 * i], y = xy[2 * i + 1], mass = mats[4 * i];
+        double gx = x * g->pw - 0.5, gy = y * g->ph - 0.5;
+        double flx = floor(gx), fly = floor(gy);
+        int bx = (int)flx, by = (int)fly;
+        double fx = gx - flx, fy = gy - fly;
+        double w[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
+        for (int k = 0; k < 4; k++) {
+            int mx = wrap_axis(bx + (k & 1), g->mw), my = wrap_axis(by + (k >> 1), g->mh);
+            g->rho[2 * ((size_t)my * g->mw + mx)] += w[k] * mass;
+        }
+        int hx = (int)floor(x * g->pw), hy = (int)floor(y * g->ph);
+        home[i] = wrap_axis(hx, g->pw) + wrap_axis(hy, g->ph) * g->pw;
+        count[home[i]]++;
+        g->cell_mass[home[i]] += mass;
+    }
+    g->cell_start[0] = 0;
+    for (size_t c = 0; c < cells; c++) g->cell_start[c + 1] = g->cell_start[c] + count[c];
+    int *fill = malloc(sizeof(int) * cells);
+    for (size_t c = 0; c < cells; c++) fill[c] = g->cell_start[c];
+    /* descending body index within a cell, the canonical list order */
+    for (int i = g->n - 1; i >= 0; i--) g->cell_list[fill[home[i]]++] = i;
+    free(fill);
+    orc_fft2d_f64(g->rho, g->mw, g->mh, g->tw, 0);
+    for (size_t id = 0; id < m; id++) {
+        double rr = g->rho[2 * id], ri = g->rho[2 * id + 1];
+        const double *a = g->kx + 2 * id, *b = g->ky + 2 * id;
+        g->fx[2 * id] = rr * a[0] - ri * a[1];
+        g->fx[2 * id + 1] = rr * a[1] + ri * a[0];
+        g->fy[2 * id] = rr * b[0] - ri * b[1];
+        g->fy[2 * id + 1] = rr * b[1] + ri * b[0];
+    }
+    orc_fft2d_f64(g->fx, g->mw, g->mh, g->tw, 1);
+    orc_fft2d_f64(g->fy, g->mw, g->mh, g->tw, 1);
+    for (int i = 0; i < g->n; i++) {
+        double x = xy[2 * i], y = xy[2 * i + 1];
+        double gx = x * g->pw - 0.5, gy = y * g->ph - 0.5;
+        double flx = floor(gx), fly = floor(gy);
+        int bx = (int)flx, by = (int)fly;
+        double fx = gx - flx, fy = gy - fly;
+        double w[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
+        double ax = 0.0, ay = 0.0;
+        for (int k = 0; k < 4; k++) {
+            int mx = wrap_axis(bx + (k & 1), g->mw), my = wrap_axis(by + (k >> 1), g->mh);
+            size_t idx = (size_t)my * g->mw + mx;
+            ax += w[k] * g->fx[2 * idx];
+            ay += w[k] * g->fy[2 * idx];
+        }
+        ax += p->field_x / p->width;
+        ay += p->field_y / p->height;
+        int hx = (int)floor(x * g->pw), hy = (int)floor(y * g->ph);
+        for (int dy = -g->ry; dy <= g->ry; dy++) {
+            for (int dx = -g->rx; dx <= g->rx; dx++) {
+                int cx = hx + dx, cy = hy + dy;
+                if (p->bounded && (cx < 0 || cy < 0 || cx >= g->pw || cy >= g->ph)) continue;
+                int wx = wrap_axis(cx, g->pw), wy = wrap_axis(cy, g->ph);
+                int index = wx + wy * g->pw;
+                int cnt = count[index];
+                int is_home = dx == 0 && dy == 0;
+                if (!is_home && cnt > p->exact_limit) {
+                    g64_local(g, p, (wx + 0.5) / g->pw, (wy + 0.5) / g->ph, x, y,
+                              g->cell_mass[index], &ax, &ay);
+                } else {
+                    int limit = is_home ? p->home_limit : p->exact_limit;
+                    int visited = 0;
+                    for (int t = g->cell_start[index]; t < g->cell_start[index + 1] && visited < limit; t++) {
+                        int other = g->cell_list[t];
+                        if (other != i)
+                            g64_local(g, p, xy[2 * other], xy[2 * other + 1], x, y, mats[4 * other], &ax, &ay);
+                        visited++;
+                    }
+                }
+            }
+        }
+        acc[2 * i] = ax;
+        acc[2 * i + 1] = ay;
+    }
+    free(count); free(home);
+}
+
+/* `steps` gravity-only leapfrog steps from bodies (normalised x, y, vx, vy),
+ * initial velocities offset by dv (nullable, normalised), in FP64. Writes
+ * the final positions (nullable) and returns the loss
+ * mean |minimum image((x_T - target) * viewport)|^2 in px^2. */
+double orc_grav64_loss(const orc_params *p, int n, const float *bodies, const float *mats,
+                       int steps, const float *targets, const double *dv, double *final_xy) {
+    grav64 g;
+    g64_init(&g, p, n);
+    size_t nn = (size_t)(n > 0 ? n : 1);
+    double *xy = malloc(16 * nn), *v = malloc(16 * nn), *a = malloc(16 * nn);
+    for (int i = 0; i < n; i++) {
+        xy[2 * i] = bodies[4 * i]; xy[2 * i + 1] = bodies[4 * i + 1];
+        v[2 * i] = bodies[4 * i + 2] + (dv ? dv[2 * i] : 0.0);
+        v[2 * i + 1] = bodies[4 * i + 3] + (dv ? dv[2 * i + 1] : 0.0);
+    }
+    double dt = p->dt, half = 0.5 * p->dt;
+    g64_accel(&g, p, xy, mats, a);
+    for (int s = 0; s < steps; s++) {
+        for (int i = 0; i < 2 * n; i++) v[i] += a[i] * half;
+        for (int i = 0; i < 2 * n; i++) {
+            double x = xy[i] + v[i] * dt;
+            if (p->bounded) x = x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x);
+            else x -= floor(x);
+            xy[i] = x;
+        }
+        g64_accel(&g, p, xy, mats, a);
+        for (int i = 0; i < 2 * n; i++) v[i] += a[i] * half;
+    }
+    double loss = 0.0;
+    for (int i = 0; i < n; i++) {
+        double dx = (xy[2 * i] - targets[2 * i]) * p->width;
+        double dy = (xy[2 * i + 1] - targets[2 * i + 1]) * p->height;
+        if (!p->bounded) {
+            dx -= floor(dx / p->width + 0.5) * p->width;
+            dy -= floor(dy / p->height + 0.5) * p->height;
+        }
+        loss += dx * dx + dy * dy;
+    }
+    if (final_xy) memcpy(final_xy, xy, 16 * (size_t)n);
+    free(xy); free(v); free(a);
+    g64_free(&g);
+    return n > 0 ? loss / n : 0.0;
+}

@@ -132,6 +132,14 @@ void orc_sim_body_awake(const orc_sim *s, uint32_t *awake);
 long orc_sim_steps(const orc_sim *s);
 void orc_sim_last_sort_dest(const orc_sim *s, uint32_t *dest);
 
+/* ---- adjoint reference: FP64 gravity-only leapfrog ----------------------- */
+/* `steps` steps from bodies with initial velocities offset by dv (nullable),
+ * in FP64 with the CIC deposit unrounded and no spatial sort; returns
+ * mean |minimum image((x_T - target) * viewport)|^2 (px^2). */
+double orc_grav64_loss(const orc_params *p, int n, const float *bodies,
+                       const float *mats, int steps, const float *targets,
+                       const double *dv, double *final_xy);
+
 /* ---- d: liquid ---------------------------------------------------------- */
 void orc_liquid_constants(float diameter, float radius_diameters, float *h,
                           float *poly6, float *spiky, float *rest_density,

@@ -80,6 +80,7 @@ def load():
             "orc_sim_initialize_acceleration": (None, [P]),
             "orc_sim_step": (None, [P]),
             "orc_sim_wake": (None, [P]),
+            "orc_grav64_loss": (ctypes.c_double, [P, I, P, P, I, P, P, P]),
             "orc_sim_sleep_stamp": (U, [P]),
             "orc_sim_body_awake": (None, [P, P]),
             "orc_sim_steps": (ctypes.c_long, [P]),
@@ -118,6 +119,18 @@ def params(width, height, *, bounded=False, mutual_gravity=True, liquid=False, c
     p.kernel_radius_diameters, p.liquid_iterations, p.viscosity = kernel_radius, liquid_iterations, viscosity
     p.sleep, p.sleep_speed = int(sleep), sleep_speed
     return p
+
+
+def grav64_loss(p: OrcParams, bodies, mats, steps, targets, dv=None, final=False):
+    """FP64 gravity-only leapfrog loss (and final positions if final=True)."""
+    b = np.ascontiguousarray(bodies, dtype=np.float32)
+    m = np.ascontiguousarray(mats, dtype=np.float32)
+    t = np.ascontiguousarray(targets, dtype=np.float32)
+    d = None if dv is None else np.ascontiguousarray(dv, dtype=np.float64)
+    out = np.empty((len(b), 2), np.float64) if final else None
+    loss = load().orc_grav64_loss(ctypes.byref(p), len(b), _p(b), _p(m), int(steps), _p(t),
+                                  _p(d) if d is not None else None, _p(out) if final else None)
+    return (loss, out) if final else loss
 
 
 def sort_key(x, y, wb, hb) -> int:
