@@ -14,7 +14,9 @@
 // The broad phase is a linked-cell grid of cells at least diameter + guard
 // wide, rebuilt every pass; heads of a pass are cleared by the next pass's
 // count kernel, so no clearing launch is needed. Two grid buffers alternate.
-// Bodies are float4 (x, y, vx, vy): a neighbour is one 16-byte load.
+// Bodies are float4 (x, y, vx, vy): a neighbour is one 16-byte load. Grid
+// nodes hold positions in pixels (x * width, y * height: the products the
+// distance test forms), so a visit is two subtractions and a sum of squares.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -22,17 +24,23 @@ namespace gs {
 
 struct GridBuf {
     uint32_t* heads;   // per cell: first body + 1, 0 when empty
-    float4* nodes;     // per body: x, y, next body + 1 (bits), unused
+    float4* nodes;     // per body: x, y in px, next body + 1 (bits), unused
     int* cellOf;       // per body: the cell it was inserted in
 };
 
+// The broad-phase cell of a normalised position.
+__device__ __forceinline__ int cell_x(const SimDesc& s, float x) {
+    return floor_index(x * (float)s.cgw, s.cgw);
+}
+__device__ __forceinline__ int cell_y(const SimDesc& s, float y) {
+    return floor_index(y * (float)s.cgh, s.cgh);
+}
+
 __device__ __forceinline__ void grid_insert(const SimDesc& s, int g, float px, float py,
                                             const GridBuf& grid) {
-    int cx = floor_index(px * (float)s.cgw, s.cgw);
-    int cy = floor_index(py * (float)s.cgh, s.cgh);
-    int c = s.cellBase + cx + cy * s.cgw;
+    int c = s.cellBase + cell_x(s, px) + cell_y(s, py) * s.cgw;
     uint32_t old = atomicExch(&grid.heads[c], (uint32_t)(g + 1));
-    grid.nodes[g] = make_float4(px, py, __uint_as_float(old), 0.0f);
+    grid.nodes[g] = make_float4(px * s.vw, py * s.vh, __uint_as_float(old), 0.0f);
     grid.cellOf[g] = c;
 }
 
@@ -41,34 +49,36 @@ __device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c) {
 }
 
 // S/collision_crowd.glsl:2-15: the lower index goes first; squares summed
-// without contraction. Without the torus's minimum image the order cannot
-// matter: fl(a - b) = -fl(b - a) exactly, so the squares are the same.
-__device__ __forceinline__ float pair_distance_squared(int id, float px, float py, int other,
-                                                       float ox, float oy, const SimDesc& s) {
-    if (s.bounded) {
-        float dx = px * s.vw - ox * s.vw;
-        float dy = py * s.vh - oy * s.vh;
+// without contraction. Positions are in pixels (the products x * width and
+// y * height the reference forms first). Without the torus's minimum image the
+// order cannot matter: fl(a - b) = -fl(b - a) exactly, so the squares are the
+// same.
+template <bool BOUNDED>
+__device__ __forceinline__ float pair_d2(int id, float px, float py, int other, float ox,
+                                         float oy, float vw, float vh) {
+    if (BOUNDED) {
+        float dx = px - ox;
+        float dy = py - oy;
         return dx * dx + dy * dy;
     }
     bool ownFirst = id < other;
-    float ax = ownFirst ? px : ox, ay = ownFirst ? py : oy;
-    float bx = ownFirst ? ox : px, by = ownFirst ? oy : py;
-    float dx = ax * s.vw - bx * s.vw;
-    float dy = ay * s.vh - by * s.vh;
-    if (!s.bounded) {
-        dx -= floorf(dx / s.vw + 0.5f) * s.vw;
-        dy -= floorf(dy / s.vh + 0.5f) * s.vh;
-    }
+    float dx = ownFirst ? px - ox : ox - px;
+    float dy = ownFirst ? py - oy : oy - py;
+    dx -= floorf(dx / vw + 0.5f) * vw;
+    dy -= floorf(dy / vh + 0.5f) * vh;
     return dx * dx + dy * dy;
 }
 
-// The heads of the 3x3 stencil around the cell of (px, py), loaded together
-// (independent loads); cells cut by a wall, or repeated on a grid narrower
-// than 3, read as empty.
-__device__ __forceinline__ void stencil_heads(const SimDesc& s, float px, float py,
+__device__ __forceinline__ float pair_distance_squared(int id, float px, float py, int other,
+                                                       float ox, float oy, const SimDesc& s) {
+    return s.bounded ? pair_d2<true>(id, px, py, other, ox, oy, s.vw, s.vh)
+                     : pair_d2<false>(id, px, py, other, ox, oy, s.vw, s.vh);
+}
+
+// The heads of the 3x3 stencil around cell (cx, cy), cell by cell; cells
+// cut by a wall, or repeated on a grid narrower than 3, read as empty.
+__device__ __forceinline__ void stencil_heads(const SimDesc& s, int cx, int cy,
                                               const GridBuf& grid, uint32_t (&head)[9]) {
-    int cx = floor_index(px * (float)s.cgw, s.cgw);
-    int cy = floor_index(py * (float)s.cgh, s.cgh);
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         int y = s.cgh < 3 ? j : cy + j - 1;
@@ -84,19 +94,96 @@ __device__ __forceinline__ void stencil_heads(const SimDesc& s, float px, float 
     }
 }
 
-// Visits every body of the stencil (the callers' results do not depend on
-// the order). The lists are walked one cell after another: a wave then pays
-// for the longest list of each cell over its lanes, fewer visits than
-// walking the nine lists in lockstep (the kernels are VALU-bound, and the
-// lockstep walk's registers cost occupancy). guardLimit bounds the nodes
-// visited.
-template <typename F>
-__device__ __forceinline__ void walk_stencil(const SimDesc& s, float px, float py,
-                                             const GridBuf& grid, int guardLimit, F&& visit) {
-    uint32_t head[9];
-    stencil_heads(s, px, py, grid, head);
-    int guard = 0;
+// The heads of three neighbouring cells of a grid row, one 12-byte load.
+struct HeadRow {
+    uint32_t a, b, c;
+};
+
+// The heads of the 3x3 stencil around cell (cx, cy) of a grid at least 3
+// cells wide and high, one load per row: the heads array has a zero entry
+// before its first cell and after its last, so a row's load stays in the
+// allocation; cells past a wall read as empty. On the torus a row at the left
+// or right edge wraps and is loaded cell by cell.
+template <bool BOUNDED>
+__device__ __forceinline__ void stencil_rows(const SimDesc& s, int cx, int cy,
+                                             const uint32_t* __restrict__ heads,
+                                             uint32_t (&e)[9]) {
+    const int cgw = s.cgw, cgh = s.cgh;
+    const bool edge = !BOUNDED && (cx == 0 || cx == cgw - 1);
+    const int xl = cx == 0 ? cgw - 1 : cx - 1;
+    const int xr = cx == cgw - 1 ? 0 : cx + 1;
 #pragma unroll
+    for (int j = 0; j < 3; j++) {
+        int y = cy + j - 1;
+        bool oky = true;
+        if (BOUNDED) oky = y >= 0 && y < cgh;
+        else y = wrap_axis(y, cgh);
+        const uint32_t* row = heads + s.cellBase + y * cgw;
+        HeadRow h{0u, 0u, 0u};
+        if (oky) {
+            if (edge) {
+                h.a = row[xl];
+                h.b = row[cx];
+                h.c = row[xr];
+            } else {
+                h = *reinterpret_cast<const HeadRow*>(row + cx - 1);
+            }
+        }
+        e[j * 3 + 0] = BOUNDED && cx == 0 ? 0u : h.a;
+        e[j * 3 + 1] = h.b;
+        e[j * 3 + 2] = BOUNDED && cx == cgw - 1 ? 0u : h.c;
+    }
+}
+
+// Visits every body of the stencil around cell (cx, cy) (the callers'
+// results do not depend on the order), with its position in pixels. The
+// nine lists are walked together, one node of each per round: a round's node
+// loads are independent, so a lane pays one memory round trip per round, and
+// the rounds number the longest of its nine lists (at most four on a packed
+// disc of diameter-wide cells), where walking the lists one after another
+// chains a round trip per node. guardLimit bounds the nodes visited.
+template <bool BOUNDED, typename F>
+__device__ __forceinline__ void walk_stencil(const SimDesc& s, int cx, int cy, const GridBuf& grid,
+                                             int guardLimit, F&& visit) {
+    uint32_t e[9];
+    if (s.cgw >= 3 && s.cgh >= 3) {
+        stencil_rows<BOUNDED>(s, cx, cy, grid.heads, e);
+    } else {
+        stencil_heads(s, cx, cy, grid, e);
+    }
+    int guard = 0;
+    for (;;) {
+        uint32_t live = 0u;
+#pragma unroll
+        for (int k = 0; k < 9; k++) live |= e[k];
+        if (live == 0u || guard >= guardLimit) break;
+        float4 nd[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            if (e[k] != 0u) nd[k] = grid.nodes[e[k] - 1u];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            if (e[k] != 0u) {
+                int other = (int)e[k] - 1;
+                e[k] = __float_as_uint(nd[k].z);
+                guard++;
+                visit(other, nd[k].x, nd[k].y);
+            }
+        }
+    }
+}
+
+// The same visits, one list after another: fewer registers, for the rare
+// walks (a body with more contacts than slots) where the nine lists' nodes
+// need not be in flight together.
+template <typename F>
+__device__ __forceinline__ void walk_stencil_serial(const SimDesc& s, int cx, int cy,
+                                                    const GridBuf& grid, int guardLimit,
+                                                    F&& visit) {
+    uint32_t head[9];
+    stencil_heads(s, cx, cy, grid, head);
+    int guard = 0;
+#pragma unroll 1
     for (int k = 0; k < 9; k++) {
         uint32_t e = head[k];
         while (e != 0u && guard < guardLimit) {
@@ -168,9 +255,44 @@ __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__
 // ---------------------------------------------------------------------------
 // Count (S/collision_cache_count.comp:61-95, 130-285 without RIGID): every
 // hard pair with d^2 < D^2, strictly.
+template <bool BOUNDED>
+__device__ __forceinline__ void count_contacts(const SimDesc& s, int g, float4 pos,
+                                               const GridBuf& grid,
+                                               const float4* __restrict__ mats,
+                                               uint32_t* __restrict__ counts,
+                                               uint32_t* __restrict__ contacts, int planeStride,
+                                               int guardLimit) {
+    const float4 own = grid.nodes[g];
+    const float px = own.x, py = own.y, vw = s.vw, vh = s.vh, D2 = s.diameter2;
+    const bool ownFluid = s.liquid && mats[g].z == 0.0f;
+    uint32_t slot[CONTACT_SLOTS];
+#pragma unroll
+    for (int k = 0; k < CONTACT_SLOTS; k++) slot[k] = 0xffffffffu;
+    uint32_t n = 0;
+    walk_stencil<BOUNDED>(s, cell_x(s, pos.x), cell_y(s, pos.y), grid, guardLimit,
+                          [&](int other, float ox, float oy) {
+        float d2 = pair_d2<BOUNDED>(g, px, py, other, ox, oy, vw, vh);
+        if (!(d2 < D2) || other == g) return;
+        if (ownFluid && mats[other].z == 0.0f) return;
+        n++;
+        // Insertion keeping the CONTACT_SLOTS smallest indices, ascending:
+        // with s sorted, the new slot k is the median of s[k-1], u, s[k]
+        // (one v_med3_u32 each, all independent of one another).
+        uint32_t u = (uint32_t)other;
+#pragma unroll
+        for (int k = CONTACT_SLOTS - 1; k > 0; k--) slot[k] = med3u(slot[k - 1], u, slot[k]);
+        slot[0] = min(u, slot[0]);
+    });
+    counts[g] = n;
+#pragma unroll
+    for (int k = 0; k < CONTACT_SLOTS; k++)
+        if ((uint32_t)k < n) contacts[(size_t)k * planeStride + g] = slot[k];
+}
+
 __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
                                               const BodyBlock* __restrict__ blocks,
-                                              GridBuf grid, GridBuf prev, int clearPrev,
+                                              const float4* __restrict__ body, GridBuf grid,
+                                              GridBuf prev, int clearPrev,
                                               const float4* __restrict__ mats,
                                               uint32_t* __restrict__ counts,
                                               uint32_t* __restrict__ contacts, int planeStride,
@@ -185,31 +307,11 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
         const SleepState st = sleep.state[b.sim];
         if (sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp) return;
     }
-    float4 own = grid.nodes[g];
-    float px = own.x, py = own.y;
-    bool ownFluid = s.liquid && mats[g].z == 0.0f;
-    uint32_t slot[CONTACT_SLOTS];
-#pragma unroll
-    for (int k = 0; k < CONTACT_SLOTS; k++) slot[k] = 0xffffffffu;
-    uint32_t n = 0;
-    walk_stencil(s, px, py, grid, guardLimit, [&](int other, float ox, float oy) {
-        if (other == g) return;
-        if (ownFluid && mats[other].z == 0.0f) return;
-        float d2 = pair_distance_squared(g, px, py, other, ox, oy, s);
-        if (!(d2 < s.diameter2)) return;
-        n++;
-        // Insertion keeping the CONTACT_SLOTS smallest indices, ascending:
-        // with s sorted, the new slot k is the median of s[k-1], u, s[k]
-        // (one v_med3_u32 each, all independent of one another).
-        uint32_t u = (uint32_t)other;
-#pragma unroll
-        for (int k = CONTACT_SLOTS - 1; k > 0; k--) slot[k] = med3u(slot[k - 1], u, slot[k]);
-        slot[0] = min(u, slot[0]);
-    });
-    counts[g] = n;
-#pragma unroll
-    for (int k = 0; k < CONTACT_SLOTS; k++)
-        if ((uint32_t)k < n) contacts[(size_t)k * planeStride + g] = slot[k];
+    const float4 pos = body[g];
+    if (s.bounded)
+        count_contacts<true>(s, g, pos, grid, mats, counts, contacts, planeStride, guardLimit);
+    else
+        count_contacts<false>(s, g, pos, grid, mats, counts, contacts, planeStride, guardLimit);
 }
 
 // ---------------------------------------------------------------------------
@@ -396,12 +498,30 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     int listed = own < (uint32_t)CONTACT_SLOTS ? (int)own : CONTACT_SLOTS;
     uint32_t last = 0u;
     const float2* __restrict__ matXY = reinterpret_cast<const float2*>(mats);
+    // A batch's ids are in registers before its loads: the first batch's
+    // are loaded with the body's own data, before its count arrives, and
+    // each batch loads the next one's with its own loads. Slots past the
+    // count hold ids of earlier passes (or nothing yet): clamped into the
+    // simulation's bodies, their loads stay in bounds, and their results are
+    // not used.
+    const uint32_t idLo = (uint32_t)s.base, idHi = (uint32_t)(s.base + s.n - 1);
+    uint32_t ahead[SOLVE_BATCH];
+#pragma unroll
+    for (int q = 0; q < SOLVE_BATCH; q++)
+        ahead[q] = min(max(contacts[(size_t)q * planeStride + g], idLo), idHi);
 #pragma unroll 1
     for (int i0 = 0; i0 < listed; i0 += SOLVE_BATCH) {
         uint32_t id[SOLVE_BATCH];
 #pragma unroll
-        for (int q = 0; q < SOLVE_BATCH; q++)
-            id[q] = i0 + q < listed ? contacts[(size_t)(i0 + q) * planeStride + g] : (uint32_t)g;
+        for (int q = 0; q < SOLVE_BATCH; q++) id[q] = ahead[q];
+        if (i0 + SOLVE_BATCH < listed) {
+#pragma unroll
+            for (int q = 0; q < SOLVE_BATCH; q++) {
+                int k = i0 + SOLVE_BATCH + q;
+                if (k < CONTACT_SLOTS)
+                    ahead[q] = min(max(contacts[(size_t)k * planeStride + g], idLo), idHi);
+            }
+        }
         float4 ob[SOLVE_BATCH];
         float2 mf[SOLVE_BATCH];
         uint32_t oc[SOLVE_BATCH];
@@ -429,11 +549,12 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
         bool ownFluid = s.liquid && ownMat.z == 0.0f;
         for (uint32_t rest = own - (uint32_t)CONTACT_SLOTS; rest > 0u; rest--) {
             uint32_t best = 0xffffffffu;
-            walk_stencil(s, me.x, me.y, grid, guardLimit, [&](int other, float ox, float oy) {
+            walk_stencil_serial(s, cell_x(s, me.x), cell_y(s, me.y), grid, guardLimit,
+                                [&](int other, float ox, float oy) {
                 uint32_t u = (uint32_t)other;
                 if (other == g || u <= last || u >= best) return;
                 if (ownFluid && mats[other].z == 0.0f) return;
-                float d2 = pair_distance_squared(g, me.x, me.y, other, ox, oy, s);
+                float d2 = pair_distance_squared(g, ppx, ppy, other, ox, oy, s);
                 if (d2 < s.diameter2) best = u;
             });
             if (best == 0xffffffffu) break;
@@ -533,11 +654,11 @@ void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, co
                        body, acc, mats, kickDt, driftDt, insert, grid_of(grid), ls.sleep);
 }
 
-void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
-                  const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
-                  uint32_t* contacts) {
+void launch_count(hipStream_t st, const LaunchShape& ls, const float4* body,
+                  const CollideGrid& grid, const CollideGrid& prev, int clearPrev,
+                  const float4* mats, uint32_t* counts, uint32_t* contacts) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    hipLaunchKernelGGL(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body,
                        grid_of(grid), grid_of(prev), clearPrev, mats, counts, contacts,
                        ls.planeStride, ls.guardLimit, ls.sleep);
 }

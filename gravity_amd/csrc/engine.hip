@@ -154,7 +154,10 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
         body_[b] = dalloc<float4>(S);
         acc_[b] = dalloc<float2>(S);
         mat_[b] = dalloc<float4>(S);
-        grid_[b].heads = dalloc<uint32_t>((size_t)totalCells_);
+        // one zero entry before the first cell and after the last, for the
+        // stencil's row loads (the allocation is padded to keep alignment)
+        headsAlloc_[b] = dalloc<uint32_t>((size_t)totalCells_ + 2 * HEADS_PAD);
+        grid_[b].heads = headsAlloc_[b] + HEADS_PAD;
         grid_[b].nodes = dalloc<float4>(S);
         grid_[b].cellOf = dalloc<int>(S);
     }
@@ -252,7 +255,7 @@ Engine::~Engine() {
         hipEventDestroy(e.second.second);
     }
     void* ptrs[] = {body_[0], body_[1], acc_[0], acc_[1], mat_[0], mat_[1],
-                    grid_[0].heads, grid_[1].heads, grid_[0].nodes, grid_[1].nodes,
+                    headsAlloc_[0], headsAlloc_[1], grid_[0].nodes, grid_[1].nodes,
                     grid_[0].cellOf, grid_[1].cellOf, press_, counts_, contacts_, tileCount_,
                     tilePrefix_, keyOf_, slotSrc_, dest_, scanTmp_, cellCount_, cellPrefix_,
                     homeOf_, slotBody_, cellRec_, cellBody_, spec_, green_, tw_, fallback_,
@@ -654,7 +657,8 @@ void Engine::enqueue_pass(float kickDt, float driftDt, bool writePressure) {
                             driftDt, 1, grid_[g]);
     });
     timed("count", [&] {
-        launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+        launch_count(st_, shape_, body_[bodyPar_], grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_,
+                         contacts_);
     });
     timed("solve", [&] {
         launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
@@ -680,7 +684,8 @@ void Engine::enqueue_step(bool sortDue) {
         for (int k = 0; k < S; k++) {
             int g = gridPar_;
             timed("count", [&] {
-                launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+                launch_count(st_, shape_, body_[bodyPar_], grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_,
+                         contacts_);
             });
             bool last = k == S - 1;
             float nextDrift = last ? 0.0f : sub;
@@ -704,7 +709,8 @@ void Engine::enqueue_step(bool sortDue) {
         // The final zero-drift pass writes pressures (K/GravityPipeline.kt:480-486).
         int g = gridPar_;
         timed("count", [&] {
-            launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+            launch_count(st_, shape_, body_[bodyPar_], grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_,
+                         contacts_);
         });
         timed("solve", [&] {
             launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],

@@ -130,6 +130,8 @@ private:
     uint32_t* counts_ = nullptr;
     uint32_t* contacts_ = nullptr;
     CollideGrid grid_[2] = {};
+    static constexpr size_t HEADS_PAD = 64;   // zero heads before and after the cells
+    uint32_t* headsAlloc_[2] = {nullptr, nullptr};
     // sort
     uint32_t *tileCount_ = nullptr, *tilePrefix_ = nullptr, *keyOf_ = nullptr;
     uint32_t *slotSrc_ = nullptr, *dest_ = nullptr, *scanTmp_ = nullptr;

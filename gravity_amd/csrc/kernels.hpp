@@ -7,8 +7,8 @@ namespace gs {
 
 // Broad-phase grid buffer of one pass parity.
 struct CollideGrid {
-    uint32_t* heads;   // per cell: first body + 1, or 0
-    float4* nodes;     // per body: x, y, next body + 1 (bits), unused
+    uint32_t* heads;   // per cell: first body + 1, or 0; heads[-1] and heads[cells] are 0
+    float4* nodes;     // per body: x, y in px, next body + 1 (bits), unused
     int* cellOf;       // per body: the cell of its insertion
 };
 
@@ -47,9 +47,9 @@ void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b);
 void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
                          const float4* mats, float kickDt, float driftDt, int insert,
                          const CollideGrid& grid);
-void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
-                  const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
-                  uint32_t* contacts);
+void launch_count(hipStream_t st, const LaunchShape& ls, const float4* body,
+                  const CollideGrid& grid, const CollideGrid& prev, int clearPrev,
+                  const float4* mats, uint32_t* counts, uint32_t* contacts);
 void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, float4* bodyOut,
                   const float4* mats, const uint32_t* counts, const uint32_t* contacts,
                   const CollideGrid& grid, const float2* fallback, float* pressures,
