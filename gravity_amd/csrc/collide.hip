@@ -321,16 +321,19 @@ struct Accum {
     float pcx, pcy, vcx, vcy, pressure;
 };
 
-// 1/k for small contact counts, each the correctly rounded quotient (the
-// same float the division gives), read instead of dividing.
+// 1/k for small contact counts: each workgroup of the solve divides once
+// into LDS (the correctly rounded quotients) and reads the table per contact.
 constexpr int RECIP_TABLE = 64;
-struct RecipTable {
-    float v[RECIP_TABLE];
-    constexpr RecipTable() : v() {
-        for (int k = 0; k < RECIP_TABLE; k++) v[k] = k ? 1.0f / (float)k : 1.0f;
-    }
+
+// The simulation's constants a solve reads per contact, loaded once.
+struct SolveK {
+    float vw, vh, diameter, diameter2, eps, e1, subDt;
+    bool bounded, liquid;
 };
-__constant__ RecipTable c_recip = RecipTable();
+__device__ __forceinline__ SolveK solve_k(const SimDesc& s) {
+    return SolveK{s.vw, s.vh, s.diameter, s.diameter2, s.coincidentEps, 1.0f + s.restitution,
+                  s.subDt, s.bounded != 0, s.liquid != 0};
+}
 
 // S/collision_material.glsl:128-135
 __device__ __forceinline__ void friction_correction(float rvx, float rvy, float nx, float ny,
@@ -348,15 +351,15 @@ __device__ __forceinline__ void friction_correction(float rvx, float rvy, float 
 }
 
 // S/collision_cache_solve.comp:55-86
-__device__ __forceinline__ void wall_contact(const SimDesc& s, float distance, float nx, float ny,
+__device__ __forceinline__ void wall_contact(const SolveK& c, float distance, float nx, float ny,
                                              float vx, float vy, float weight, float friction,
                                              int writePressure, Accum& acc) {
-    float penetration = 0.5f * s.diameter - distance;
+    float penetration = 0.5f * c.diameter - distance;
     if (penetration <= 0.0f) return;
     acc.pcx += nx * (penetration * weight);
     acc.pcy += ny * (penetration * weight);
     float ns = vx * nx + vy * ny;
-    float e1 = 1.0f + s.restitution;
+    float e1 = c.e1;
     if (ns < 0.0f) {
         float k = e1 * ns * weight;
         acc.vcx -= nx * k;
@@ -365,19 +368,20 @@ __device__ __forceinline__ void wall_contact(const SimDesc& s, float distance, f
     if (friction > 0.0f) {
         float approach = -ns > 0.0f ? -ns : 0.0f;
         float fx, fy;
-        friction_correction(vx, vy, nx, ny, ns, weight * (e1 * approach + penetration / s.subDt),
+        friction_correction(vx, vy, nx, ny, ns, weight * (e1 * approach + penetration / c.subDt),
                             weight, friction, fx, fy);
         acc.vcx += fx;
         acc.vcy += fy;
     }
     if (writePressure) {
         float approach = -ns > 0.0f ? -ns : 0.0f;
-        acc.pressure += weight * (penetration / s.subDt + approach);
+        acc.pressure += weight * (penetration / c.subDt + approach);
     }
 }
 
 // One contact of the Jacobi sum, with the other body's data already loaded.
-__device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int other, float4 ob,
+__device__ __forceinline__ void accumulate_contact(const SolveK& s, const float* recip, int id,
+                                                   int other, float4 ob,
                                                    float4 om, uint32_t oc, bool otherSleeps,
                                                    const float2* __restrict__ fallback, float ppx,
                                                    float ppy, float vpx, float vpy, float4 ownMat,
@@ -392,7 +396,7 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
     }
     float d2 = dx * dx + dy * dy;
     if (!(d2 < s.diameter2)) return;
-    float eps = s.coincidentEps;
+    float eps = s.eps;
     float distance = sqrtf(d2 > eps ? d2 : eps);
     float nx, ny;
     if (d2 > eps) {
@@ -409,7 +413,7 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
     float penetration = s.diameter - distance;
     uint32_t crowd = ownContacts > oc ? ownContacts : oc;
     uint32_t weightCount = crowd > 1u ? crowd : 1u;
-    float pairWeight = weightCount < RECIP_TABLE ? c_recip.v[weightCount] : 1.0f / (float)weightCount;
+    float pairWeight = weightCount < RECIP_TABLE ? recip[weightCount] : 1.0f / (float)weightCount;
     // m / (m + m) is exactly 0.5 for every normal m, and sqrt(mu * mu) is
     // exactly mu for mu = 0 and for every mu in [2^-63, 2^63) (both checked
     // over all floats of those ranges): equal materials skip the division
@@ -429,7 +433,7 @@ __device__ __forceinline__ void accumulate_contact(const SimDesc& s, int id, int
     float push = share * penetration * pairWeight;
     acc.pcx += nx * push;
     acc.pcy += ny * push;
-    float e1 = 1.0f + s.restitution;
+    float e1 = s.e1;
     if (rvn < 0.0f) {
         float k = share * e1 * rvn * pairWeight;
         acc.vcx -= nx * k;
@@ -472,10 +476,14 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               float* __restrict__ pressures, int writePressure,
                                               float nextDriftDt, int insertNext, GridBuf next,
                                               SleepBufs sleep) {
+    __shared__ float recip[RECIP_TABLE];
+    if (threadIdx.x < RECIP_TABLE) recip[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 1.0f;
+    __syncthreads();
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
+    const SolveK k = solve_k(s);
     float4 me = bodyIn[g];
     bool sleeping = false;
     uint32_t stamp = 0u;
@@ -491,8 +499,8 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
         return;
     }
     float4 ownMat = mats[g];
-    float ppx = me.x * s.vw, ppy = me.y * s.vh;
-    float vpx = me.z * s.vw, vpy = me.w * s.vh;
+    float ppx = me.x * k.vw, ppy = me.y * k.vh;
+    float vpx = me.z * k.vw, vpy = me.w * k.vh;
     Accum acc{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t own = counts[g];
     int listed = own < (uint32_t)CONTACT_SLOTS ? (int)own : CONTACT_SLOTS;
@@ -535,9 +543,9 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
         for (int q = 0; q < SOLVE_BATCH; q++) {
             if (i0 + q < listed) {
                 last = id[q];
-                float solid = s.liquid ? mats[last].z : 1.0f;
+                float solid = k.liquid ? mats[last].z : 1.0f;
                 bool otherSleeps = sleeping && sleep.bodyAwake[last] != stamp;
-                accumulate_contact(s, g, (int)last, ob[q], make_float4(mf[q].x, mf[q].y, solid, 0.0f),
+                accumulate_contact(k, recip, g, (int)last, ob[q], make_float4(mf[q].x, mf[q].y, solid, 0.0f),
                                    oc[q], otherSleeps, fallback, ppx, ppy, vpx, vpy, ownMat, own,
                                    writePressure, acc);
             }
@@ -546,7 +554,7 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     if (own > (uint32_t)CONTACT_SLOTS) {
         // More contacts than slots: take the rest in ascending index, one
         // ordered walk of the grid per contact.
-        bool ownFluid = s.liquid && ownMat.z == 0.0f;
+        bool ownFluid = k.liquid && ownMat.z == 0.0f;
         for (uint32_t rest = own - (uint32_t)CONTACT_SLOTS; rest > 0u; rest--) {
             uint32_t best = 0xffffffffu;
             walk_stencil_serial(s, cell_x(s, me.x), cell_y(s, me.y), grid, guardLimit,
@@ -560,36 +568,36 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
             if (best == 0xffffffffu) break;
             last = best;
             bool otherSleeps = sleeping && sleep.bodyAwake[best] != stamp;
-            accumulate_contact(s, g, (int)best, bodyIn[best], mats[best], counts[best],
+            accumulate_contact(k, recip, g, (int)best, bodyIn[best], mats[best], counts[best],
                                otherSleeps, fallback, ppx, ppy, vpx, vpy, ownMat, own,
                                writePressure, acc);
         }
     }
-    if (s.bounded) {
-        float weight = 1.0f / (float)(own + 1u);
+    if (k.bounded) {
+        float weight = own + 1u < (uint32_t)RECIP_TABLE ? recip[own + 1u] : 1.0f / (float)(own + 1u);
         float fr = ownMat.y;
-        wall_contact(s, ppx, 1.0f, 0.0f, vpx, vpy, weight, fr, writePressure, acc);
-        wall_contact(s, s.vw - ppx, -1.0f, 0.0f, vpx, vpy, weight, fr, writePressure, acc);
-        wall_contact(s, ppy, 0.0f, 1.0f, vpx, vpy, weight, fr, writePressure, acc);
-        wall_contact(s, s.vh - ppy, 0.0f, -1.0f, vpx, vpy, weight, fr, writePressure, acc);
+        wall_contact(k, ppx, 1.0f, 0.0f, vpx, vpy, weight, fr, writePressure, acc);
+        wall_contact(k, k.vw - ppx, -1.0f, 0.0f, vpx, vpy, weight, fr, writePressure, acc);
+        wall_contact(k, ppy, 0.0f, 1.0f, vpx, vpy, weight, fr, writePressure, acc);
+        wall_contact(k, k.vh - ppy, 0.0f, -1.0f, vpx, vpy, weight, fr, writePressure, acc);
     }
     ppx += acc.pcx;
     ppy += acc.pcy;
-    vpx += acc.vcx + acc.pcx / s.subDt;
-    vpy += acc.vcy + acc.pcy / s.subDt;
+    vpx += acc.vcx + acc.pcx / k.subDt;
+    vpy += acc.vcy + acc.pcy / k.subDt;
     float4 o;
-    if (s.bounded) {
-        o.x = clampf(ppx / s.vw, 0.0f, MAXIMUM_UNIT);
-        o.y = clampf(ppy / s.vh, 0.0f, MAXIMUM_UNIT);
+    if (k.bounded) {
+        o.x = clampf(ppx / k.vw, 0.0f, MAXIMUM_UNIT);
+        o.y = clampf(ppy / k.vh, 0.0f, MAXIMUM_UNIT);
     } else {
-        ppx = glsl_mod(ppx, s.vw);
-        ppy = glsl_mod(ppy, s.vh);
-        o.x = ppx / s.vw;
-        o.y = ppy / s.vh;
+        ppx = glsl_mod(ppx, k.vw);
+        ppy = glsl_mod(ppy, k.vh);
+        o.x = ppx / k.vw;
+        o.y = ppy / k.vh;
     }
-    o.z = vpx / s.vw;
-    o.w = vpy / s.vh;
-    bool fluid = s.liquid && ownMat.z == 0.0f;
+    o.z = vpx / k.vw;
+    o.w = vpy / k.vh;
+    bool fluid = k.liquid && ownMat.z == 0.0f;
     if (writePressure && s.sleepOn) {
         // S/collision_cache_solve.comp:449-471: a body that moved this step,
         // or that nothing holds, keeps its tile awake for the next one.
