@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__
 // Count (S/collision_cache_count.comp:61-95, 130-285 without RIGID): every
 // hard pair with d^2 < D^2, strictly.
 template <bool BOUNDED>
-__device__ __forceinline__ void count_contacts(const SimDesc& s, int g, float4 pos,
+__device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, float4 pos,
                                                const GridBuf& grid,
                                                const float4* __restrict__ mats,
                                                uint32_t* __restrict__ counts,
@@ -287,31 +287,89 @@ __device__ __forceinline__ void count_contacts(const SimDesc& s, int g, float4 p
 #pragma unroll
     for (int k = 0; k < CONTACT_SLOTS; k++)
         if ((uint32_t)k < n) contacts[(size_t)k * planeStride + g] = slot[k];
+    return n;
 }
 
 __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
                                               const BodyBlock* __restrict__ blocks,
+                                              const int* __restrict__ order,
+                                              uint32_t* __restrict__ waveWork,
                                               const float4* __restrict__ body, GridBuf grid,
                                               GridBuf prev, int clearPrev,
                                               const float4* __restrict__ mats,
                                               uint32_t* __restrict__ counts,
                                               uint32_t* __restrict__ contacts, int planeStride,
                                               int guardLimit, SleepBufs sleep) {
-    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const int bi = order[blockIdx.x];
+    BodyBlock b = blocks[bi];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
-    if (g - s.base >= s.n) return;
-    if (clearPrev) prev.heads[prev.cellOf[g]] = 0u;
-    // A sleeping body keeps its last count (S/collision_cache_count.comp:147-160).
-    if (s.sleepOn) {
-        const SleepState st = sleep.state[b.sim];
-        if (sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp) return;
+    uint32_t n = 0;
+    if (g - s.base < s.n) {
+        if (clearPrev) prev.heads[prev.cellOf[g]] = 0u;
+        // A sleeping body keeps its last count (S/collision_cache_count.comp:147-160).
+        bool asleep = false;
+        if (s.sleepOn) {
+            const SleepState st = sleep.state[b.sim];
+            asleep = sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp;
+        }
+        if (!asleep) {
+            const float4 pos = body[g];
+            n = s.bounded ? count_contacts<true>(s, g, pos, grid, mats, counts, contacts,
+                                                 planeStride, guardLimit)
+                          : count_contacts<false>(s, g, pos, grid, mats, counts, contacts,
+                                                  planeStride, guardLimit);
+        }
     }
-    const float4 pos = body[g];
-    if (s.bounded)
-        count_contacts<true>(s, g, pos, grid, mats, counts, contacts, planeStride, guardLimit);
-    else
-        count_contacts<false>(s, g, pos, grid, mats, counts, contacts, planeStride, guardLimit);
+    // The wave's contacts, for the next step's dispatch order.
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n += (uint32_t)__shfl_xor((int)n, o);
+    if ((threadIdx.x & 63u) == 0u) waveWork[bi * (WG / 64) + (int)(threadIdx.x >> 6)] = n;
+}
+
+// Once per step: per XCD, its body blocks ranked by descending contacts (the
+// rank of a block is the number of blocks with a larger key; keys are
+// distinct), written back to that XCD's workgroup positions. Blocks past the
+// last full XCD round, and XCDs with more than ORDER_MAX blocks, keep the
+// xcd_block order.
+constexpr int ORDER_MAX = 512;
+__global__ void __launch_bounds__(WG) k_block_order(int nb, const uint32_t* __restrict__ waveWork,
+                                                    int* __restrict__ order) {
+    __shared__ uint32_t key[ORDER_MAX];
+    __shared__ int item[ORDER_MAX];
+    const int x = (int)blockIdx.x;
+    const int full = nb / (XCDS * XCD_CHUNK) * (XCDS * XCD_CHUNK);
+    const int m = full / XCDS;
+    if (x == 0)
+        for (int i = full + (int)threadIdx.x; i < nb; i += WG) order[i] = i;
+    if (m > ORDER_MAX) {
+        for (int j = (int)threadIdx.x; j < m; j += WG) order[x + XCDS * j] = xcd_block(x + XCDS * j, nb);
+        return;
+    }
+    for (int j = (int)threadIdx.x; j < m; j += WG) {
+        int bb = xcd_block(x + XCDS * j, nb);
+        const uint32_t* w = waveWork + (size_t)bb * (WG / 64);
+        uint32_t work = w[0] + w[1] + w[2] + w[3];
+        key[j] = (min(work, (1u << 20) - 1u) << 11) | (uint32_t)(2047 - j);
+        item[j] = bb;
+    }
+    __syncthreads();
+    for (int j = (int)threadIdx.x; j < m; j += WG) {
+        uint32_t kj = key[j];
+        int rank = 0;
+        for (int k = 0; k < m; k++) rank += key[k] > kj ? 1 : 0;
+        order[x + XCDS * rank] = item[j];
+    }
+}
+
+void launch_block_order(hipStream_t st, const LaunchShape& ls, int* order) {
+    if (ls.bodyBlocks == 0) return;
+    hipLaunchKernelGGL(k_block_order, dim3(XCDS), dim3(WG), 0, st, ls.bodyBlocks, ls.waveWork,
+                       order);
+}
+
+void identity_block_order(int blocks, int* order) {
+    for (int i = 0; i < blocks; i++) order[i] = xcd_block(i, blocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -466,6 +524,7 @@ constexpr int SOLVE_BATCH = 4;
 
 __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               const BodyBlock* __restrict__ blocks,
+                                              const int* __restrict__ order,
                                               const float4* __restrict__ bodyIn,
                                               float4* __restrict__ bodyOut,
                                               const float4* __restrict__ mats,
@@ -479,7 +538,7 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     __shared__ float recip[RECIP_TABLE];
     if (threadIdx.x < RECIP_TABLE) recip[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 1.0f;
     __syncthreads();
-    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    BodyBlock b = blocks[order[blockIdx.x]];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
@@ -666,7 +725,8 @@ void launch_count(hipStream_t st, const LaunchShape& ls, const float4* body,
                   const CollideGrid& grid, const CollideGrid& prev, int clearPrev,
                   const float4* mats, uint32_t* counts, uint32_t* contacts) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body,
+    hipLaunchKernelGGL(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
+                       ls.waveWork, body,
                        grid_of(grid), grid_of(prev), clearPrev, mats, counts, contacts,
                        ls.planeStride, ls.guardLimit, ls.sleep);
 }
@@ -676,7 +736,8 @@ void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, f
                   const CollideGrid& grid, const float2* fallback, float* pressures,
                   int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, bodyIn,
+    hipLaunchKernelGGL(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
+                       bodyIn,
                        bodyOut, mats, counts, contacts, ls.planeStride, grid_of(grid),
                        ls.guardLimit, fallback, pressures, writePressure, nextDriftDt, insertNext,
                        grid_of(next), ls.sleep);

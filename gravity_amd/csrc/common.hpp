@@ -136,7 +136,7 @@ struct BodyBlock { int sim; int first; };
 // regions spread over every XCD. Blocks past the last full round keep their
 // index (the map is a bijection of [0, n)).
 constexpr int XCD_CHUNK = 8;
-__device__ __forceinline__ int xcd_block(int i, int n) {
+__host__ __device__ inline int xcd_block(int i, int n) {
     const int round = XCDS * XCD_CHUNK;
     if (i >= n / round * round) return i;
     int xcd = i % XCDS, local = i / XCDS;

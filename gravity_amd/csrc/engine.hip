@@ -212,6 +212,8 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     dSims_ = dalloc<SimDesc>(sims_.size());
     int maxBlocks = totalSlots_ / WG;
     dBlocks_ = dalloc<BodyBlock>((size_t)maxBlocks);
+    order_ = dalloc<int>((size_t)maxBlocks);
+    waveWork_ = dalloc<uint32_t>((size_t)maxBlocks * (WG / 64));
     int maxLines = 0;
     for (auto& h : sims_) maxLines += 2 * std::max(h.cfg.grid_width, h.cfg.grid_height) * 2 + 2;
     dRowsF_ = dalloc<LineWork>((size_t)maxLines);
@@ -261,7 +263,7 @@ Engine::~Engine() {
                     homeOf_, slotBody_, cellRec_, cellBody_, spec_, green_, tw_, fallback_,
                     liqCount_, liqPrefix_, liqCellOf_, liqUnordered_, liqItems_, lambdas_,
                     sleepState_, bodyAwake_, tileAwake_, tileActive_, dTileBlocks_,
-                    dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_};
+                    dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (st_) hipStreamDestroy(st_);
@@ -412,9 +414,14 @@ void Engine::rebuild_tables() {
     rowsF_ = (int)rowsF.size();
     rowsI_ = (int)rowsI.size();
     cols_ = (int)cols.size();
-    if (!blocks.empty())
+    if (!blocks.empty()) {
         check(hipMemcpy(dBlocks_, blocks.data(), blocks.size() * sizeof(BodyBlock),
                         hipMemcpyHostToDevice), "blocks");
+        std::vector<int> order(blocks.size());
+        identity_block_order(bodyBlocks_, order.data());
+        check(hipMemcpy(order_, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice),
+              "order");
+    }
     if (!rowsF.empty()) {
         check(hipMemcpy(dRowsF_, rowsF.data(), rowsF.size() * sizeof(LineWork), hipMemcpyHostToDevice), "rows");
         check(hipMemcpy(dRowsI_, rowsI.data(), rowsI.size() * sizeof(LineWork), hipMemcpyHostToDevice), "rows");
@@ -426,6 +433,8 @@ void Engine::rebuild_tables() {
     shape_.planeStride = totalSlots_;
     shape_.guardLimit = totalSlots_ + 1;
     shape_.sleep = SleepBufs{sleepState_, bodyAwake_, tileAwake_, tileActive_};
+    shape_.order = order_;
+    shape_.waveWork = waveWork_;
     invalidate_graphs();
 }
 
@@ -719,6 +728,7 @@ void Engine::enqueue_step(bool sortDue) {
         });
         bodyPar_ ^= 1;
         gridPar_ ^= 1;
+        timed("block_order", [&] { launch_block_order(st_, shape_, order_); });
         // K/GravityPipeline.kt:487-490: the tiles awake in the next step.
         if (anySleep_) {
             timed("sleep_tiles", [&] {

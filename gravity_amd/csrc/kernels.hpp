@@ -20,6 +20,12 @@ struct LaunchShape {
     int planeStride;     // body slots of the batch (stride of the contact planes)
     int guardLimit;      // bound on any linked-list walk
     SleepBufs sleep;     // sleeping tiles of the batch
+    // Dispatch order of the contact kernels: workgroup i takes body block
+    // order[i] (a permutation keeping every block on the XCD xcd_block gives
+    // it), and waveWork[4 * block + wave] receives the contacts the count
+    // found in that wave, from which the order is rebuilt once per step.
+    const int* order;
+    uint32_t* waveWork;
 };
 
 // Mesh work item: {simulation, first line, lines}.
@@ -47,6 +53,11 @@ void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b);
 void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
                          const float4* mats, float kickDt, float driftDt, int insert,
                          const CollideGrid& grid);
+// Rebuilds ls.order: on every XCD, its body blocks by descending contacts of
+// the last count, so the heaviest blocks are dispatched first, one per CU.
+void launch_block_order(hipStream_t st, const LaunchShape& ls, int* order);
+// The order xcd_block alone gives, on the host.
+void identity_block_order(int blocks, int* order);
 void launch_count(hipStream_t st, const LaunchShape& ls, const float4* body,
                   const CollideGrid& grid, const CollideGrid& prev, int clearPrev,
                   const float4* mats, uint32_t* counts, uint32_t* contacts);
