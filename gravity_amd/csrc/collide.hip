@@ -24,7 +24,7 @@ namespace gs {
 
 struct GridBuf {
     uint32_t* heads;   // per cell: first body + 1, 0 when empty
-    float4* nodes;     // per body: x, y in px, next body + 1 (bits), unused
+    float4* nodes;     // per body: x, y in px, next body + 1 (bits), cx | cy << 16 (bits)
     int* cellOf;       // per body: the cell it was inserted in
 };
 
@@ -36,11 +36,15 @@ __device__ __forceinline__ int cell_y(const SimDesc& s, float y) {
     return floor_index(y * (float)s.cgh, s.cgh);
 }
 
+// A node keeps the body's cell as cx | cy << 16 (grids are at most 65535
+// cells a side), so the count finds its stencil from the node alone.
 __device__ __forceinline__ void grid_insert(const SimDesc& s, int g, float px, float py,
                                             const GridBuf& grid) {
-    int c = s.cellBase + cell_x(s, px) + cell_y(s, py) * s.cgw;
+    int cx = cell_x(s, px), cy = cell_y(s, py);
+    int c = s.cellBase + cx + cy * s.cgw;
     uint32_t old = atomicExch(&grid.heads[c], (uint32_t)(g + 1));
-    grid.nodes[g] = make_float4(px * s.vw, py * s.vh, __uint_as_float(old), 0.0f);
+    grid.nodes[g] = make_float4(px * s.vw, py * s.vh, __uint_as_float(old),
+                                __uint_as_float((uint32_t)cx | ((uint32_t)cy << 16)));
     grid.cellOf[g] = c;
 }
 
@@ -256,20 +260,20 @@ __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__
 // Count (S/collision_cache_count.comp:61-95, 130-285 without RIGID): every
 // hard pair with d^2 < D^2, strictly.
 template <bool BOUNDED>
-__device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, float4 pos,
-                                               const GridBuf& grid,
+__device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, const GridBuf& grid,
                                                const float4* __restrict__ mats,
                                                uint32_t* __restrict__ counts,
                                                uint32_t* __restrict__ contacts, int planeStride,
                                                int guardLimit) {
     const float4 own = grid.nodes[g];
     const float px = own.x, py = own.y, vw = s.vw, vh = s.vh, D2 = s.diameter2;
+    const uint32_t cell = __float_as_uint(own.w);
     const bool ownFluid = s.liquid && mats[g].z == 0.0f;
     uint32_t slot[CONTACT_SLOTS];
 #pragma unroll
     for (int k = 0; k < CONTACT_SLOTS; k++) slot[k] = 0xffffffffu;
     uint32_t n = 0;
-    walk_stencil<BOUNDED>(s, cell_x(s, pos.x), cell_y(s, pos.y), grid, guardLimit,
+    walk_stencil<BOUNDED>(s, (int)(cell & 0xffffu), (int)(cell >> 16), grid, guardLimit,
                           [&](int other, float ox, float oy) {
         float d2 = pair_d2<BOUNDED>(g, px, py, other, ox, oy, vw, vh);
         if (!(d2 < D2) || other == g) return;
@@ -293,8 +297,7 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, floa
 __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
                                               const BodyBlock* __restrict__ blocks,
                                               const int* __restrict__ order,
-                                              uint32_t* __restrict__ waveWork,
-                                              const float4* __restrict__ body, GridBuf grid,
+                                              uint32_t* __restrict__ waveWork, GridBuf grid,
                                               GridBuf prev, int clearPrev,
                                               const float4* __restrict__ mats,
                                               uint32_t* __restrict__ counts,
@@ -314,11 +317,10 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
             asleep = sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp;
         }
         if (!asleep) {
-            const float4 pos = body[g];
-            n = s.bounded ? count_contacts<true>(s, g, pos, grid, mats, counts, contacts,
-                                                 planeStride, guardLimit)
-                          : count_contacts<false>(s, g, pos, grid, mats, counts, contacts,
-                                                  planeStride, guardLimit);
+            n = s.bounded ? count_contacts<true>(s, g, grid, mats, counts, contacts, planeStride,
+                                                 guardLimit)
+                          : count_contacts<false>(s, g, grid, mats, counts, contacts, planeStride,
+                                                  guardLimit);
         }
     }
     // The wave's contacts, for the next step's dispatch order.
@@ -565,30 +567,12 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     int listed = own < (uint32_t)CONTACT_SLOTS ? (int)own : CONTACT_SLOTS;
     uint32_t last = 0u;
     const float2* __restrict__ matXY = reinterpret_cast<const float2*>(mats);
-    // A batch's ids are in registers before its loads: the first batch's
-    // are loaded with the body's own data, before its count arrives, and
-    // each batch loads the next one's with its own loads. Slots past the
-    // count hold ids of earlier passes (or nothing yet): clamped into the
-    // simulation's bodies, their loads stay in bounds, and their results are
-    // not used.
-    const uint32_t idLo = (uint32_t)s.base, idHi = (uint32_t)(s.base + s.n - 1);
-    uint32_t ahead[SOLVE_BATCH];
-#pragma unroll
-    for (int q = 0; q < SOLVE_BATCH; q++)
-        ahead[q] = min(max(contacts[(size_t)q * planeStride + g], idLo), idHi);
 #pragma unroll 1
     for (int i0 = 0; i0 < listed; i0 += SOLVE_BATCH) {
         uint32_t id[SOLVE_BATCH];
 #pragma unroll
-        for (int q = 0; q < SOLVE_BATCH; q++) id[q] = ahead[q];
-        if (i0 + SOLVE_BATCH < listed) {
-#pragma unroll
-            for (int q = 0; q < SOLVE_BATCH; q++) {
-                int k = i0 + SOLVE_BATCH + q;
-                if (k < CONTACT_SLOTS)
-                    ahead[q] = min(max(contacts[(size_t)k * planeStride + g], idLo), idHi);
-            }
-        }
+        for (int q = 0; q < SOLVE_BATCH; q++)
+            id[q] = i0 + q < listed ? contacts[(size_t)(i0 + q) * planeStride + g] : (uint32_t)g;
         float4 ob[SOLVE_BATCH];
         float2 mf[SOLVE_BATCH];
         uint32_t oc[SOLVE_BATCH];
@@ -721,12 +705,12 @@ void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, co
                        body, acc, mats, kickDt, driftDt, insert, grid_of(grid), ls.sleep);
 }
 
-void launch_count(hipStream_t st, const LaunchShape& ls, const float4* body,
-                  const CollideGrid& grid, const CollideGrid& prev, int clearPrev,
-                  const float4* mats, uint32_t* counts, uint32_t* contacts) {
+void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
+                  const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
+                  uint32_t* contacts) {
     if (ls.bodyBlocks == 0) return;
     hipLaunchKernelGGL(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
-                       ls.waveWork, body,
+                       ls.waveWork,
                        grid_of(grid), grid_of(prev), clearPrev, mats, counts, contacts,
                        ls.planeStride, ls.guardLimit, ls.sleep);
 }

@@ -292,6 +292,8 @@ void Engine::fill_desc(HostSim& h) {
     float guard = 64.0f * (std::nextafter(mx, INFINITY) - mx);
     d.cgw = std::max(1, (int)std::floor(d.vw / (d.diameter + guard)));
     d.cgh = std::max(1, (int)std::floor(d.vh / (d.diameter + guard)));
+    if (d.cgw > 65535 || d.cgh > 65535)
+        throw ArgumentError("viewport too large for the collision grid (65535 cells a side)");
     d.sortWb = sort_axis_bits(c.viewport_width);
     d.sortHb = sort_axis_bits(c.viewport_height);
     d.pw = c.grid_width;
@@ -666,8 +668,7 @@ void Engine::enqueue_pass(float kickDt, float driftDt, bool writePressure) {
                             driftDt, 1, grid_[g]);
     });
     timed("count", [&] {
-        launch_count(st_, shape_, body_[bodyPar_], grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_,
-                         contacts_);
+        launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
     });
     timed("solve", [&] {
         launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
@@ -693,8 +694,7 @@ void Engine::enqueue_step(bool sortDue) {
         for (int k = 0; k < S; k++) {
             int g = gridPar_;
             timed("count", [&] {
-                launch_count(st_, shape_, body_[bodyPar_], grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_,
-                         contacts_);
+                launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
             });
             bool last = k == S - 1;
             float nextDrift = last ? 0.0f : sub;
@@ -718,8 +718,7 @@ void Engine::enqueue_step(bool sortDue) {
         // The final zero-drift pass writes pressures (K/GravityPipeline.kt:480-486).
         int g = gridPar_;
         timed("count", [&] {
-            launch_count(st_, shape_, body_[bodyPar_], grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_,
-                         contacts_);
+            launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
         });
         timed("solve", [&] {
             launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],

@@ -25,10 +25,14 @@
 
 namespace gs {
 
-constexpr int ROW_LDS = 4096;   // complex values per row workgroup
-constexpr int ROW_NT = 512;     // lanes per row workgroup
-constexpr int COL_LDS = 8192;   // complex values per column workgroup
-constexpr int COL_NT = 1024;    // lanes per column workgroup
+// A mesh line holds at most 2048 values. Row workgroups take one 2048-value
+// row (or several shorter ones) with a radix-8 butterfly per lane and stage;
+// column workgroups take 4096 values: 2 to 4 workgroups share a CU, so one's
+// global loads overlap another's LDS stages.
+constexpr int ROW_LDS = 2048;   // complex values per row workgroup
+constexpr int ROW_NT = 256;     // lanes per row workgroup
+constexpr int COL_LDS = 4096;   // complex values per column workgroup
+constexpr int COL_NT = 512;     // lanes per column workgroup
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }

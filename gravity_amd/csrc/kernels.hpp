@@ -8,7 +8,7 @@ namespace gs {
 // Broad-phase grid buffer of one pass parity.
 struct CollideGrid {
     uint32_t* heads;   // per cell: first body + 1, or 0; heads[-1] and heads[cells] are 0
-    float4* nodes;     // per body: x, y in px, next body + 1 (bits), unused
+    float4* nodes;     // per body: x, y in px, next body + 1 (bits), cx | cy << 16 (bits)
     int* cellOf;       // per body: the cell of its insertion
 };
 
@@ -58,9 +58,9 @@ void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, co
 void launch_block_order(hipStream_t st, const LaunchShape& ls, int* order);
 // The order xcd_block alone gives, on the host.
 void identity_block_order(int blocks, int* order);
-void launch_count(hipStream_t st, const LaunchShape& ls, const float4* body,
-                  const CollideGrid& grid, const CollideGrid& prev, int clearPrev,
-                  const float4* mats, uint32_t* counts, uint32_t* contacts);
+void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
+                  const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
+                  uint32_t* contacts);
 void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, float4* bodyOut,
                   const float4* mats, const uint32_t* counts, const uint32_t* contacts,
                   const CollideGrid& grid, const float2* fallback, float* pressures,
