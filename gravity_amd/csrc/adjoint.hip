@@ -230,7 +230,7 @@ void launch_adj_direct(hipStream_t st, const LaunchShape& ls, const float4* body
                        const float4* cellBody, const float2* xbar, const float2* vbar, float dt,
                        float2* gbar, float2* spread) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_adj_direct, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_adj_direct, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        body, mats, force, cellRec, cellBody, xbar, vbar, dt, gbar, spread);
 }
 
@@ -238,21 +238,21 @@ void launch_adj_finish(hipStream_t st, const LaunchShape& ls, const float4* body
                        const float4* mats, const float2* result, const float2* gbar, float2* xbar,
                        float2* vbar, float dt, float coef) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_adj_finish, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_adj_finish, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        body, mats, result, gbar, xbar, vbar, dt, coef);
 }
 
 void launch_adj_unsort(hipStream_t st, const LaunchShape& ls, const uint32_t* dest,
                        const float2* xin, const float2* vin, float2* xout, float2* vout) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_adj_unsort, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_adj_unsort, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        dest, xin, vin, xout, vout);
 }
 
 void launch_adj_compose(hipStream_t st, const LaunchShape& ls, const uint32_t* dest,
                         uint32_t* where) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_adj_compose, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_adj_compose, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        dest, where);
 }
 

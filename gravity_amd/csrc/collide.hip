@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(WG) k_block_order(int nb, const uint32_t* __re
 
 void launch_block_order(hipStream_t st, const LaunchShape& ls, int* order) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_block_order, dim3(XCDS), dim3(WG), 0, st, ls.bodyBlocks, ls.waveWork,
+    launch(k_block_order, dim3(XCDS), dim3(WG), 0, st, ls.bodyBlocks, ls.waveWork,
                        order);
 }
 
@@ -694,14 +694,14 @@ static inline GridBuf grid_of(const CollideGrid& c) { return GridBuf{c.heads, c.
 
 void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_stop, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, a, b);
+    launch(k_stop, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, a, b);
 }
 
 void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
                          const float4* mats, float kickDt, float driftDt, int insert,
                          const CollideGrid& grid) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_drift_insert, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_drift_insert, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        body, acc, mats, kickDt, driftDt, insert, grid_of(grid), ls.sleep);
 }
 
@@ -709,7 +709,7 @@ void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid
                   const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
                   uint32_t* contacts) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
+    launch(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
                        ls.waveWork,
                        grid_of(grid), grid_of(prev), clearPrev, mats, counts, contacts,
                        ls.planeStride, ls.guardLimit, ls.sleep);
@@ -720,7 +720,7 @@ void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, f
                   const CollideGrid& grid, const float2* fallback, float* pressures,
                   int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
+    launch(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
                        bodyIn,
                        bodyOut, mats, counts, contacts, ls.planeStride, grid_of(grid),
                        ls.guardLimit, fallback, pressures, writePressure, nextDriftDt, insertNext,

@@ -12,8 +12,11 @@
 // and the CPU oracle reproduces it bit for bit.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <vector>
 
 namespace gs {
 
@@ -175,6 +178,31 @@ __device__ __forceinline__ int floor_index(float v, int size) {
     if (!(f >= 0.0f)) return 0;
     if (f > (float)(size - 1)) return size - 1;
     return (int)f;
+}
+
+// Every launch goes through launch(). While a profiling sink is set (the
+// engine's eager profiling steps), each launch carries its own event pair,
+// stamped by the kernel's dispatch itself: a kernel's time then excludes the
+// queue's gaps between launches, as a kernel trace reports it.
+struct LaunchEvents {
+    hipEvent_t start, stop;
+};
+std::vector<LaunchEvents>*& launch_sink();
+
+template <typename K, typename... A>
+inline void launch(K kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t st, A... args) {
+    std::vector<LaunchEvents>* sink = launch_sink();
+    if (sink == nullptr) {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, st, args...);
+        return;
+    }
+    LaunchEvents e{};
+    if (hipEventCreate(&e.start) != hipSuccess || hipEventCreate(&e.stop) != hipSuccess) {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, st, args...);
+        return;
+    }
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, e.start, e.stop, 0u, args...);
+    sink->push_back(e);
 }
 
 }  // namespace gs

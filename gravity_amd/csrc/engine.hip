@@ -64,6 +64,11 @@ static void validate(const gs_config& c) {
         "Walls need an FFT grid of at most 1024 cells per axis");
 }
 
+std::vector<LaunchEvents>*& launch_sink() {
+    static thread_local std::vector<LaunchEvents>* sink = nullptr;
+    return sink;
+}
+
 void Engine::check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw StateError(std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -602,13 +607,11 @@ void Engine::synchronize() { check(hipStreamSynchronize(st_), "synchronize"); }
 template <typename F>
 void Engine::timed(const char* name, F&& f) {
     if (!profiling_) { f(); return; }
-    hipEvent_t a, b;
-    check(hipEventCreate(&a), "event");
-    check(hipEventCreate(&b), "event");
-    check(hipEventRecord(a, st_), "event");
+    std::vector<LaunchEvents> launched;
+    launch_sink() = &launched;
     f();
-    check(hipEventRecord(b, st_), "event");
-    events_.push_back({name, {a, b}});
+    launch_sink() = nullptr;
+    for (const LaunchEvents& e : launched) events_.push_back({name, {e.start, e.stop}});
 }
 
 // ---------------------------------------------------------------------------

@@ -337,7 +337,7 @@ void launch_fft_rows_forward_values(hipStream_t st, const SimDesc* sims, const L
                                     const float2* value, float2* spec, const float2* twiddles,
                                     int twiddleLength) {
     if (items == 0) return;
-    hipLaunchKernelGGL(k_fft_rows_forward_values, dim3(items), dim3(ROW_NT), 0, st, sims, work,
+    launch(k_fft_rows_forward_values, dim3(items), dim3(ROW_NT), 0, st, sims, work,
                        cellPrefix, cellBody, value, spec, twiddles, twiddleLength);
 }
 
@@ -345,7 +345,7 @@ void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
                              const float2* twiddles, int twiddleLength) {
     if (items == 0) return;
-    hipLaunchKernelGGL(k_fft_rows_forward, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
+    launch(k_fft_rows_forward, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
                        cellBody, spec, twiddles, twiddleLength);
 }
 
@@ -353,14 +353,14 @@ void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* wor
                         float2* spec, const float2* green, const float2* twiddles,
                         int twiddleLength) {
     if (items == 0) return;
-    hipLaunchKernelGGL(k_fft_columns, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green,
+    launch(k_fft_columns, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green,
                        twiddles, twiddleLength);
 }
 
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              float2* spec, const float2* twiddles, int twiddleLength) {
     if (items == 0) return;
-    hipLaunchKernelGGL(k_fft_rows_inverse, dim3(items), dim3(ROW_NT), 0, st, sims, work, spec,
+    launch(k_fft_rows_inverse, dim3(items), dim3(ROW_NT), 0, st, sims, work, spec,
                        twiddles, twiddleLength);
 }
 

@@ -352,21 +352,21 @@ __global__ void __launch_bounds__(WG) k_liq_viscosity(const SimDesc* __restrict_
 void launch_liquid_bin(hipStream_t st, const LaunchShape& ls, const float4* body,
                        uint32_t* liqCount, uint32_t* cellOf) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_liq_bin, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body,
+    launch(k_liq_bin, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body,
                        liqCount, cellOf);
 }
 
 void launch_liquid_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* cellOf,
                            uint32_t* liqCount, const uint32_t* prefix, uint32_t* unordered) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_liq_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_liq_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        cellOf, liqCount, prefix, unordered);
 }
 
 void launch_liquid_canon(hipStream_t st, int slots, const uint32_t* cellOf,
                          const uint32_t* prefix, const uint32_t* unordered, uint32_t* items) {
     if (slots <= 0) return;
-    hipLaunchKernelGGL(k_liq_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, cellOf,
+    launch(k_liq_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, cellOf,
                        prefix, unordered, items);
 }
 
@@ -375,7 +375,7 @@ void launch_liquid_lambda(hipStream_t st, const LaunchShape& ls, const float4* b
                           const uint32_t* items, const float2* fallback, float* lambdas,
                           float* pressures, int writePressure) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_liq_lambda, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_liq_lambda, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        body, mats, cellOf, prefix, items, fallback, lambdas, pressures,
                        writePressure);
 }
@@ -384,7 +384,7 @@ void launch_liquid_apply(hipStream_t st, const LaunchShape& ls, const float4* bo
                          const float4* mats, const uint32_t* cellOf, const uint32_t* prefix,
                          const uint32_t* items, const float2* fallback, const float* lambdas) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_liq_apply, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_liq_apply, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        body, out, mats, cellOf, prefix, items, fallback, lambdas);
 }
 
@@ -392,7 +392,7 @@ void launch_liquid_viscosity(hipStream_t st, const LaunchShape& ls, const float4
                              float4* out, const float4* mats, const uint32_t* cellOf,
                              const uint32_t* prefix, const uint32_t* items) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_liq_viscosity, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_liq_viscosity, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        body, out, mats, cellOf, prefix, items);
 }
 

@@ -275,14 +275,14 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
 void launch_bin(hipStream_t st, const LaunchShape& ls, const float4* body, uint32_t* cellCount,
                 uint32_t* homeOf) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_bin, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body,
+    launch(k_bin, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body,
                        cellCount, homeOf);
 }
 
 void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* homeOf,
                          uint32_t* cellCount, const uint32_t* cellPrefix, uint32_t* slotBody) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_cell_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_cell_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        homeOf, cellCount, cellPrefix, slotBody);
 }
 
@@ -290,14 +290,14 @@ void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
                        const float4* mats, float4* cellBody, uint4* cellRec) {
     if (slots <= 0) return;
-    hipLaunchKernelGGL(k_cell_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, homeOf,
+    launch(k_cell_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, homeOf,
                        cellPrefix, slotBody, body, mats, cellBody, cellRec);
 }
 
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
                  const float2* force, const uint4* cellRec, const float4* cellBody, int initial) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_kick, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
+    launch(k_kick, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
                        force, cellRec, cellBody, initial, ls.sleep);
 }
 

@@ -51,15 +51,15 @@ __global__ void k_sleep_wake(SleepBufs sleep, int sim, int steps) {
 }
 
 void launch_sleep_wake(hipStream_t st, const SleepBufs& sleep, int sim, int steps) {
-    hipLaunchKernelGGL(k_sleep_wake, dim3(1), dim3(64), 0, st, sleep, sim, steps);
+    launch(k_sleep_wake, dim3(1), dim3(64), 0, st, sleep, sim, steps);
 }
 
 void launch_sleep_advance(hipStream_t st, const SimDesc* sims, int simCount,
                           const BodyBlock* tileBlocks, int tileBlockCount, const SleepBufs& sleep) {
     if (tileBlockCount > 0)
-        hipLaunchKernelGGL(k_sleep_tiles, dim3(tileBlockCount), dim3(WG), 0, st, sims, tileBlocks,
+        launch(k_sleep_tiles, dim3(tileBlockCount), dim3(WG), 0, st, sims, tileBlocks,
                            sleep);
-    hipLaunchKernelGGL(k_sleep_end, dim3((simCount + 63) / 64), dim3(64), 0, st, sims, simCount,
+    launch(k_sleep_end, dim3((simCount + 63) / 64), dim3(64), 0, st, sims, simCount,
                        sleep);
 }
 

@@ -114,9 +114,9 @@ int scan_totals_size(int len) { return (len + 1 + SCAN_BLOCK - 1) / SCAN_BLOCK; 
 void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals,
                  uint4* rec) {
     int blocks = scan_totals_size(len);
-    hipLaunchKernelGGL(k_scan_local, dim3(blocks), dim3(WG), 0, st, in, out, len, totals);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(WG), 0, st, totals, blocks);
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in,
+    launch(k_scan_local, dim3(blocks), dim3(WG), 0, st, in, out, len, totals);
+    launch(k_scan_totals, dim3(1), dim3(WG), 0, st, totals, blocks);
+    launch(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in,
                        reinterpret_cast<uint2*>(rec));
 }
 
@@ -203,14 +203,14 @@ __global__ void __launch_bounds__(WG) k_sort_canon(
 void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float4* body,
                        uint32_t* tileCount, uint32_t* keyOf) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_sort_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_sort_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        body, tileCount, keyOf);
 }
 
 void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
                          uint32_t* tileCount, const uint32_t* tilePrefix, uint32_t* slotSrc) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_sort_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_sort_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        keyOf, tileCount, tilePrefix, slotSrc);
 }
 
@@ -219,7 +219,7 @@ void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* ke
                        const float4* bodyIn, const float2* accIn, const float4* matIn,
                        float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest) {
     if (ls.bodyBlocks == 0) return;
-    hipLaunchKernelGGL(k_sort_canon, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
+    launch(k_sort_canon, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        keyOf, tilePrefix, slotSrc, bodyIn, accIn, matIn, bodyOut, accOut, matOut,
                        dest);
 }
