@@ -180,6 +180,26 @@ __device__ __forceinline__ int floor_index(float v, int size) {
     return (int)f;
 }
 
+// A run of equal keys in consecutive lanes of a wave: the first lane of the
+// run, its length and this lane's place in it. Every lane of the wave must
+// call it; a lane without a key passes a value no key takes (0xffffffff).
+// Bodies arrive spatially sorted, so the lanes of a wave share few cells:
+// one atomic per run replaces one per lane on the same address.
+struct LaneRun {
+    int start, length, rank;
+};
+__device__ __forceinline__ LaneRun lane_run(uint32_t key) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t prev = (uint32_t)__shfl_up((int)key, 1u, 64);
+    const bool first = lane == 0 || key != prev;
+    const unsigned long long starts = __ballot(first);
+    const unsigned long long upTo = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __clzll((long long)(starts & upTo));
+    const unsigned long long above = starts & ~upTo;
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    return LaneRun{start, next - start, lane - start};
+}
+
 // Every launch goes through launch(). While a profiling sink is set (the
 // engine's eager profiling steps), each launch carries its own event pair,
 // stamped by the kernel's dispatch itself: a kernel's time then excludes the

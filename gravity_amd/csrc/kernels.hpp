@@ -33,7 +33,7 @@ struct LineWork { int sim; int first; int count; int pad; };
 
 // ---- scan: out[i] = sum of in[0..i) for i in [0, len]; totals is scratch ----
 // Exclusive scan of len values (out[len] = total). rec, if given, receives
-// {prefix, count} in the first half of each 16-byte element record.
+// {prefix, count, 0, 0} as each element's 16-byte record.
 void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals,
                  uint4* rec = nullptr);
 int scan_totals_size(int len);

@@ -20,7 +20,8 @@ __device__ __forceinline__ int home_cell(const SimDesc& s, float px, float py) {
     return s.physBase + wrap_axis(hx, s.pw) + wrap_axis(hy, s.ph) * s.pw;
 }
 
-// S/build_grid.comp:71-74: the home cell of every body.
+// S/build_grid.comp:71-74: the home cell of every body; one atomic per run
+// of equal cells in a wave.
 __global__ void __launch_bounds__(WG) k_bin(const SimDesc* __restrict__ sims,
                                             const BodyBlock* __restrict__ blocks,
                                             const float4* __restrict__ body, uint32_t* cellCount,
@@ -28,11 +29,15 @@ __global__ void __launch_bounds__(WG) k_bin(const SimDesc* __restrict__ sims,
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
-    if (g - s.base >= s.n || !s.mutual) return;
-    float4 p = body[g];
-    int c = home_cell(s, p.x, p.y);
-    homeOf[g] = (uint32_t)c;
-    atomicAdd(&cellCount[c], 1u);
+    const bool active = g - s.base < s.n && s.mutual;
+    uint32_t c = 0xffffffffu;
+    if (active) {
+        float4 p = body[g];
+        c = (uint32_t)home_cell(s, p.x, p.y);
+        homeOf[g] = c;
+    }
+    const LaneRun run = lane_run(c);
+    if (active && run.rank == 0) atomicAdd(&cellCount[c], (uint32_t)run.length);
 }
 
 __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__ sims,
@@ -44,15 +49,21 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
-    if (g - s.base >= s.n || !s.mutual) return;
-    uint32_t c = homeOf[g];
-    uint32_t rank = atomicSub(&cellCount[c], 1u) - 1u;
+    const bool active = g - s.base < s.n && s.mutual;
+    const uint32_t c = active ? homeOf[g] : 0xffffffffu;
+    const LaneRun run = lane_run(c);
+    uint32_t top = 0u;
+    if (active && run.rank == 0) top = atomicSub(&cellCount[c], (uint32_t)run.length);
+    top = (uint32_t)__shfl((int)top, run.start, 64);
+    if (!active) return;
+    uint32_t rank = top - 1u - (uint32_t)run.rank;
     slotBody[cellPrefix[c] + rank] = (uint32_t)g;
 }
 
-// Orders each cell by descending body index; the first body of a cell also
-// sums the cell mass in 1/256 units (S/build_grid.comp:75), exactly, into
-// the cell record (whose first half the cell scan wrote).
+// Orders each cell by descending body index and sums the cell mass in 1/256
+// units (S/build_grid.comp:75), exactly: the slots of a cell are contiguous,
+// so a wave adds its runs of one cell in registers and one atomic per run
+// accumulates them into the cell record (whose mass the cell scan zeroed).
 __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __restrict__ homeOf,
                                                    const uint32_t* __restrict__ cellPrefix,
                                                    const uint32_t* __restrict__ slotBody,
@@ -61,20 +72,27 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
                                                    float4* __restrict__ cellBody,
                                                    uint4* __restrict__ cellRec) {
     int d = (int)(blockIdx.x * WG + threadIdx.x);
-    if (d >= slots) return;
-    uint32_t g = slotBody[d];
-    uint32_t c = homeOf[g];
-    int s0 = (int)cellPrefix[c], s1 = (int)cellPrefix[c + 1];
-    int rank = 0;
-    for (int e = s0; e < s1; e++) rank += slotBody[e] > g ? 1 : 0;
-    float4 p = body[g];
-    float m = mats[g].x;
-    cellBody[s0 + rank] = make_float4(p.x, p.y, m, __uint_as_float(g));
-    if (rank == 0) {
-        uint32_t sum = 0u;
-        for (int e = s0; e < s1; e++) sum += (uint32_t)rintf(mats[slotBody[e]].x * 256.0f);
-        reinterpret_cast<uint32_t*>(cellRec)[4 * (size_t)c + 2] = sum;
+    const bool active = d < slots;
+    uint32_t c = 0xffffffffu, units = 0u;
+    if (active) {
+        uint32_t g = slotBody[d];
+        c = homeOf[g];
+        int s0 = (int)cellPrefix[c], s1 = (int)cellPrefix[c + 1];
+        int rank = 0;
+        for (int e = s0; e < s1; e++) rank += slotBody[e] > g ? 1 : 0;
+        float4 p = body[g];
+        float m = mats[g].x;
+        cellBody[s0 + rank] = make_float4(p.x, p.y, m, __uint_as_float(g));
+        units = (uint32_t)rintf(m * 256.0f);
     }
+    const LaneRun run = lane_run(c);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = (uint32_t)__shfl_up((int)units, (unsigned)o, 64);
+        if (run.rank >= o) units += t;
+    }
+    if (active && run.rank == run.length - 1)
+        atomicAdd(&reinterpret_cast<uint32_t*>(cellRec)[4 * (size_t)c + 2], units);
 }
 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }

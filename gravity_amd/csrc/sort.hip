@@ -91,12 +91,13 @@ __global__ void __launch_bounds__(WG) k_scan_totals(uint32_t* totals, int blocks
     }
 }
 
-// Adds the block offsets. With rec, also writes {prefix, count} into the
-// first half of each element's 16-byte record (the cell tables of the kick).
+// Adds the block offsets. With rec, also writes each element's 16-byte
+// record {prefix, count, 0, 0} (the cell tables of the kick; the cell canon
+// accumulates the mass into the third word).
 __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int len,
                                                  const uint32_t* __restrict__ totals,
                                                  const uint32_t* __restrict__ in,
-                                                 uint2* __restrict__ rec) {
+                                                 uint4* __restrict__ rec) {
     int base = (int)blockIdx.x * SCAN_BLOCK;
     uint32_t add = totals[blockIdx.x];
     for (int i = (int)threadIdx.x; i < SCAN_BLOCK; i += WG) {
@@ -104,7 +105,7 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int
         if (idx <= len) {
             uint32_t v = out[idx] + add;
             out[idx] = v;
-            if (rec != nullptr && idx < len) rec[2 * (size_t)idx] = make_uint2(v, in[idx]);
+            if (rec != nullptr && idx < len) rec[idx] = make_uint4(v, in[idx], 0u, 0u);
         }
     }
 }
@@ -116,8 +117,7 @@ void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uin
     int blocks = scan_totals_size(len);
     launch(k_scan_local, dim3(blocks), dim3(WG), 0, st, in, out, len, totals);
     launch(k_scan_totals, dim3(1), dim3(WG), 0, st, totals, blocks);
-    launch(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in,
-                       reinterpret_cast<uint2*>(rec));
+    launch(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in, rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -143,7 +143,7 @@ __device__ __forceinline__ uint32_t sort_key(float x, float y, int wb, int hb) {
     return morton | (((tx | ty) >> common) << (2 * common));
 }
 
-// S/spatial_sort_count.comp:8-12
+// S/spatial_sort_count.comp:8-12, one atomic per run of equal keys.
 __global__ void __launch_bounds__(WG) k_sort_count(const SimDesc* __restrict__ sims,
                                                    const BodyBlock* __restrict__ blocks,
                                                    const float4* __restrict__ body,
@@ -151,14 +151,19 @@ __global__ void __launch_bounds__(WG) k_sort_count(const SimDesc* __restrict__ s
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
-    if (g - s.base >= s.n) return;
-    float4 p = body[g];
-    uint32_t t = (uint32_t)s.tileBase + sort_key(p.x, p.y, s.sortWb, s.sortHb);
-    keyOf[g] = t;
-    atomicAdd(&tileCount[t], 1u);
+    const bool active = g - s.base < s.n;
+    uint32_t t = 0xffffffffu;
+    if (active) {
+        float4 p = body[g];
+        t = (uint32_t)s.tileBase + sort_key(p.x, p.y, s.sortWb, s.sortHb);
+        keyOf[g] = t;
+    }
+    const LaneRun run = lane_run(t);
+    if (active && run.rank == 0) atomicAdd(&tileCount[t], (uint32_t)run.length);
 }
 
 // S/spatial_sort_scatter.comp:14-28: counting down returns every count to 0.
+// A run takes its ranks with one atomic and hands them out in lane order.
 __global__ void __launch_bounds__(WG) k_sort_scatter(const SimDesc* __restrict__ sims,
                                                      const BodyBlock* __restrict__ blocks,
                                                      const uint32_t* __restrict__ keyOf,
@@ -168,9 +173,14 @@ __global__ void __launch_bounds__(WG) k_sort_scatter(const SimDesc* __restrict__
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
-    if (g - s.base >= s.n) return;
-    uint32_t t = keyOf[g];
-    uint32_t rank = atomicSub(&tileCount[t], 1u) - 1u;
+    const bool active = g - s.base < s.n;
+    const uint32_t t = active ? keyOf[g] : 0xffffffffu;
+    const LaneRun run = lane_run(t);
+    uint32_t top = 0u;
+    if (active && run.rank == 0) top = atomicSub(&tileCount[t], (uint32_t)run.length);
+    top = (uint32_t)__shfl((int)top, run.start, 64);
+    if (!active) return;
+    uint32_t rank = top - 1u - (uint32_t)run.rank;
     int slot = s.base + (int)(tilePrefix[t] - (uint32_t)s.rank0 + rank);
     slotSrc[slot] = (uint32_t)g;
 }
