@@ -67,39 +67,19 @@ __global__ void __launch_bounds__(WG) k_scan_local(const uint32_t* __restrict__ 
     if (threadIdx.x == 0) totals[blockIdx.x] = total;
 }
 
-// One workgroup: exclusive prefix of the block totals, in place.
-__global__ void __launch_bounds__(WG) k_scan_totals(uint32_t* totals, int blocks) {
-    uint32_t carry = 0u;
-    for (int chunk = 0; chunk < blocks; chunk += WG * 4) {
-        int base = chunk + (int)threadIdx.x * 4;
-        uint32_t v[4];
-        uint32_t sum = 0u;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            v[i] = base + i < blocks ? totals[base + i] : 0u;
-            sum += v[i];
-        }
-        uint32_t total;
-        uint32_t before = block_exclusive(sum, &total) + carry;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            if (base + i < blocks) totals[base + i] = before;
-            before += v[i];
-        }
-        carry += total;
-        __syncthreads();
-    }
-}
-
-// Adds the block offsets. With rec, also writes each element's 16-byte
-// record {prefix, count, 0, 0} (the cell tables of the kick; the cell canon
+// Adds the block offsets: each workgroup sums the totals of the blocks
+// before it. With rec, also writes each element's 16-byte record
+// {prefix, count, 0, 0} (the cell tables of the kick; the cell canon
 // accumulates the mass into the third word).
 __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int len,
                                                  const uint32_t* __restrict__ totals,
                                                  const uint32_t* __restrict__ in,
                                                  uint4* __restrict__ rec) {
+    uint32_t part = 0u;
+    for (int i = (int)threadIdx.x; i < (int)blockIdx.x; i += WG) part += totals[i];
+    uint32_t add;
+    block_exclusive(part, &add);
     int base = (int)blockIdx.x * SCAN_BLOCK;
-    uint32_t add = totals[blockIdx.x];
     for (int i = (int)threadIdx.x; i < SCAN_BLOCK; i += WG) {
         int idx = base + i;
         if (idx <= len) {
@@ -116,7 +96,6 @@ void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uin
                  uint4* rec) {
     int blocks = scan_totals_size(len);
     launch(k_scan_local, dim3(blocks), dim3(WG), 0, st, in, out, len, totals);
-    launch(k_scan_totals, dim3(1), dim3(WG), 0, st, totals, blocks);
     launch(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in, rec);
 }
 
