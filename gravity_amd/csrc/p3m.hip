@@ -154,8 +154,22 @@ __device__ __forceinline__ float2 mesh_force(const SimDesc& s, const float2* __r
 // exp2), and a monopole's Gaussian is the product of its column and row
 // factors; the mesh gather, the field and the half kick round as written.
 
+// One source taken body by body (x, y its normalised position, z its mass).
+__device__ __forceinline__ void body_term(const SimDesc& s, float4 o, float x, float y, float k2,
+                                          float& ax, float& ay) {
+#pragma clang fp contract(fast)
+    float ox = min_image(s, o.x - x) * s.aspect;
+    float oy = min_image(s, o.y - y);
+    float r2 = ox * ox + oy * oy;
+    near_term(s, ox, oy, r2, __expf(-r2 * k2), o.z, ax, ay);
+}
+
 // Near field of rx = ry = R cells: the column offsets and Gaussian factors
-// of a body are computed once, and each row's records are loaded together.
+// of a body are computed once. Per stencil row the records load together,
+// then the first body of every cell taken body by body loads together, then
+// the second ones, so a row costs three memory round trips however many of
+// its cells are taken body by body; the monopoles are summed while the
+// bodies load.
 template <int R>
 __device__ __forceinline__ void near_field_fixed(const SimDesc& s,
                                                  const uint4* __restrict__ cellRec,
@@ -164,17 +178,16 @@ __device__ __forceinline__ void near_field_fixed(const SimDesc& s,
 #pragma clang fp contract(fast)
     const float k2 = s.invTwoSigma2;
     const float cellMassUnit = 1.0f / 256.0f;
+    const int exactLimit = s.exactLimit, homeLimit = s.homeLimit;
     int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
     constexpr int D = 2 * R + 1;
     float cpx[D], cex[D];
-    int cwx[D];
     uint32_t colOk = 0u;
 #pragma unroll
     for (int k = 0; k < D; k++) {
         int cx = hx + k - R;
         if (!s.bounded || (cx >= 0 && cx < s.pw)) colOk |= 1u << k;
         int wx = wrap_axis(cx, s.pw);
-        cwx[k] = wx;
         float px = min_image(s, ((float)wx + 0.5f) * s.invPw - x) * s.aspect;
         cpx[k] = px;
         cex[k] = __expf(-(px * px) * k2);
@@ -191,19 +204,55 @@ __device__ __forceinline__ void near_field_fixed(const SimDesc& s,
         uint4 rec[D];
 #pragma unroll
         for (int k = 0; k < D; k++)
-            rec[k] = (colOk >> k) & 1u ? row[cwx[k]] : make_uint4(0u, 0u, 0u, 0u);
+            rec[k] = (colOk >> k) & 1u ? row[wrap_axis(hx + k - R, s.pw)] : make_uint4(0u, 0u, 0u, 0u);
+        const bool homeRow = j == R;
+        // Bodies a cell contributes one by one: the home cell's first
+        // homeLimit list entries, or all of a cell of at most exactLimit.
+        int take[D];
 #pragma unroll
         for (int k = 0; k < D; k++) {
             int count = (int)rec[k].y;
-            bool home = j == R && k == R;
-            if (!home && count > s.exactLimit) {
+            bool home = homeRow && k == R;
+            take[k] = home ? min(count, homeLimit) : (count <= exactLimit ? count : 0);
+        }
+        float4 b0[D];
+#pragma unroll
+        for (int k = 0; k < D; k++)
+            if (take[k] > 0) b0[k] = cellBody[rec[k].x];
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            int count = (int)rec[k].y;
+            bool home = homeRow && k == R;
+            if (!home && count > exactLimit) {
                 float px = cpx[k];
                 near_term(s, px, py, px * px + py2, cex[k] * ey, (float)rec[k].z * cellMassUnit,
                           ax, ay);
-            } else if (count > 0) {
-                int limit = home ? s.homeLimit : s.exactLimit;
-                exact_cell(s, cellBody, (int)rec[k].x, count < limit ? count : limit, g, x, y, k2,
-                           ax, ay);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            // only the home cell can hold the body itself
+            bool self = homeRow && k == R && __float_as_uint(b0[k].w) == (uint32_t)g;
+            if (take[k] > 0 && !self) body_term(s, b0[k], x, y, k2, ax, ay);
+        }
+        // the second bodies load after the first ones are consumed (registers)
+        asm volatile("" ::: "memory");
+        float4 b1[D];
+#pragma unroll
+        for (int k = 0; k < D; k++)
+            if (take[k] > 1) b1[k] = cellBody[rec[k].x + 1u];
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            bool self = homeRow && k == R && __float_as_uint(b1[k].w) == (uint32_t)g;
+            if (take[k] > 1 && !self) body_term(s, b1[k], x, y, k2, ax, ay);
+        }
+        // Further bodies: the home cell's, and any cell's under a larger exactLimit.
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            for (int t = 2; t < take[k]; t++) {
+                float4 o = cellBody[rec[k].x + (uint32_t)t];
+                bool self = homeRow && k == R && __float_as_uint(o.w) == (uint32_t)g;
+                if (!self) body_term(s, o, x, y, k2, ax, ay);
             }
         }
     }
