@@ -16,7 +16,8 @@
 // Transforms are Stockham autosort in LDS: one lane owns one radix-8
 // butterfly of a stage (radix 4 or 2 for the last stage, a few butterflies a
 // lane), so a lane holds 8 complex values and many waves share a CU.
-// Twiddles come from an FP64-rounded table. Under walls the mesh doubles;
+// Twiddles come from an FP64-rounded table of 2048 entries, staged in LDS by
+// every workgroup. Under walls the mesh doubles;
 // rows that hold only zero padding are neither loaded nor transformed in
 // the forward row pass, and only rows the force gather reads are produced by
 // the inverse passes.
@@ -131,7 +132,7 @@ __device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int l
 // Transforms nl lines (nl * L <= E). DIR = -1 forward, +1 inverse
 // (unscaled, like S/fft_shared.comp).
 template <int DIR, int NT, int E>
-__device__ void lds_fft(float2* buf, int L, int nl, int ls, int es, const float2* __restrict__ tw,
+__device__ __forceinline__ void lds_fft(float2* buf, int L, int nl, int ls, int es, const float2* __restrict__ tw,
                         int twLen) {
     const bool jFast = es == 1;
     int Ns = 1;
@@ -150,6 +151,15 @@ __device__ void lds_fft(float2* buf, int L, int nl, int ls, int es, const float2
     }
 }
 
+// The twiddle table (at most TW_LDS entries) staged in LDS with the first
+// loads of a kernel: the stages then read it without a memory round trip.
+constexpr int TW_LDS = 2048;
+template <int NT>
+__device__ __forceinline__ void stage_twiddles(float2* twl, const float2* __restrict__ tw,
+                                               int twLen) {
+    for (int i = (int)threadIdx.x; i < twLen; i += NT) twl[i] = tw[i];
+}
+
 // Mesh row of forward/inverse work row a: all rows on the torus; under walls
 // rows 0..ph and the last one (cells at -1 wrap there).
 __device__ __forceinline__ int mesh_row(const SimDesc& s, int a) {
@@ -166,9 +176,11 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
                                                              const float2* __restrict__ tw, int twLen) {
     __shared__ float2 data[ROW_LDS];
     __shared__ uint32_t mass[ROW_LDS];
+    __shared__ float2 twl[TW_LDS];
     LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
+    stage_twiddles<ROW_NT>(twl, tw, twLen);
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) mass[i] = 0u;
     __syncthreads();
     const float scale = s.massScale;
@@ -188,23 +200,33 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
             int r = s.bounded || s.ph < 3 ? rr : wrap_axis(rr, s.ph);
             int c0 = s.physBase + r * s.pw;
             int t0 = (int)cellPrefix[c0], t1 = (int)cellPrefix[c0 + s.pw];
-            for (int t = t0 + (int)threadIdx.x; t < t1; t += ROW_NT) {
-                float4 bd = cellBody[t];
-                // S/build_grid.comp:53-69, round() half to even.
-                float gx = bd.x * (float)s.pw - 0.5f, gy = bd.y * (float)s.ph - 0.5f;
-                float flx = floorf(gx), fly = floorf(gy);
-                int bx = (int)flx, by = (int)fly;
-                float fx = gx - flx, fy = gy - fly;
-                float wgt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy,
-                                fx * fy};
+            // Four bodies a lane per round, their loads issued together.
+            constexpr int UNROLL = 4;
+            for (int t = t0 + (int)threadIdx.x; t < t1; t += UNROLL * ROW_NT) {
+                float4 bds[UNROLL];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    int my = wrap_axis(by + (k >> 1), s.mh);
-                    if (my != y) continue;
-                    int mx = wrap_axis(bx + (k & 1), s.mw);
-                    float wk = wgt[k] * bd.z;
-                    uint32_t q = (uint32_t)rintf(wk * scale);
-                    atomicAdd(&mass[l * L + mx], q);
+                for (int u = 0; u < UNROLL; u++)
+                    if (t + u * ROW_NT < t1) bds[u] = cellBody[t + u * ROW_NT];
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++) {
+                    if (t + u * ROW_NT >= t1) continue;
+                    float4 bd = bds[u];
+                    // S/build_grid.comp:53-69, round() half to even.
+                    float gx = bd.x * (float)s.pw - 0.5f, gy = bd.y * (float)s.ph - 0.5f;
+                    float flx = floorf(gx), fly = floorf(gy);
+                    int bx = (int)flx, by = (int)fly;
+                    float fx = gx - flx, fy = gy - fly;
+                    float wgt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy),
+                                    (1.0f - fx) * fy, fx * fy};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        int my = wrap_axis(by + (k >> 1), s.mh);
+                        if (my != y) continue;
+                        int mx = wrap_axis(bx + (k & 1), s.mw);
+                        float wk = wgt[k] * bd.z;
+                        uint32_t q = (uint32_t)rintf(wk * scale);
+                        atomicAdd(&mass[l * L + mx], q);
+                    }
                 }
             }
         }
@@ -213,7 +235,7 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT)
         data[i] = make_float2((float)mass[i] * s.invMassScale, 0.0f);
     __syncthreads();
-    lds_fft<-1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, tw, twLen);
+    lds_fft<-1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, twl, twLen);
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
         int l = i / L, x = i - l * L;
         int y = mesh_row(s, w.first + l);
@@ -227,25 +249,42 @@ __global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restric
                                                         const float2* __restrict__ green,
                                                         const float2* __restrict__ tw, int twLen) {
     __shared__ float2 data[COL_LDS];
+    __shared__ float2 twl[TW_LDS];
     LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int H = s.mh, CB = w.count, c0 = w.first;
+    stage_twiddles<COL_NT>(twl, tw, twLen);
     const float2* src = spec + s.meshBase;
     for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
         int y = i / CB, c = i - y * CB;
         bool live = !s.bounded || y <= s.ph || y == H - 1;
         data[i] = live ? src[y * s.mw + c0 + c] : make_float2(0.0f, 0.0f);
     }
-    __syncthreads();
-    lds_fft<-1, COL_NT, COL_LDS>(data, H, CB, 1, CB, tw, twLen);
+    // The spectrum values of this workgroup's nodes load now and arrive
+    // during the forward transform (COL_LDS / COL_NT of them per lane).
+    constexpr int PER = COL_LDS / COL_NT;
     const float2* G = green + s.meshBase;
-    for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
-        int y = i / CB, c = i - y * CB;
-        float2 z = cmul(data[i], G[y * s.mw + c0 + c]);
-        data[i] = make_float2(z.x * s.invTotalMass, z.y * s.invTotalMass);
+    float2 gv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int i = (int)threadIdx.x + q * COL_NT;
+        if (i < CB * H) {
+            int y = i / CB, c = i - y * CB;
+            gv[q] = G[y * s.mw + c0 + c];
+        }
     }
     __syncthreads();
-    lds_fft<1, COL_NT, COL_LDS>(data, H, CB, 1, CB, tw, twLen);
+    lds_fft<-1, COL_NT, COL_LDS>(data, H, CB, 1, CB, twl, twLen);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int i = (int)threadIdx.x + q * COL_NT;
+        if (i < CB * H) {
+            float2 z = cmul(data[i], gv[q]);
+            data[i] = make_float2(z.x * s.invTotalMass, z.y * s.invTotalMass);
+        }
+    }
+    __syncthreads();
+    lds_fft<1, COL_NT, COL_LDS>(data, H, CB, 1, CB, twl, twLen);
     float2* dst = spec + s.meshBase;
     for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
         int y = i / CB, c = i - y * CB;
@@ -259,15 +298,17 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_inverse(const SimDesc* __re
                                                              float2* __restrict__ spec,
                                                              const float2* __restrict__ tw, int twLen) {
     __shared__ float2 data[ROW_LDS];
+    __shared__ float2 twl[TW_LDS];
     LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
+    stage_twiddles<ROW_NT>(twl, tw, twLen);
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
         int l = i / L, x = i - l * L;
         data[i] = spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x];
     }
     __syncthreads();
-    lds_fft<1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, tw, twLen);
+    lds_fft<1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, twl, twLen);
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
         int l = i / L, x = i - l * L;
         spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = data[i];
@@ -283,9 +324,11 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward_values(
     const float2* __restrict__ value, float2* __restrict__ spec, const float2* __restrict__ tw,
     int twLen) {
     __shared__ float2 data[ROW_LDS];
+    __shared__ float2 twl[TW_LDS];
     LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
+    stage_twiddles<ROW_NT>(twl, tw, twLen);
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) data[i] = make_float2(0.0f, 0.0f);
     __syncthreads();
     for (int l = 0; l < w.count; l++) {
@@ -324,7 +367,7 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward_values(
         }
     }
     __syncthreads();
-    lds_fft<-1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, tw, twLen);
+    lds_fft<-1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, twl, twLen);
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
         int l = i / L, x = i - l * L;
         int y = mesh_row(s, w.first + l);
