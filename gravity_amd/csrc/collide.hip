@@ -519,6 +519,31 @@ __device__ __forceinline__ void accumulate_contact(const SolveK& s, const float*
     }
 }
 
+// The OVERFLOW_CHUNK smallest contact indices above floorId, ascending, by
+// one walk of the stencil (unused entries 0xffffffff).
+constexpr int OVERFLOW_CHUNK = 6;
+constexpr int OVERFLOW_BATCH = 2;   // their loads issued together
+template <bool BOUNDED>
+__device__ __forceinline__ void next_contacts(const SimDesc& s, int cx, int cy, const GridBuf& grid,
+                                              int guardLimit, int g, float px, float py,
+                                              uint32_t floorId, bool ownFluid,
+                                              const float4* __restrict__ mats,
+                                              uint32_t (&next)[OVERFLOW_CHUNK]) {
+#pragma unroll
+    for (int q = 0; q < OVERFLOW_CHUNK; q++) next[q] = 0xffffffffu;
+    const float vw = s.vw, vh = s.vh, D2 = s.diameter2;
+    walk_stencil_serial(s, cx, cy, grid, guardLimit, [&](int other, float ox, float oy) {
+        uint32_t u = (uint32_t)other;
+        if (u <= floorId || other == g || u >= next[OVERFLOW_CHUNK - 1]) return;
+        if (ownFluid && mats[other].z == 0.0f) return;
+        float d2 = pair_d2<BOUNDED>(g, px, py, other, ox, oy, vw, vh);
+        if (!(d2 < D2)) return;
+#pragma unroll
+        for (int q = OVERFLOW_CHUNK - 1; q > 0; q--) next[q] = med3u(next[q - 1], u, next[q]);
+        next[0] = min(u, next[0]);
+    });
+}
+
 // Contacts whose loads are issued together. The batch loop is not
 // unrolled: four contacts' registers keep the kernel at 7 waves per SIMD,
 // which holds every wave of a 400k-body launch resident at once.
@@ -595,25 +620,51 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
         }
     }
     if (own > (uint32_t)CONTACT_SLOTS) {
-        // More contacts than slots: take the rest in ascending index, one
-        // ordered walk of the grid per contact.
-        bool ownFluid = k.liquid && ownMat.z == 0.0f;
-        for (uint32_t rest = own - (uint32_t)CONTACT_SLOTS; rest > 0u; rest--) {
-            uint32_t best = 0xffffffffu;
-            walk_stencil_serial(s, cell_x(s, me.x), cell_y(s, me.y), grid, guardLimit,
-                                [&](int other, float ox, float oy) {
-                uint32_t u = (uint32_t)other;
-                if (other == g || u <= last || u >= best) return;
-                if (ownFluid && mats[other].z == 0.0f) return;
-                float d2 = pair_distance_squared(g, ppx, ppy, other, ox, oy, s);
-                if (d2 < s.diameter2) best = u;
-            });
-            if (best == 0xffffffffu) break;
-            last = best;
-            bool otherSleeps = sleeping && sleep.bodyAwake[best] != stamp;
-            accumulate_contact(k, recip, g, (int)best, bodyIn[best], mats[best], counts[best],
-                               otherSleeps, fallback, ppx, ppy, vpx, vpy, ownMat, own,
-                               writePressure, acc);
+        // More contacts than slots: the rest in ascending index, OVERFLOW_CHUNK
+        // at a time, each chunk the smallest indices above the last one taken,
+        // gathered by one walk of the grid.
+        const bool ownFluid = k.liquid && ownMat.z == 0.0f;
+        const int cx = cell_x(s, me.x), cy = cell_y(s, me.y);
+        uint32_t rest = own - (uint32_t)CONTACT_SLOTS;
+        while (rest > 0u) {
+            uint32_t next[OVERFLOW_CHUNK];
+            if (k.bounded)
+                next_contacts<true>(s, cx, cy, grid, guardLimit, g, ppx, ppy, last, ownFluid, mats, next);
+            else
+                next_contacts<false>(s, cx, cy, grid, guardLimit, g, ppx, ppy, last, ownFluid, mats, next);
+            uint32_t taken = 0u;
+#pragma unroll
+            for (int i0 = 0; i0 < OVERFLOW_CHUNK; i0 += OVERFLOW_BATCH) {
+                uint32_t id[OVERFLOW_BATCH];
+#pragma unroll
+                for (int q = 0; q < OVERFLOW_BATCH; q++)
+                    id[q] = i0 + q < OVERFLOW_CHUNK && next[i0 + q] != 0xffffffffu ? next[i0 + q]
+                                                                                     : (uint32_t)g;
+                float4 ob[OVERFLOW_BATCH];
+                float2 mf[OVERFLOW_BATCH];
+                uint32_t oc[OVERFLOW_BATCH];
+#pragma unroll
+                for (int q = 0; q < OVERFLOW_BATCH; q++) {
+                    ob[q] = bodyIn[id[q]];
+                    mf[q] = matXY[2 * (size_t)id[q]];
+                    oc[q] = counts[id[q]];
+                }
+#pragma unroll
+                for (int q = 0; q < OVERFLOW_BATCH; q++) {
+                    if (i0 + q < OVERFLOW_CHUNK && next[i0 + q] != 0xffffffffu && taken < rest) {
+                        last = id[q];
+                        taken++;
+                        float solid = k.liquid ? mats[last].z : 1.0f;
+                        bool otherSleeps = sleeping && sleep.bodyAwake[last] != stamp;
+                        accumulate_contact(k, recip, g, (int)last, ob[q],
+                                           make_float4(mf[q].x, mf[q].y, solid, 0.0f), oc[q], otherSleeps,
+                                           fallback, ppx, ppy, vpx, vpy, ownMat, own,
+                                           writePressure, acc);
+                    }
+                }
+            }
+            if (taken == 0u) break;
+            rest -= taken;
         }
     }
     if (k.bounded) {
