@@ -8,7 +8,7 @@
 //           of the CONTACT_SLOTS smallest contact indices (kept sorted in
 //           registers while walking);
 //   solve : Jacobi correction in ascending contact order (a body with more
-//           contacts continues past its slots by an ordered grid walk), walls,
+//           contacts takes the rest six per ordered grid walk), walls,
 //           integration, then, fused, the next pass's drift and its insertion
 //           into the other grid buffer.
 // The broad phase is a linked-cell grid of cells at least diameter + guard
