@@ -55,6 +55,193 @@ __device__ __forceinline__ float2 adjoint_w(const float2* __restrict__ xbar,
     return make_float2(b.x + dt * a.x, b.y + dt * a.y);
 }
 
+// The near-field terms of J_A^T w for any radius, a cell at a time: this
+// body's own terms (S/kick.comp:118-151) against its position, and the terms
+// of the bodies that see this body exactly against this body's position.
+__device__ __forceinline__ void near_vjp_any(const SimDesc& s, const uint4* __restrict__ cellRec,
+                                             const float4* __restrict__ cellBody,
+                                             const float2* __restrict__ xbar,
+                                             const float2* __restrict__ vbar, float dt, int g,
+                                             float4 p, float mass, float2 w, int myCount,
+                                             int myRank, float& gx, float& gy) {
+#pragma clang fp contract(fast)
+    const float cellMassUnit = 1.0f / 256.0f;
+    int hx = (int)floorf(p.x * (float)s.pw), hy = (int)floorf(p.y * (float)s.ph);
+    for (int dy = -s.ry; dy <= s.ry; dy++) {
+        int cy = hy + dy;
+        if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
+        int wy = wrap_axis(cy, s.ph);
+        float cpy = adj_min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
+        const uint4* row = cellRec + (s.physBase + wy * s.pw);
+        for (int dx = -s.rx; dx <= s.rx; dx++) {
+            int cx = hx + dx;
+            if (s.bounded && (cx < 0 || cx >= s.pw)) continue;
+            int wx = wrap_axis(cx, s.pw);
+            uint4 rec = row[wx];
+            int count = (int)rec.y;
+            if (count == 0) continue;
+            bool isHome = dx == 0 && dy == 0;
+            float dpx, dpy;
+            if (!isHome && count > s.exactLimit) {
+                float cpx = adj_min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
+                near_term_vjp(s, cpx, cpy, (float)rec.z * cellMassUnit, w.x, w.y, dpx, dpy);
+                gx -= s.aspect * dpx;
+                gy -= dpy;
+            } else {
+                int limit = isHome ? s.homeLimit : s.exactLimit;
+                int stop = (int)rec.x + (count < limit ? count : limit);
+                for (int t = (int)rec.x; t < stop; t++) {
+                    float4 o = cellBody[t];
+                    if (__float_as_uint(o.w) == (uint32_t)g) continue;
+                    float ox = adj_min_image(s, o.x - p.x) * s.aspect;
+                    float oy = adj_min_image(s, o.y - p.y);
+                    near_term_vjp(s, ox, oy, o.z, w.x, w.y, dpx, dpy);
+                    gx -= s.aspect * dpx;
+                    gy -= dpy;
+                }
+            }
+            bool seen = isHome ? myRank < s.homeLimit : myCount <= s.exactLimit;
+            if (!seen) continue;
+            for (int t = (int)rec.x; t < (int)rec.x + count; t++) {
+                float4 o = cellBody[t];
+                int k = (int)__float_as_uint(o.w);
+                if (k == g) continue;
+                float2 wk = adjoint_w(xbar, vbar, k, dt);
+                float ox = adj_min_image(s, p.x - o.x) * s.aspect;
+                float oy = adj_min_image(s, p.y - o.y);
+                near_term_vjp(s, ox, oy, mass, wk.x, wk.y, dpx, dpy);
+                gx += s.aspect * dpx;
+                gy += dpy;
+            }
+        }
+    }
+}
+
+// The same terms for rx = ry = R, a stencil row at a time: the row's records
+// load together, then the first body of every cell this body takes or is
+// seen by (and the adjoint of each body that sees it), then the second ones,
+// then any further ones, so a row costs a few memory round trips however
+// many of its cells hold bodies taken one by one.
+template <int R>
+__device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __restrict__ cellRec,
+                                              const float4* __restrict__ cellBody,
+                                              const float2* __restrict__ xbar,
+                                              const float2* __restrict__ vbar, float dt, int g,
+                                              float4 p, float mass, float2 w, int myCount,
+                                              int myRank, float& gx, float& gy) {
+#pragma clang fp contract(fast)
+    const float cellMassUnit = 1.0f / 256.0f;
+    const int exactLimit = s.exactLimit, homeLimit = s.homeLimit;
+    constexpr int D = 2 * R + 1;
+    int hx = (int)floorf(p.x * (float)s.pw), hy = (int)floorf(p.y * (float)s.ph);
+    uint32_t colOk = 0u;
+    float cpx[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        int cx = hx + k - R;
+        if (!s.bounded || (cx >= 0 && cx < s.pw)) colOk |= 1u << k;
+        int wx = wrap_axis(cx, s.pw);
+        cpx[k] = adj_min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
+    }
+    // A body sees this one exactly if it shares its home cell and this body is
+    // among the first homeLimit entries, or if this body's cell is exact.
+    const bool seenOutside = myCount <= exactLimit;
+    const bool seenHome = myRank < homeLimit;
+#pragma unroll 1
+    for (int j = 0; j < D; j++) {
+        int cy = hy + j - R;
+        if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
+        int wy = wrap_axis(cy, s.ph);
+        float cpy = adj_min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
+        const uint4* row = cellRec + (s.physBase + wy * s.pw);
+        const bool homeRow = j == R;
+        uint4 rec[D];
+#pragma unroll
+        for (int k = 0; k < D; k++)
+            rec[k] = (colOk >> k) & 1u ? row[wrap_axis(hx + k - R, s.pw)] : make_uint4(0u, 0u, 0u, 0u);
+        int take[D], need[D];
+        uint32_t seenMask = 0u;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            int count = (int)rec[k].y;
+            bool home = homeRow && k == R;
+            take[k] = home ? min(count, homeLimit) : (count <= exactLimit ? count : 0);
+            bool seen = count > 0 && (home ? seenHome : seenOutside);
+            if (seen) seenMask |= 1u << k;
+            need[k] = seen ? count : take[k];
+        }
+        float dpx, dpy;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            int count = (int)rec[k].y;
+            bool home = homeRow && k == R;
+            if (!home && count > exactLimit) {
+                near_term_vjp(s, cpx[k], cpy, (float)rec[k].z * cellMassUnit, w.x, w.y, dpx, dpy);
+                gx -= s.aspect * dpx;
+                gy -= dpy;
+            }
+        }
+        // Entries 0 and 1 of every cell, each round's loads together.
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            float4 o[D];
+#pragma unroll
+            for (int k = 0; k < D; k++)
+                if (need[k] > t) o[k] = cellBody[rec[k].x + (uint32_t)t];
+            float2 wk[D];
+#pragma unroll
+            for (int k = 0; k < D; k++)
+                if (need[k] > t && ((seenMask >> k) & 1u) && __float_as_uint(o[k].w) != (uint32_t)g)
+                    wk[k] = adjoint_w(xbar, vbar, (int)__float_as_uint(o[k].w), dt);
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                if (need[k] <= t || __float_as_uint(o[k].w) == (uint32_t)g) continue;
+                if (take[k] > t) {
+                    float ox = adj_min_image(s, o[k].x - p.x) * s.aspect;
+                    float oy = adj_min_image(s, o[k].y - p.y);
+                    near_term_vjp(s, ox, oy, o[k].z, w.x, w.y, dpx, dpy);
+                    gx -= s.aspect * dpx;
+                    gy -= dpy;
+                }
+                if ((seenMask >> k) & 1u) {
+                    float ox = adj_min_image(s, p.x - o[k].x) * s.aspect;
+                    float oy = adj_min_image(s, p.y - o[k].y);
+                    near_term_vjp(s, ox, oy, mass, wk[k].x, wk[k].y, dpx, dpy);
+                    gx += s.aspect * dpx;
+                    gy += dpy;
+                }
+            }
+            // the next round's loads after this round's values are consumed (registers)
+            asm volatile("" ::: "memory");
+        }
+        // Further entries: the home cell's, and every entry of a cell that sees
+        // this body and holds more than two.
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            for (int t = 2; t < need[k]; t++) {
+                float4 o = cellBody[rec[k].x + (uint32_t)t];
+                int id = (int)__float_as_uint(o.w);
+                if (id == g) continue;
+                if (take[k] > t) {
+                    float ox = adj_min_image(s, o.x - p.x) * s.aspect;
+                    float oy = adj_min_image(s, o.y - p.y);
+                    near_term_vjp(s, ox, oy, o.z, w.x, w.y, dpx, dpy);
+                    gx -= s.aspect * dpx;
+                    gy -= dpy;
+                }
+                if ((seenMask >> k) & 1u) {
+                    float2 wo = adjoint_w(xbar, vbar, id, dt);
+                    float ox = adj_min_image(s, p.x - o.x) * s.aspect;
+                    float oy = adj_min_image(s, p.y - o.y);
+                    near_term_vjp(s, ox, oy, mass, wo.x, wo.y, dpx, dpy);
+                    gx += s.aspect * dpx;
+                    gy += dpy;
+                }
+            }
+        }
+    }
+}
+
 // Per body: the gather's own-position term and the near-field terms of
 // J_A^T w; also the value the transposed gather spreads, (wx, -wy).
 __global__ void __launch_bounds__(WG) k_adj_direct(const SimDesc* __restrict__ sims,
@@ -95,63 +282,18 @@ __global__ void __launch_bounds__(WG) k_adj_direct(const SimDesc* __restrict__ s
         gy += (float)s.ph * ((d01 - d00) * (1.0f - fx) + (d11 - d10) * fx);
     }
     if (s.mutual) {
-        const float cellMassUnit = 1.0f / 256.0f;
         int hx = (int)floorf(p.x * (float)s.pw), hy = (int)floorf(p.y * (float)s.ph);
         int home = s.physBase + wrap_axis(hx, s.pw) + wrap_axis(hy, s.ph) * s.pw;
         uint4 own = cellRec[home];
         int myCount = (int)own.y, myRank = 0;
         for (int t = (int)own.x; t < (int)own.x + myCount; t++)
             if (__float_as_uint(cellBody[t].w) == (uint32_t)g) myRank = t - (int)own.x;
-        for (int dy = -s.ry; dy <= s.ry; dy++) {
-            int cy = hy + dy;
-            if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
-            int wy = wrap_axis(cy, s.ph);
-            float cpy = adj_min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
-            const uint4* row = cellRec + (s.physBase + wy * s.pw);
-            for (int dx = -s.rx; dx <= s.rx; dx++) {
-                int cx = hx + dx;
-                if (s.bounded && (cx < 0 || cx >= s.pw)) continue;
-                int wx = wrap_axis(cx, s.pw);
-                uint4 rec = row[wx];
-                int count = (int)rec.y;
-                if (count == 0) continue;
-                bool isHome = dx == 0 && dy == 0;
-                float dpx, dpy;
-                // this body's own terms (S/kick.comp:118-151), against its position
-                if (!isHome && count > s.exactLimit) {
-                    float cpx = adj_min_image(s, ((float)wx + 0.5f) * s.invPw - p.x) * s.aspect;
-                    near_term_vjp(s, cpx, cpy, (float)rec.z * cellMassUnit, w.x, w.y, dpx, dpy);
-                    gx -= s.aspect * dpx;
-                    gy -= dpy;
-                } else {
-                    int limit = isHome ? s.homeLimit : s.exactLimit;
-                    int stop = (int)rec.x + (count < limit ? count : limit);
-                    for (int t = (int)rec.x; t < stop; t++) {
-                        float4 o = cellBody[t];
-                        if (__float_as_uint(o.w) == (uint32_t)g) continue;
-                        float ox = adj_min_image(s, o.x - p.x) * s.aspect;
-                        float oy = adj_min_image(s, o.y - p.y);
-                        near_term_vjp(s, ox, oy, o.z, w.x, w.y, dpx, dpy);
-                        gx -= s.aspect * dpx;
-                        gy -= dpy;
-                    }
-                }
-                // the terms of the bodies of this cell that see this body exactly
-                bool seen = isHome ? myRank < s.homeLimit : myCount <= s.exactLimit;
-                if (!seen) continue;
-                for (int t = (int)rec.x; t < (int)rec.x + count; t++) {
-                    float4 o = cellBody[t];
-                    int k = (int)__float_as_uint(o.w);
-                    if (k == g) continue;
-                    float2 wk = adjoint_w(xbar, vbar, k, dt);
-                    float ox = adj_min_image(s, p.x - o.x) * s.aspect;
-                    float oy = adj_min_image(s, p.y - o.y);
-                    near_term_vjp(s, ox, oy, mass, wk.x, wk.y, dpx, dpy);
-                    gx += s.aspect * dpx;
-                    gy += dpy;
-                }
-            }
-        }
+        if (s.rx == 3 && s.ry == 3)
+            near_vjp_rows<3>(s, cellRec, cellBody, xbar, vbar, dt, g, p, mass, w, myCount, myRank,
+                             gx, gy);
+        else
+            near_vjp_any(s, cellRec, cellBody, xbar, vbar, dt, g, p, mass, w, myCount, myRank, gx,
+                         gy);
     }
     gbar[g] = make_float2(gx, gy);
 }
