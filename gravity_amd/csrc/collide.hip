@@ -264,7 +264,7 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, cons
                                                const float4* __restrict__ mats,
                                                uint32_t* __restrict__ counts,
                                                uint32_t* __restrict__ contacts, int planeStride,
-                                               int guardLimit) {
+                                               int guardLimit, const ContactPool& pool) {
     const float4 own = grid.nodes[g];
     const float px = own.x, py = own.y, vw = s.vw, vh = s.vh, D2 = s.diameter2;
     const uint32_t cell = __float_as_uint(own.w);
@@ -273,8 +273,8 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, cons
 #pragma unroll
     for (int k = 0; k < CONTACT_SLOTS; k++) slot[k] = 0xffffffffu;
     uint32_t n = 0;
-    walk_stencil<BOUNDED>(s, (int)(cell & 0xffffu), (int)(cell >> 16), grid, guardLimit,
-                          [&](int other, float ox, float oy) {
+    const int cx = (int)(cell & 0xffffu), cy = (int)(cell >> 16);
+    walk_stencil<BOUNDED>(s, cx, cy, grid, guardLimit, [&](int other, float ox, float oy) {
         float d2 = pair_d2<BOUNDED>(g, px, py, other, ox, oy, vw, vh);
         if (!(d2 < D2) || other == g) return;
         if (ownFluid && mats[other].z == 0.0f) return;
@@ -291,6 +291,27 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, cons
 #pragma unroll
     for (int k = 0; k < CONTACT_SLOTS; k++)
         if ((uint32_t)k < n) contacts[(size_t)k * planeStride + g] = slot[k];
+    if (n > (uint32_t)CONTACT_SLOTS) {
+        // The contacts past the listed ones, found by a second walk, go to the
+        // pool, so the solve takes them without walking the grid again.
+        const uint32_t need = n - (uint32_t)CONTACT_SLOTS, floorId = slot[CONTACT_SLOTS - 1];
+        uint32_t b = atomicAdd(pool.top, need);
+        if (b > pool.capacity || pool.capacity - b < need) {
+            b = NO_POOL;
+        } else {
+            uint32_t e = 0u;
+            walk_stencil<BOUNDED>(s, cx, cy, grid, guardLimit, [&](int other, float ox, float oy) {
+                uint32_t u = (uint32_t)other;
+                if (u <= floorId || other == g) return;
+                float d2 = pair_d2<BOUNDED>(g, px, py, other, ox, oy, vw, vh);
+                if (!(d2 < D2)) return;
+                if (ownFluid && mats[other].z == 0.0f) return;
+                if (e < need) pool.items[b + e] = u;
+                e++;
+            });
+        }
+        pool.base[g] = b;
+    }
     return n;
 }
 
@@ -302,7 +323,7 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
                                               const float4* __restrict__ mats,
                                               uint32_t* __restrict__ counts,
                                               uint32_t* __restrict__ contacts, int planeStride,
-                                              int guardLimit, SleepBufs sleep) {
+                                              int guardLimit, ContactPool pool, SleepBufs sleep) {
     const int bi = order[blockIdx.x];
     BodyBlock b = blocks[bi];
     const SimDesc& s = sims[b.sim];
@@ -318,9 +339,9 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
         }
         if (!asleep) {
             n = s.bounded ? count_contacts<true>(s, g, grid, mats, counts, contacts, planeStride,
-                                                 guardLimit)
+                                                 guardLimit, pool)
                           : count_contacts<false>(s, g, grid, mats, counts, contacts, planeStride,
-                                                  guardLimit);
+                                                  guardLimit, pool);
         }
     }
     // The wave's contacts, for the next step's dispatch order.
@@ -544,6 +565,27 @@ __device__ __forceinline__ void next_contacts(const SimDesc& s, int cx, int cy, 
     });
 }
 
+// The same from a body's pool entries (its contacts past the listed ones, in
+// any order): the OVERFLOW_CHUNK smallest above floorId, ascending.
+__device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, uint32_t count,
+                                          uint32_t floorId, uint32_t (&next)[OVERFLOW_CHUNK]) {
+#pragma unroll
+    for (int q = 0; q < OVERFLOW_CHUNK; q++) next[q] = 0xffffffffu;
+    constexpr uint32_t U = 4;
+    for (uint32_t e = 0; e < count; e += U) {
+        uint32_t u[U];
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++) u[q] = e + q < count ? items[e + q] : 0xffffffffu;
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++) {
+            if (u[q] <= floorId || u[q] >= next[OVERFLOW_CHUNK - 1]) continue;
+#pragma unroll
+            for (int k = OVERFLOW_CHUNK - 1; k > 0; k--) next[k] = med3u(next[k - 1], u[q], next[k]);
+            next[0] = min(u[q], next[0]);
+        }
+    }
+}
+
 // Contacts whose loads are issued together. The batch loop is not
 // unrolled: four contacts' registers keep the kernel at 7 waves per SIMD,
 // which holds every wave of a 400k-body launch resident at once.
@@ -561,7 +603,10 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               const float2* __restrict__ fallback,
                                               float* __restrict__ pressures, int writePressure,
                                               float nextDriftDt, int insertNext, GridBuf next,
-                                              SleepBufs sleep) {
+                                              ContactPool pool, SleepBufs sleep) {
+    // The pool's entries were all written by the count; its counter starts
+    // the next count from zero (no kernel of this launch reads it).
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pool.top = 0u;
     __shared__ float recip[RECIP_TABLE];
     if (threadIdx.x < RECIP_TABLE) recip[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 1.0f;
     __syncthreads();
@@ -622,13 +667,17 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     if (own > (uint32_t)CONTACT_SLOTS) {
         // More contacts than slots: the rest in ascending index, OVERFLOW_CHUNK
         // at a time, each chunk the smallest indices above the last one taken,
-        // gathered by one walk of the grid.
+        // selected from the body's pool entries (or, when the pool was full,
+        // gathered by one walk of the grid).
         const bool ownFluid = k.liquid && ownMat.z == 0.0f;
         const int cx = cell_x(s, me.x), cy = cell_y(s, me.y);
+        const uint32_t pb = pool.base[g];
         uint32_t rest = own - (uint32_t)CONTACT_SLOTS;
         while (rest > 0u) {
             uint32_t next[OVERFLOW_CHUNK];
-            if (k.bounded)
+            if (pb != NO_POOL)
+                pool_next(pool.items + pb, own - (uint32_t)CONTACT_SLOTS, last, next);
+            else if (k.bounded)
                 next_contacts<true>(s, cx, cy, grid, guardLimit, g, ppx, ppy, last, ownFluid, mats, next);
             else
                 next_contacts<false>(s, cx, cy, grid, guardLimit, g, ppx, ppy, last, ownFluid, mats, next);
@@ -763,7 +812,7 @@ void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid
     launch(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
                        ls.waveWork,
                        grid_of(grid), grid_of(prev), clearPrev, mats, counts, contacts,
-                       ls.planeStride, ls.guardLimit, ls.sleep);
+                       ls.planeStride, ls.guardLimit, ls.pool, ls.sleep);
 }
 
 void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, float4* bodyOut,
@@ -775,7 +824,7 @@ void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, f
                        bodyIn,
                        bodyOut, mats, counts, contacts, ls.planeStride, grid_of(grid),
                        ls.guardLimit, fallback, pressures, writePressure, nextDriftDt, insertNext,
-                       grid_of(next), ls.sleep);
+                       grid_of(next), ls.pool, ls.sleep);
 }
 
 }  // namespace gs

@@ -169,6 +169,9 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     press_ = dalloc<float>(S);
     counts_ = dalloc<uint32_t>(S);
     contacts_ = dalloc<uint32_t>(S * CONTACT_SLOTS);
+    poolItems_ = dalloc<uint32_t>(S * CONTACT_SLOTS);
+    poolBase_ = dalloc<uint32_t>(S);
+    poolTop_ = dalloc<uint32_t>(1);
     tileCount_ = dalloc<uint32_t>((size_t)totalTiles_ + 1);
     tilePrefix_ = dalloc<uint32_t>((size_t)totalTiles_ + 1);
     keyOf_ = dalloc<uint32_t>(S);
@@ -268,7 +271,7 @@ Engine::~Engine() {
                     homeOf_, slotBody_, cellRec_, cellBody_, spec_, green_, tw_, fallback_,
                     liqCount_, liqPrefix_, liqCellOf_, liqUnordered_, liqItems_, lambdas_,
                     sleepState_, bodyAwake_, tileAwake_, tileActive_, dTileBlocks_,
-                    dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_};
+                    dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_, poolItems_, poolBase_, poolTop_};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (st_) hipStreamDestroy(st_);
@@ -442,6 +445,7 @@ void Engine::rebuild_tables() {
     shape_.sleep = SleepBufs{sleepState_, bodyAwake_, tileAwake_, tileActive_};
     shape_.order = order_;
     shape_.waveWork = waveWork_;
+    shape_.pool = ContactPool{poolItems_, poolBase_, poolTop_, (uint32_t)totalSlots_ * CONTACT_SLOTS};
     invalidate_graphs();
 }
 

@@ -129,6 +129,7 @@ private:
     float* press_ = nullptr;
     uint32_t* counts_ = nullptr;
     uint32_t* contacts_ = nullptr;
+    uint32_t *poolItems_ = nullptr, *poolBase_ = nullptr, *poolTop_ = nullptr;  // contact pool
     CollideGrid grid_[2] = {};
     static constexpr size_t HEADS_PAD = 64;   // zero heads before and after the cells
     uint32_t* headsAlloc_[2] = {nullptr, nullptr};

@@ -12,6 +12,19 @@ struct CollideGrid {
     int* cellOf;       // per body: the cell of its insertion
 };
 
+// Contacts past the listed ones, for bodies with more than CONTACT_SLOTS:
+// the count appends each such body's remaining contacts (in any order) to
+// items[base[g], base[g] + count - CONTACT_SLOTS), from a counter `top` that
+// the solve returns to zero; a body that finds the pool full gets NO_POOL and
+// the solve walks the grid for its remaining contacts instead.
+constexpr uint32_t NO_POOL = 0xffffffffu;
+struct ContactPool {
+    uint32_t* items;
+    uint32_t* base;
+    uint32_t* top;
+    uint32_t capacity;
+};
+
 // Shape of the body launches of a batch (device pointers + sizes).
 struct LaunchShape {
     const SimDesc* sims;
@@ -19,6 +32,7 @@ struct LaunchShape {
     int bodyBlocks;      // workgroups over all active bodies
     int planeStride;     // body slots of the batch (stride of the contact planes)
     int guardLimit;      // bound on any linked-list walk
+    ContactPool pool;    // remaining contacts of bodies with more than the listed ones
     SleepBufs sleep;     // sleeping tiles of the batch
     // Dispatch order of the contact kernels: workgroup i takes body block
     // order[i] (a permutation keeping every block on the XCD xcd_block gives
