@@ -185,6 +185,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     homeOf_ = dalloc<uint32_t>(S);
     slotBody_ = dalloc<uint32_t>(S);
     cellRec_ = dalloc<uint4>((size_t)totalPhys_);
+    cellNear_ = dalloc<uint4>(2 * (size_t)totalPhys_);
     cellBody_ = dalloc<float4>(S);
     liqCount_ = dalloc<uint32_t>((size_t)totalLiq_ + 1);
     liqPrefix_ = dalloc<uint32_t>((size_t)totalLiq_ + 1);
@@ -268,7 +269,7 @@ Engine::~Engine() {
                     headsAlloc_[0], headsAlloc_[1], grid_[0].nodes, grid_[1].nodes,
                     grid_[0].cellOf, grid_[1].cellOf, press_, counts_, contacts_, tileCount_,
                     tilePrefix_, keyOf_, slotSrc_, dest_, scanTmp_, cellCount_, cellPrefix_,
-                    homeOf_, slotBody_, cellRec_, cellBody_, spec_, green_, tw_, fallback_,
+                    homeOf_, slotBody_, cellRec_, cellNear_, cellBody_, spec_, green_, tw_, fallback_,
                     liqCount_, liqPrefix_, liqCellOf_, liqUnordered_, liqItems_, lambdas_,
                     sleepState_, bodyAwake_, tileAwake_, tileActive_, dTileBlocks_,
                     dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_, poolItems_, poolBase_, poolTop_};
@@ -645,12 +646,12 @@ void Engine::enqueue_force() {
     if (anyMutual_) {
         timed("bin", [&] { launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_); });
         timed("cell_scan", [&] {
-            launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, cellRec_);
+            launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, cellRec_, cellNear_);
         });
         timed("cell_scatter", [&] { launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_); });
         timed("cell_canon", [&] {
             launch_cell_canon(st_, gravitySlots_, homeOf_, cellPrefix_, slotBody_, body_[bodyPar_],
-                              mat_[auxPar_], cellBody_, cellRec_);
+                              mat_[auxPar_], cellBody_, cellRec_, cellNear_);
         });
         timed("fft_rows_forward", [&] {
             launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_);
@@ -662,8 +663,8 @@ void Engine::enqueue_force() {
 
 void Engine::enqueue_kick(bool initial) {
     timed("kick", [&] {
-        launch_kick(st_, shape_, body_[bodyPar_], acc_[auxPar_], spec_, cellRec_, cellBody_,
-                    initial ? 1 : 0);
+        launch_kick(st_, shape_, body_[bodyPar_], acc_[auxPar_], spec_, cellRec_, cellNear_,
+                    cellBody_, initial ? 1 : 0);
     });
 }
 

@@ -140,6 +140,7 @@ private:
     uint32_t *cellCount_ = nullptr, *cellPrefix_ = nullptr, *homeOf_ = nullptr;
     uint32_t* slotBody_ = nullptr;
     uint4* cellRec_ = nullptr;  // per physical cell: first slot, count, mass / 256, 0
+    uint4* cellNear_ = nullptr;  // per physical cell: the kick's 32-byte near record
     float4* cellBody_ = nullptr;
     // liquid: bodies counting-sorted by their start cell, ascending per cell
     uint32_t *liqCount_ = nullptr, *liqPrefix_ = nullptr, *liqCellOf_ = nullptr;

@@ -47,9 +47,10 @@ struct LineWork { int sim; int first; int count; int pad; };
 
 // ---- scan: out[i] = sum of in[0..i) for i in [0, len]; totals is scratch ----
 // Exclusive scan of len values (out[len] = total). rec, if given, receives
-// {prefix, count, 0, 0} as each element's 16-byte record.
+// {prefix, count, 0, 0} as each element's 16-byte record, and near (two
+// uint4 per element) {count, 0, 0, 0} as the first half of its near record.
 void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals,
-                 uint4* rec = nullptr);
+                 uint4* rec = nullptr, uint4* near = nullptr);
 int scan_totals_size(int len);
 
 // ---- spatial sort ----
@@ -85,9 +86,13 @@ void launch_bin(hipStream_t st, const LaunchShape& ls, const float4* body, uint3
                 uint32_t* homeOf);
 void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* homeOf,
                          uint32_t* cellCount, const uint32_t* cellPrefix, uint32_t* slotBody);
+// cellNear: per physical cell two uint4, {count, mass in 1/256, x0, y0} and
+// {x1, y1, m1, m0}: a cell of at most NEAR_INLINE bodies carries its list's
+// bodies (x, y normalised, m mass) in the record itself.
+constexpr int NEAR_INLINE = 2;
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
-                       const float4* mats, float4* cellBody, uint4* cellRec);
+                       const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear);
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
                              const float2* twiddles, int twiddleLength);
@@ -98,7 +103,8 @@ void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork
                              float2* spec, const float2* twiddles, int twiddleLength);
 // cellRec: per physical cell {first slot, body count, mass in 1/256, 0}.
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
-                 const float2* force, const uint4* cellRec, const float4* cellBody, int initial);
+                 const float2* force, const uint4* cellRec, const uint4* cellNear,
+                 const float4* cellBody, int initial);
 
 // ---- sleeping tiles: wake the next step's tiles, then advance the stamps ----
 void launch_sleep_advance(hipStream_t st, const SimDesc* sims, int simCount,

@@ -62,15 +62,18 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
 
 // Orders each cell by descending body index and sums the cell mass in 1/256
 // units (S/build_grid.comp:75), exactly: the slots of a cell are contiguous,
-// so a wave adds its runs of one cell in registers and one atomic per run
-// accumulates them into the cell record (whose mass the cell scan zeroed).
+// so a wave adds its runs of one cell in registers; a run that is a whole
+// cell stores its sum, any other accumulates it with one atomic into the
+// cell record and the near record (whose masses the cell scan zeroed). The
+// bodies of a cell of at most NEAR_INLINE go into its near record as well.
 __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __restrict__ homeOf,
                                                    const uint32_t* __restrict__ cellPrefix,
                                                    const uint32_t* __restrict__ slotBody,
                                                    const float4* __restrict__ body,
                                                    const float4* __restrict__ mats,
                                                    float4* __restrict__ cellBody,
-                                                   uint4* __restrict__ cellRec) {
+                                                   uint4* __restrict__ cellRec,
+                                                   uint4* __restrict__ cellNear) {
     int d = (int)(blockIdx.x * WG + threadIdx.x);
     const bool active = d < slots;
     uint32_t c = 0xffffffffu, units = 0u;
@@ -84,6 +87,16 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
         float m = mats[g].x;
         cellBody[s0 + rank] = make_float4(p.x, p.y, m, __uint_as_float(g));
         units = (uint32_t)rintf(m * 256.0f);
+        if (s1 - s0 <= NEAR_INLINE) {
+            float* rec = reinterpret_cast<float*>(cellNear + 2 * (size_t)c);
+            if (rank == 0) {
+                *reinterpret_cast<float2*>(rec + 2) = make_float2(p.x, p.y);
+                rec[7] = m;
+            } else {
+                *reinterpret_cast<float2*>(rec + 4) = make_float2(p.x, p.y);
+                rec[6] = m;
+            }
+        }
     }
     const LaneRun run = lane_run(c);
 #pragma unroll
@@ -91,8 +104,18 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
         uint32_t t = (uint32_t)__shfl_up((int)units, (unsigned)o, 64);
         if (run.rank >= o) units += t;
     }
-    if (active && run.rank == run.length - 1)
-        atomicAdd(&reinterpret_cast<uint32_t*>(cellRec)[4 * (size_t)c + 2], units);
+    if (active && run.rank == run.length - 1) {
+        uint32_t* mass = &reinterpret_cast<uint32_t*>(cellRec)[4 * (size_t)c + 2];
+        uint32_t* nearMass = &reinterpret_cast<uint32_t*>(cellNear)[8 * (size_t)c + 1];
+        if (cellPrefix[c] + (uint32_t)run.length == cellPrefix[c + 1]) {
+            // the run is the whole cell: no other wave adds to it
+            *mass = units;
+            *nearMass = units;
+        } else {
+            atomicAdd(mass, units);
+            atomicAdd(nearMass, units);
+        }
+    }
 }
 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
@@ -164,23 +187,44 @@ __device__ __forceinline__ void body_term(const SimDesc& s, float4 o, float x, f
     near_term(s, ox, oy, r2, __expf(-r2 * k2), o.z, ax, ay);
 }
 
-// Near field of rx = ry = R cells: the column offsets and Gaussian factors
-// of a body are computed once. Per stencil row the records load together,
-// then the first body of every cell taken body by body loads together, then
-// the second ones, so a row costs three memory round trips however many of
-// its cells are taken body by body; the monopoles are summed while the
-// bodies load.
+// Near field of rx = ry = R cells for exactNeighborBodyLimit <= NEAR_INLINE:
+// the column offsets and Gaussian factors of a body are computed once; the
+// home cell's bodies (its record hrec is loaded with the mesh force) load
+// together; then each stencil row loads the 32-byte near records of its
+// cells together, which hold the count, the mass and, for a cell taken body
+// by body, its bodies, so a row costs one memory round trip.
 template <int R>
-__device__ __forceinline__ void near_field_fixed(const SimDesc& s,
-                                                 const uint4* __restrict__ cellRec,
+__device__ __forceinline__ void near_field_fixed(const SimDesc& s, uint4 hrec,
+                                                 const uint4* __restrict__ cellNear,
                                                  const float4* __restrict__ cellBody, int g,
                                                  float x, float y, float& ax, float& ay) {
 #pragma clang fp contract(fast)
     const float k2 = s.invTwoSigma2;
     const float cellMassUnit = 1.0f / 256.0f;
-    const int exactLimit = s.exactLimit, homeLimit = s.homeLimit;
+    const uint32_t exactLimit = (uint32_t)s.exactLimit;
     int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
     constexpr int D = 2 * R + 1;
+    constexpr int HOME_BATCH = 8;
+    // The home cell: its first homeLimit list entries, the body itself
+    // skipped (under walls, none for a body on or beyond the far edge).
+    {
+        const bool homeOk = !s.bounded || (hx >= 0 && hx < s.pw && hy >= 0 && hy < s.ph);
+        const int take = homeOk ? min((int)hrec.y, s.homeLimit) : 0;
+        // The home cell holds the body itself, so its list is never empty and
+        // the loads, clamped to the list, are unconditional (no load waits on
+        // a branch).
+        const int last = max((int)hrec.y, 1) - 1;
+        float4 hb[HOME_BATCH];
+#pragma unroll
+        for (int t = 0; t < HOME_BATCH; t++) hb[t] = cellBody[hrec.x + (uint32_t)min(t, last)];
+#pragma unroll
+        for (int t = 0; t < HOME_BATCH; t++)
+            if (t < take && __float_as_uint(hb[t].w) != (uint32_t)g) body_term(s, hb[t], x, y, k2, ax, ay);
+        for (int t = HOME_BATCH; t < take; t++) {
+            float4 o = cellBody[hrec.x + (uint32_t)t];
+            if (__float_as_uint(o.w) != (uint32_t)g) body_term(s, o, x, y, k2, ax, ay);
+        }
+    }
     float cpx[D], cex[D];
     uint32_t colOk = 0u;
 #pragma unroll
@@ -200,59 +244,33 @@ __device__ __forceinline__ void near_field_fixed(const SimDesc& s,
         float py = min_image(s, ((float)wy + 0.5f) * s.invPh - y);
         float py2 = py * py;
         float ey = __expf(-py2 * k2);
-        const uint4* row = cellRec + (s.physBase + wy * s.pw);
-        uint4 rec[D];
-#pragma unroll
-        for (int k = 0; k < D; k++)
-            rec[k] = (colOk >> k) & 1u ? row[wrap_axis(hx + k - R, s.pw)] : make_uint4(0u, 0u, 0u, 0u);
-        const bool homeRow = j == R;
-        // Bodies a cell contributes one by one: the home cell's first
-        // homeLimit list entries, or all of a cell of at most exactLimit.
-        int take[D];
+        const uint4* row = cellNear + 2 * (size_t)(s.physBase + wy * s.pw);
+        const uint32_t rowOk = j == R ? colOk & ~(1u << R) : colOk;
+        // Every wrapped column is a valid record: the loads are unconditional
+        // and a column that does not take part reads as an empty cell.
+        uint4 lo[D], hi[D];
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            int count = (int)rec[k].y;
-            bool home = homeRow && k == R;
-            take[k] = home ? min(count, homeLimit) : (count <= exactLimit ? count : 0);
+            const uint4* rec = row + 2 * wrap_axis(hx + k - R, s.pw);
+            lo[k] = rec[0];
+            hi[k] = rec[1];
         }
-        float4 b0[D];
-#pragma unroll
-        for (int k = 0; k < D; k++)
-            if (take[k] > 0) b0[k] = cellBody[rec[k].x];
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            int count = (int)rec[k].y;
-            bool home = homeRow && k == R;
-            if (!home && count > exactLimit) {
+            const uint32_t count = (rowOk >> k) & 1u ? lo[k].x : 0u;
+            if (count > exactLimit) {
                 float px = cpx[k];
-                near_term(s, px, py, px * px + py2, cex[k] * ey, (float)rec[k].z * cellMassUnit,
+                near_term(s, px, py, px * px + py2, cex[k] * ey, (float)lo[k].y * cellMassUnit,
                           ax, ay);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            // only the home cell can hold the body itself
-            bool self = homeRow && k == R && __float_as_uint(b0[k].w) == (uint32_t)g;
-            if (take[k] > 0 && !self) body_term(s, b0[k], x, y, k2, ax, ay);
-        }
-        // the second bodies load after the first ones are consumed (registers)
-        asm volatile("" ::: "memory");
-        float4 b1[D];
-#pragma unroll
-        for (int k = 0; k < D; k++)
-            if (take[k] > 1) b1[k] = cellBody[rec[k].x + 1u];
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            bool self = homeRow && k == R && __float_as_uint(b1[k].w) == (uint32_t)g;
-            if (take[k] > 1 && !self) body_term(s, b1[k], x, y, k2, ax, ay);
-        }
-        // Further bodies: the home cell's, and any cell's under a larger exactLimit.
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            for (int t = 2; t < take[k]; t++) {
-                float4 o = cellBody[rec[k].x + (uint32_t)t];
-                bool self = homeRow && k == R && __float_as_uint(o.w) == (uint32_t)g;
-                if (!self) body_term(s, o, x, y, k2, ax, ay);
+            } else {
+                if (count > 0u)
+                    body_term(s, make_float4(__uint_as_float(lo[k].z), __uint_as_float(lo[k].w),
+                                             __uint_as_float(hi[k].w), 0.0f),
+                              x, y, k2, ax, ay);
+                if (count > 1u)
+                    body_term(s, make_float4(__uint_as_float(hi[k].x), __uint_as_float(hi[k].y),
+                                             __uint_as_float(hi[k].z), 0.0f),
+                              x, y, k2, ax, ay);
             }
         }
     }
@@ -294,15 +312,17 @@ __device__ __forceinline__ void near_field_any(const SimDesc& s, const uint4* __
     }
 }
 
-// S/kick.comp:105-156. A simulation with the default 3-cell near field
-// takes near_field_fixed<3>, any other near_field_any; the choice is per
-// simulation, so a simulation kicks the same alone or in a batch.
+// S/kick.comp:105-156. A simulation with the default 3-cell near field (and
+// exact cells of at most NEAR_INLINE bodies) takes near_field_fixed<3>, any
+// other near_field_any; the choice is per simulation, so a simulation kicks
+// the same alone or in a batch.
 __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
                                              const BodyBlock* __restrict__ blocks,
                                              float4* __restrict__ body,
                                              float2* __restrict__ acc,
                                              const float2* __restrict__ force,
                                              const uint4* __restrict__ cellRec,
+                                             const uint4* __restrict__ cellNear,
                                              const float4* __restrict__ cellBody, int initial,
                                              SleepBufs sleep) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
@@ -325,12 +345,19 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
         acc[g] = make_float2(s.fieldX, s.fieldY);
         return;
     }
-    float2 mf = mesh_force(s, force, p.x, p.y);
-    float ax = mf.x + s.fieldX, ay = mf.y + s.fieldY;
-    if (s.rx == 3 && s.ry == 3)
-        near_field_fixed<3>(s, cellRec, cellBody, g, p.x, p.y, ax, ay);
-    else
+    float ax, ay;
+    if (s.rx == 3 && s.ry == 3 && s.exactLimit <= NEAR_INLINE) {
+        const uint4 hrec = cellRec[home_cell(s, p.x, p.y)];
+        float2 mf = mesh_force(s, force, p.x, p.y);
+        ax = mf.x + s.fieldX;
+        ay = mf.y + s.fieldY;
+        near_field_fixed<3>(s, hrec, cellNear, cellBody, g, p.x, p.y, ax, ay);
+    } else {
+        float2 mf = mesh_force(s, force, p.x, p.y);
+        ax = mf.x + s.fieldX;
+        ay = mf.y + s.fieldY;
         near_field_any(s, cellRec, cellBody, g, p.x, p.y, ax, ay);
+    }
     if (!initial) {
         p.z += ax * h;
         p.w += ay * h;
@@ -355,17 +382,18 @@ void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
 
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
-                       const float4* mats, float4* cellBody, uint4* cellRec) {
+                       const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear) {
     if (slots <= 0) return;
     launch(k_cell_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, homeOf,
-                       cellPrefix, slotBody, body, mats, cellBody, cellRec);
+                       cellPrefix, slotBody, body, mats, cellBody, cellRec, cellNear);
 }
 
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
-                 const float2* force, const uint4* cellRec, const float4* cellBody, int initial) {
+                 const float2* force, const uint4* cellRec, const uint4* cellNear,
+                 const float4* cellBody, int initial) {
     if (ls.bodyBlocks == 0) return;
     launch(k_kick, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
-                       force, cellRec, cellBody, initial, ls.sleep);
+                       force, cellRec, cellNear, cellBody, initial, ls.sleep);
 }
 
 }  // namespace gs

@@ -69,12 +69,14 @@ __global__ void __launch_bounds__(WG) k_scan_local(const uint32_t* __restrict__ 
 
 // Adds the block offsets: each workgroup sums the totals of the blocks
 // before it. With rec, also writes each element's 16-byte record
-// {prefix, count, 0, 0} (the cell tables of the kick; the cell canon
-// accumulates the mass into the third word).
+// {prefix, count, 0, 0} and the first half of its 32-byte near record
+// {count, 0, 0, 0} (the cell tables of the kick; the cell canon accumulates
+// the mass into the third word of the one and the second of the other).
 __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int len,
                                                  const uint32_t* __restrict__ totals,
                                                  const uint32_t* __restrict__ in,
-                                                 uint4* __restrict__ rec) {
+                                                 uint4* __restrict__ rec,
+                                                 uint4* __restrict__ near) {
     uint32_t part = 0u;
     for (int i = (int)threadIdx.x; i < (int)blockIdx.x; i += WG) part += totals[i];
     uint32_t add;
@@ -85,7 +87,11 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int
         if (idx <= len) {
             uint32_t v = out[idx] + add;
             out[idx] = v;
-            if (rec != nullptr && idx < len) rec[idx] = make_uint4(v, in[idx], 0u, 0u);
+            if (rec != nullptr && idx < len) {
+                uint32_t count = in[idx];
+                rec[idx] = make_uint4(v, count, 0u, 0u);
+                near[2 * (size_t)idx] = make_uint4(count, 0u, 0u, 0u);
+            }
         }
     }
 }
@@ -93,10 +99,10 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int
 int scan_totals_size(int len) { return (len + 1 + SCAN_BLOCK - 1) / SCAN_BLOCK; }
 
 void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals,
-                 uint4* rec) {
+                 uint4* rec, uint4* near) {
     int blocks = scan_totals_size(len);
     launch(k_scan_local, dim3(blocks), dim3(WG), 0, st, in, out, len, totals);
-    launch(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in, rec);
+    launch(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in, rec, near);
 }
 
 // ---------------------------------------------------------------------------
