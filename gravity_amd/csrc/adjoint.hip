@@ -155,10 +155,15 @@ __device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __r
         float cpy = adj_min_image(s, ((float)wy + 0.5f) * s.invPh - p.y);
         const uint4* row = cellRec + (s.physBase + wy * s.pw);
         const bool homeRow = j == R;
+        // Every wrapped column is a valid record: the loads are unconditional
+        // (no load waits on a branch) and a column that does not take part
+        // reads as an empty cell.
         uint4 rec[D];
 #pragma unroll
+        for (int k = 0; k < D; k++) rec[k] = row[wrap_axis(hx + k - R, s.pw)];
+#pragma unroll
         for (int k = 0; k < D; k++)
-            rec[k] = (colOk >> k) & 1u ? row[wrap_axis(hx + k - R, s.pw)] : make_uint4(0u, 0u, 0u, 0u);
+            if (!((colOk >> k) & 1u)) rec[k].y = 0u;
         int take[D], need[D];
         uint32_t seenMask = 0u;
 #pragma unroll
@@ -181,18 +186,20 @@ __device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __r
                 gy -= dpy;
             }
         }
-        // Entries 0 and 1 of every cell, each round's loads together.
+        // Entries 0 and 1 of every cell, each round's loads together and
+        // unconditional: an entry that is not needed reads slot 0 and its
+        // adjoint this body's own.
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             float4 o[D];
 #pragma unroll
-            for (int k = 0; k < D; k++)
-                if (need[k] > t) o[k] = cellBody[rec[k].x + (uint32_t)t];
+            for (int k = 0; k < D; k++) o[k] = cellBody[need[k] > t ? rec[k].x + (uint32_t)t : 0u];
             float2 wk[D];
 #pragma unroll
-            for (int k = 0; k < D; k++)
-                if (need[k] > t && ((seenMask >> k) & 1u) && __float_as_uint(o[k].w) != (uint32_t)g)
-                    wk[k] = adjoint_w(xbar, vbar, (int)__float_as_uint(o[k].w), dt);
+            for (int k = 0; k < D; k++) {
+                const bool use = need[k] > t && ((seenMask >> k) & 1u);
+                wk[k] = adjoint_w(xbar, vbar, use ? (int)__float_as_uint(o[k].w) : g, dt);
+            }
 #pragma unroll
             for (int k = 0; k < D; k++) {
                 if (need[k] <= t || __float_as_uint(o[k].w) == (uint32_t)g) continue;
@@ -215,27 +222,39 @@ __device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __r
             asm volatile("" ::: "memory");
         }
         // Further entries: the home cell's, and every entry of a cell that sees
-        // this body and holds more than two.
+        // this body and holds more than two; FURTHER_BATCH at a time, the
+        // loads clamped to the cell's list.
+        constexpr int FURTHER_BATCH = 4;
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            for (int t = 2; t < need[k]; t++) {
-                float4 o = cellBody[rec[k].x + (uint32_t)t];
-                int id = (int)__float_as_uint(o.w);
-                if (id == g) continue;
-                if (take[k] > t) {
-                    float ox = adj_min_image(s, o.x - p.x) * s.aspect;
-                    float oy = adj_min_image(s, o.y - p.y);
-                    near_term_vjp(s, ox, oy, o.z, w.x, w.y, dpx, dpy);
-                    gx -= s.aspect * dpx;
-                    gy -= dpy;
-                }
-                if ((seenMask >> k) & 1u) {
-                    float2 wo = adjoint_w(xbar, vbar, id, dt);
-                    float ox = adj_min_image(s, p.x - o.x) * s.aspect;
-                    float oy = adj_min_image(s, p.y - o.y);
-                    near_term_vjp(s, ox, oy, mass, wo.x, wo.y, dpx, dpy);
-                    gx += s.aspect * dpx;
-                    gy += dpy;
+            for (int t0 = 2; t0 < need[k]; t0 += FURTHER_BATCH) {
+                float4 o[FURTHER_BATCH];
+                float2 wo[FURTHER_BATCH];
+#pragma unroll
+                for (int q = 0; q < FURTHER_BATCH; q++)
+                    o[q] = cellBody[rec[k].x + (uint32_t)min(t0 + q, need[k] - 1)];
+#pragma unroll
+                for (int q = 0; q < FURTHER_BATCH; q++)
+                    wo[q] = adjoint_w(xbar, vbar, (seenMask >> k) & 1u ? (int)__float_as_uint(o[q].w) : g,
+                                      dt);
+#pragma unroll
+                for (int q = 0; q < FURTHER_BATCH; q++) {
+                    const int t = t0 + q;
+                    if (t >= need[k] || (int)__float_as_uint(o[q].w) == g) continue;
+                    if (take[k] > t) {
+                        float ox = adj_min_image(s, o[q].x - p.x) * s.aspect;
+                        float oy = adj_min_image(s, o[q].y - p.y);
+                        near_term_vjp(s, ox, oy, o[q].z, w.x, w.y, dpx, dpy);
+                        gx -= s.aspect * dpx;
+                        gy -= dpy;
+                    }
+                    if ((seenMask >> k) & 1u) {
+                        float ox = adj_min_image(s, p.x - o[q].x) * s.aspect;
+                        float oy = adj_min_image(s, p.y - o[q].y);
+                        near_term_vjp(s, ox, oy, mass, wo[q].x, wo[q].y, dpx, dpy);
+                        gx += s.aspect * dpx;
+                        gy += dpy;
+                    }
                 }
             }
         }
@@ -285,9 +304,19 @@ __global__ void __launch_bounds__(WG) k_adj_direct(const SimDesc* __restrict__ s
         int hx = (int)floorf(p.x * (float)s.pw), hy = (int)floorf(p.y * (float)s.ph);
         int home = s.physBase + wrap_axis(hx, s.pw) + wrap_axis(hy, s.ph) * s.pw;
         uint4 own = cellRec[home];
+        // The home cell holds this body, so its list is never empty; its
+        // entries load RANK_BATCH at a time, clamped to the list.
+        constexpr int RANK_BATCH = 8;
         int myCount = (int)own.y, myRank = 0;
-        for (int t = (int)own.x; t < (int)own.x + myCount; t++)
-            if (__float_as_uint(cellBody[t].w) == (uint32_t)g) myRank = t - (int)own.x;
+        for (int t0 = 0; t0 < myCount; t0 += RANK_BATCH) {
+            uint32_t id[RANK_BATCH];
+#pragma unroll
+            for (int q = 0; q < RANK_BATCH; q++)
+                id[q] = __float_as_uint(cellBody[own.x + (uint32_t)min(t0 + q, myCount - 1)].w);
+#pragma unroll
+            for (int q = 0; q < RANK_BATCH; q++)
+                if (t0 + q < myCount && id[q] == (uint32_t)g) myRank = t0 + q;
+        }
         if (s.rx == 3 && s.ry == 3)
             near_vjp_rows<3>(s, cellRec, cellBody, xbar, vbar, dt, g, p, mass, w, myCount, myRank,
                              gx, gy);
