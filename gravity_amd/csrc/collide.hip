@@ -6,9 +6,11 @@
 //   count : clears the previous pass's grid cells, walks the 3x3 stencil of
 //           this pass's grid, writes the contact count and the ascending list
 //           of the CONTACT_SLOTS smallest contact indices (kept sorted in
-//           registers while walking);
+//           registers while walking; the indices they push out go to the
+//           planes past them);
 //   solve : Jacobi correction in ascending contact order (a body with more
-//           contacts takes the rest six per ordered grid walk), walls,
+//           contacts takes the rest six at a time, in order, from the contact
+//           planes past the listed ones or the contact pool), walls,
 //           integration, then, fused, the next pass's drift and its insertion
 //           into the other grid buffer.
 // The broad phase is a linked-cell grid of cells at least diameter + guard
@@ -104,38 +106,51 @@ struct HeadRow {
 };
 
 // The heads of the 3x3 stencil around cell (cx, cy) of a grid at least 3
-// cells wide and high, one load per row: the heads array has a zero entry
-// before its first cell and after its last, so a row's load stays in the
-// allocation; cells past a wall read as empty. On the torus a row at the left
-// or right edge wraps and is loaded cell by cell.
+// cells wide and high, one 12-byte load per row, all three unconditional so
+// they are in flight together. The heads array has a zero entry before its
+// first cell and after its last, so a row's load stays in the allocation:
+// under walls a row past the edge is loaded clamped and reads as empty, as do
+// cells past a side wall. On the torus the load takes the three cells of the
+// row nearest the stencil, and a stencil at the left or right edge takes its
+// wrapped cell from one more load.
 template <bool BOUNDED>
 __device__ __forceinline__ void stencil_rows(const SimDesc& s, int cx, int cy,
                                              const uint32_t* __restrict__ heads,
                                              uint32_t (&e)[9]) {
     const int cgw = s.cgw, cgh = s.cgh;
-    const bool edge = !BOUNDED && (cx == 0 || cx == cgw - 1);
-    const int xl = cx == 0 ? cgw - 1 : cx - 1;
-    const int xr = cx == cgw - 1 ? 0 : cx + 1;
+    const bool left = cx == 0, right = cx == cgw - 1;
+    HeadRow h[3];
+    uint32_t wrapCell[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         int y = cy + j - 1;
-        bool oky = true;
-        if (BOUNDED) oky = y >= 0 && y < cgh;
-        else y = wrap_axis(y, cgh);
+        y = BOUNDED ? min(max(y, 0), cgh - 1) : wrap_axis(y, cgh);
         const uint32_t* row = heads + s.cellBase + y * cgw;
-        HeadRow h{0u, 0u, 0u};
-        if (oky) {
-            if (edge) {
-                h.a = row[xl];
-                h.b = row[cx];
-                h.c = row[xr];
-            } else {
-                h = *reinterpret_cast<const HeadRow*>(row + cx - 1);
-            }
+        if (BOUNDED) {
+            h[j] = *reinterpret_cast<const HeadRow*>(row + cx - 1);
+        } else {
+            h[j] = *reinterpret_cast<const HeadRow*>(row + min(max(cx, 1), cgw - 2) - 1);
+            wrapCell[j] = row[left ? cgw - 1 : 0];
         }
-        e[j * 3 + 0] = BOUNDED && cx == 0 ? 0u : h.a;
-        e[j * 3 + 1] = h.b;
-        e[j * 3 + 2] = BOUNDED && cx == cgw - 1 ? 0u : h.c;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        uint32_t a = h[j].a, b = h[j].b, c = h[j].c;
+        if (BOUNDED) {
+            const int y = cy + j - 1;
+            const bool oky = y >= 0 && y < cgh;
+            a = oky && !left ? a : 0u;
+            b = oky ? b : 0u;
+            c = oky && !right ? c : 0u;
+        } else {
+            const uint32_t ha = h[j].a, hb = h[j].b, hc = h[j].c;
+            a = left ? wrapCell[j] : (right ? hb : ha);
+            b = left ? ha : (right ? hc : hb);
+            c = left ? hb : (right ? wrapCell[j] : hc);
+        }
+        e[j * 3 + 0] = a;
+        e[j * 3 + 1] = b;
+        e[j * 3 + 2] = c;
     }
 }
 
@@ -281,19 +296,28 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, cons
         n++;
         // Insertion keeping the CONTACT_SLOTS smallest indices, ascending:
         // with s sorted, the new slot k is the median of s[k-1], u, s[k]
-        // (one v_med3_u32 each, all independent of one another).
+        // (one v_med3_u32 each, all independent of one another). The index
+        // it pushes out of the kept ones goes to the next free plane past
+        // them, while there is one (plane indices fit 32 bits: the engine
+        // caps the slots at 2^32 / CONTACT_PLANES).
         uint32_t u = (uint32_t)other;
+        const uint32_t out = max(u, slot[CONTACT_SLOTS - 1]);
 #pragma unroll
         for (int k = CONTACT_SLOTS - 1; k > 0; k--) slot[k] = med3u(slot[k - 1], u, slot[k]);
         slot[0] = min(u, slot[0]);
+        if (n > (uint32_t)CONTACT_SLOTS && n <= (uint32_t)CONTACT_PLANES)
+            contacts[(n - 1u) * (uint32_t)planeStride + (uint32_t)g] = out;
     });
     counts[g] = n;
 #pragma unroll
     for (int k = 0; k < CONTACT_SLOTS; k++)
         if ((uint32_t)k < n) contacts[(size_t)k * planeStride + g] = slot[k];
-    if (n > (uint32_t)CONTACT_SLOTS) {
-        // The contacts past the listed ones, found by a second walk, go to the
-        // pool, so the solve takes them without walking the grid again.
+    if (n > (uint32_t)CONTACT_SLOTS && n <= (uint32_t)CONTACT_PLANES) {
+        pool.base[g] = IN_PLANES;
+    } else if (n > (uint32_t)CONTACT_PLANES) {
+        // More than the planes hold: the contacts past the listed ones, found
+        // by a second walk, go to the pool, so the solve takes them without
+        // walking the grid again.
         const uint32_t need = n - (uint32_t)CONTACT_SLOTS, floorId = slot[CONTACT_SLOTS - 1];
         uint32_t b = atomicAdd(pool.top, need);
         if (b > pool.capacity || pool.capacity - b < need) {
@@ -565,17 +589,22 @@ __device__ __forceinline__ void next_contacts(const SimDesc& s, int cx, int cy, 
     });
 }
 
-// The same from a body's pool entries (its contacts past the listed ones, in
-// any order): the OVERFLOW_CHUNK smallest above floorId, ascending.
-__device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, uint32_t count,
-                                          uint32_t floorId, uint32_t (&next)[OVERFLOW_CHUNK]) {
+// The same from a body's stored contacts past the listed ones (in any
+// order, item e at items[e * stride]): the OVERFLOW_CHUNK smallest above
+// floorId, ascending. The loads go U at a time, unconditional (clamped).
+__device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, size_t stride,
+                                          uint32_t count, uint32_t floorId,
+                                          uint32_t (&next)[OVERFLOW_CHUNK]) {
 #pragma unroll
     for (int q = 0; q < OVERFLOW_CHUNK; q++) next[q] = 0xffffffffu;
-    constexpr uint32_t U = 4;
+    constexpr uint32_t U = 8;
     for (uint32_t e = 0; e < count; e += U) {
         uint32_t u[U];
 #pragma unroll
-        for (uint32_t q = 0; q < U; q++) u[q] = e + q < count ? items[e + q] : 0xffffffffu;
+        for (uint32_t q = 0; q < U; q++) u[q] = items[(size_t)min(e + q, count - 1u) * stride];
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++)
+            if (e + q >= count) u[q] = 0xffffffffu;
 #pragma unroll
         for (uint32_t q = 0; q < U; q++) {
             if (u[q] <= floorId || u[q] >= next[OVERFLOW_CHUNK - 1]) continue;
@@ -667,16 +696,20 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     if (own > (uint32_t)CONTACT_SLOTS) {
         // More contacts than slots: the rest in ascending index, OVERFLOW_CHUNK
         // at a time, each chunk the smallest indices above the last one taken,
-        // selected from the body's pool entries (or, when the pool was full,
-        // gathered by one walk of the grid).
+        // selected from the contact planes past the listed ones or the body's
+        // pool entries (or, when the pool was full, gathered by one walk of
+        // the grid).
         const bool ownFluid = k.liquid && ownMat.z == 0.0f;
         const int cx = cell_x(s, me.x), cy = cell_y(s, me.y);
         const uint32_t pb = pool.base[g];
         uint32_t rest = own - (uint32_t)CONTACT_SLOTS;
         while (rest > 0u) {
             uint32_t next[OVERFLOW_CHUNK];
-            if (pb != NO_POOL)
-                pool_next(pool.items + pb, own - (uint32_t)CONTACT_SLOTS, last, next);
+            if (pb == IN_PLANES)
+                pool_next(contacts + (size_t)CONTACT_SLOTS * planeStride + g, (size_t)planeStride,
+                          own - (uint32_t)CONTACT_SLOTS, last, next);
+            else if (pb != NO_POOL)
+                pool_next(pool.items + pb, 1, own - (uint32_t)CONTACT_SLOTS, last, next);
             else if (k.bounded)
                 next_contacts<true>(s, cx, cy, grid, guardLimit, g, ppx, ppy, last, ownFluid, mats, next);
             else

@@ -21,7 +21,9 @@
 namespace gs {
 
 constexpr int WG = 256;               // workgroup width of every body kernel
-constexpr int CONTACT_SLOTS = 16;     // contacts listed per body; more walk the grid
+constexpr int CONTACT_SLOTS = 16;     // contacts listed per body in ascending order
+constexpr int CONTACT_PLANES = 64;    // planes of the contact array: the listed contacts,
+                                      // then the rest of a crowded body's, in any order
 constexpr int XCDS = 8;               // accelerator dies of an MI355X, each with its own L2
 constexpr float MAXIMUM_UNIT = 0.99999994f;  // largest float below 1
 
@@ -198,6 +200,26 @@ __device__ __forceinline__ LaneRun lane_run(uint32_t key) {
     const unsigned long long above = starts & ~upTo;
     const int next = above ? __ffsll((long long)above) - 1 : 64;
     return LaneRun{start, next - start, lane - start};
+}
+
+// The rank of v in the segment list[s0, s1) of distinct values that holds
+// it: the number of entries above v (ABOVE) or below it. The loads go
+// SEGMENT_BATCH at a time, clamped to the segment, so a long segment (a
+// crowded cell) costs one memory round trip per batch, not per entry.
+constexpr int SEGMENT_BATCH = 16;
+template <bool ABOVE>
+__device__ __forceinline__ int segment_rank(const uint32_t* __restrict__ list, int s0, int s1,
+                                            uint32_t v) {
+    int rank = 0;
+    for (int e = s0; e < s1; e += SEGMENT_BATCH) {
+        uint32_t u[SEGMENT_BATCH];
+#pragma unroll
+        for (int q = 0; q < SEGMENT_BATCH; q++) u[q] = list[min(e + q, s1 - 1)];
+#pragma unroll
+        for (int q = 0; q < SEGMENT_BATCH; q++)
+            rank += e + q < s1 && (ABOVE ? u[q] > v : u[q] < v) ? 1 : 0;
+    }
+    return rank;
 }
 
 // Every launch goes through launch(). While a profiling sink is set (the

@@ -146,6 +146,9 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
         mesh += h.meshMax;
     }
     totalSlots_ = std::max(slot, WG);
+    // contact planes are indexed in 32 bits
+    if ((uint64_t)totalSlots_ * CONTACT_PLANES > 0xffffffffull)
+        throw ArgumentError("too many bodies for the contact planes (67,108,864 at most)");
     totalCells_ = cells;
     totalTiles_ = tiles;
     totalPhys_ = phys;
@@ -168,7 +171,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     }
     press_ = dalloc<float>(S);
     counts_ = dalloc<uint32_t>(S);
-    contacts_ = dalloc<uint32_t>(S * CONTACT_SLOTS);
+    contacts_ = dalloc<uint32_t>(S * CONTACT_PLANES);
     poolItems_ = dalloc<uint32_t>(S * CONTACT_SLOTS);
     poolBase_ = dalloc<uint32_t>(S);
     poolTop_ = dalloc<uint32_t>(1);

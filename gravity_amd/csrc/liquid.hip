@@ -181,9 +181,7 @@ __global__ void __launch_bounds__(WG) k_liq_canon(int slots, const uint32_t* __r
     uint32_t g = unordered[d];
     uint32_t c = cellOf[g];
     int s0 = (int)prefix[c], s1 = (int)prefix[c + 1];
-    int rank = 0;
-    for (int e = s0; e < s1; e++) rank += unordered[e] < g ? 1 : 0;
-    items[s0 + rank] = g;
+    items[s0 + segment_rank<false>(unordered, s0, s1, g)] = g;
 }
 
 // ---------------------------------------------------------------------------

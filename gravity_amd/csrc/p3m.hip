@@ -81,8 +81,7 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
         uint32_t g = slotBody[d];
         c = homeOf[g];
         int s0 = (int)cellPrefix[c], s1 = (int)cellPrefix[c + 1];
-        int rank = 0;
-        for (int e = s0; e < s1; e++) rank += slotBody[e] > g ? 1 : 0;
+        const int rank = segment_rank<true>(slotBody, s0, s1, g);
         float4 p = body[g];
         float m = mats[g].x;
         cellBody[s0 + rank] = make_float4(p.x, p.y, m, __uint_as_float(g));

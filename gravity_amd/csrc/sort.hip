@@ -186,9 +186,7 @@ __global__ void __launch_bounds__(WG) k_sort_canon(
     uint32_t t = keyOf[src];
     int s0 = s.base + (int)(tilePrefix[t] - (uint32_t)s.rank0);
     int s1 = s.base + (int)(tilePrefix[t + 1] - (uint32_t)s.rank0);
-    int rank = 0;
-    for (int e = s0; e < s1; e++) rank += slotSrc[e] < src ? 1 : 0;
-    int fin = s0 + rank;
+    int fin = s0 + segment_rank<false>(slotSrc, s0, s1, src);
     bodyOut[fin] = bodyIn[src];
     accOut[fin] = accIn[src];
     matOut[fin] = matIn[src];
