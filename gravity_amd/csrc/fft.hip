@@ -184,13 +184,28 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) mass[i] = 0u;
     __syncthreads();
     const float scale = s.massScale;
+    const int first = w.first, count = w.count;
     // A body of physical row r deposits into mesh rows r - 1 .. r + 1 (under
     // walls a body of row 0 below the edge reaches the last mesh row, the
     // item's line ph + 1). The physical rows that reach the item's lines are
-    // at most two ranges, and each of their bodies is visited once.
-    const int first = w.first, count = w.count;
+    // at most two ranges: rows [ra[q], rb[q]].
     int ra[2] = {0, 0}, rb[2] = {-1, -1}, nr = 0;
-    if (!s.bounded) {
+    if (count == 1) {
+        const int y = mesh_row(s, first);
+        if (s.bounded) {
+            if (y == s.mh - 1) { ra[0] = 0; rb[0] = 0; }
+            else { ra[0] = max(y - 1, 0); rb[0] = min(y + 1, s.ph - 1); }
+            nr = 1;
+        } else if (s.ph < 3) {
+            ra[0] = 0; rb[0] = s.ph - 1; nr = 1;
+        } else if (y == 0) {
+            ra[0] = s.ph - 1; rb[0] = s.ph - 1; ra[1] = 0; rb[1] = 1; nr = 2;
+        } else if (y == s.ph - 1) {
+            ra[0] = y - 1; rb[0] = s.ph - 1; ra[1] = 0; rb[1] = 0; nr = 2;
+        } else {
+            ra[0] = y - 1; rb[0] = y + 1; nr = 1;
+        }
+    } else if (!s.bounded) {
         if (count + 2 >= s.ph) {
             ra[0] = 0; rb[0] = s.ph - 1; nr = 1;
         } else if (first == 0) {
@@ -210,40 +225,59 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
             ra[nr] = 0; rb[nr] = 0; nr++;
         }
     }
+    // An item of one line (every row of a 2048-wide mesh) keeps the corners
+    // in its mesh row; an item of several lines routes each corner to the
+    // line of its mesh row, so each body of the rows is visited once.
+    const int line0 = mesh_row(s, first);
     for (int q = 0; q < nr; q++) {
-        const int t0 = (int)cellPrefix[s.physBase + ra[q] * s.pw];
-        const int t1 = (int)cellPrefix[s.physBase + (rb[q] + 1) * s.pw];
-        // Four bodies a lane per round, their loads issued together and
-        // unconditional (clamped to the range).
-        constexpr int UNROLL = 4;
-        for (int t = t0 + (int)threadIdx.x; t < t1; t += UNROLL * ROW_NT) {
-            float4 bds[UNROLL];
+        for (int r = ra[q]; r <= rb[q]; r++) {
+            const int t0 = (int)cellPrefix[s.physBase + r * s.pw];
+            const int t1 = (int)cellPrefix[s.physBase + (r + 1) * s.pw];
+            // Four bodies a lane per round, their loads issued together and
+            // unconditional (clamped to the row).
+            constexpr int UNROLL = 4;
+            for (int t = t0 + (int)threadIdx.x; t < t1; t += UNROLL * ROW_NT) {
+                float4 bds[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; u++) bds[u] = cellBody[min(t + u * ROW_NT, t1 - 1)];
+                for (int u = 0; u < UNROLL; u++) bds[u] = cellBody[min(t + u * ROW_NT, t1 - 1)];
 #pragma unroll
-            for (int u = 0; u < UNROLL; u++) {
-                if (t + u * ROW_NT >= t1) continue;
-                float4 bd = bds[u];
-                // S/build_grid.comp:53-69, round() half to even.
-                float gx = bd.x * (float)s.pw - 0.5f, gy = bd.y * (float)s.ph - 0.5f;
-                float flx = floorf(gx), fly = floorf(gy);
-                int bx = (int)flx, by = (int)fly;
-                float fx = gx - flx, fy = gy - fly;
-                float wgt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy),
-                                (1.0f - fx) * fy, fx * fy};
+                for (int u = 0; u < UNROLL; u++) {
+                    if (t + u * ROW_NT >= t1) continue;
+                    float4 bd = bds[u];
+                    // S/build_grid.comp:53-69, round() half to even.
+                    float gx = bd.x * (float)s.pw - 0.5f, gy = bd.y * (float)s.ph - 0.5f;
+                    float flx = floorf(gx), fly = floorf(gy);
+                    int bx = (int)flx, by = (int)fly;
+                    float fx = gx - flx, fy = gy - fly;
+                    float wgt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy),
+                                    (1.0f - fx) * fy, fx * fy};
+                    if (count == 1) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    int my = wrap_axis(by + (k >> 1), s.mh);
-                    int mx = wrap_axis(bx + (k & 1), s.mw);
-                    float wk = wgt[k] * bd.z;
-                    uint32_t qv = (uint32_t)rintf(wk * scale);
-                    // the item's line of mesh row my, and under walls the
-                    // last line for the last mesh row
-                    int l1 = !s.bounded || my <= s.ph ? my - first : -1;
-                    if ((unsigned)l1 < (unsigned)count) atomicAdd(&mass[l1 * L + mx], qv);
-                    if (s.bounded && my == s.mh - 1) {
-                        int l2 = s.ph + 1 - first;
-                        if ((unsigned)l2 < (unsigned)count) atomicAdd(&mass[l2 * L + mx], qv);
+                        for (int k = 0; k < 4; k++) {
+                            int my = wrap_axis(by + (k >> 1), s.mh);
+                            if (my != line0) continue;
+                            int mx = wrap_axis(bx + (k & 1), s.mw);
+                            float wk = wgt[k] * bd.z;
+                            uint32_t qv = (uint32_t)rintf(wk * scale);
+                            atomicAdd(&mass[mx], qv);
+                        }
+                        continue;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int my = wrap_axis(by + (k >> 1), s.mh);
+                        const int mx = wrap_axis(bx + (k & 1), s.mw);
+                        // the item's line of mesh row my, and under walls the
+                        // last line for the last mesh row
+                        const int l1 = !s.bounded || my <= s.ph ? my - first : -1;
+                        const int l2 = s.bounded && my == s.mh - 1 ? s.ph + 1 - first : -1;
+                        const bool in1 = (unsigned)l1 < (unsigned)count;
+                        const bool in2 = (unsigned)l2 < (unsigned)count;
+                        if (!in1 && !in2) continue;
+                        float wk = wgt[k] * bd.z;
+                        uint32_t qv = (uint32_t)rintf(wk * scale);
+                        if (in1) atomicAdd(&mass[l1 * L + mx], qv);
+                        if (in2) atomicAdd(&mass[l2 * L + mx], qv);
                     }
                 }
             }
