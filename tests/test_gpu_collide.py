@@ -63,6 +63,32 @@ def test_replay_matches_oracle_and_brute_force(name, oracle):
             state = got
 
 
+@pytest.mark.parametrize("count", [64, 65, 66, 90])
+def test_crowds_at_the_contact_plane_limit_match_oracle(count, oracle):
+    # Crowds whose bodies touch all others: 63 and 64 contacts fit the
+    # contact planes (sixteen listed, the rest past them), 65 and 89 go to the
+    # contact pool; the solve takes every contact in ascending order either way.
+    rng = np.random.default_rng(count)
+    rows = [(900 + 0.3 * rng.random(), 500 + 0.3 * rng.random(), rng.normal(0, 20), rng.normal(0, 20))
+            for _ in range(count)]
+    state = scene(rows)
+    p = params(oracle)
+    zero = np.zeros((count, 2), np.float32)
+    with GravityPipeline(config(count), WIDTH, HEIGHT) as pipe:
+        for k in range(3):
+            pipe.upload(state)
+            pipe.resolveCollisions(count, 0.0, 0.0, True, None)
+            counts = pipe.contactCounts(count)
+            if k == 0:
+                np.testing.assert_array_equal(counts, np.full(count, count - 1))
+            np.testing.assert_array_equal(counts, [len(e) for e in brute_force_contacts(state)])
+            got = pipe.bodies(count)
+            ob, opr, oc = oracle.collide_pass(p, state, zero, materials(count), 0.0, 0.0, True)
+            np.testing.assert_array_equal(oc, counts)
+            np.testing.assert_array_equal(got.view(np.uint32), ob.view(np.uint32), err_msg=f"{count}/{k}")
+            state = got
+
+
 def test_dense_lattice_counts_are_four_neighbour_counts():
     # T/CollisionShaderIntegrationTest.kt:166-171
     count = 4096
