@@ -153,11 +153,21 @@ __device__ __forceinline__ void lds_fft(float2* buf, int L, int nl, int ls, int 
 
 // The twiddle table (at most TW_LDS entries) staged in LDS with the first
 // loads of a kernel: the stages then read it without a memory round trip.
+// A lane's TW_LDS / NT loads are issued together (clamped to the table), then
+// stored: one round trip, where a load-store loop waits for every load.
 constexpr int TW_LDS = 2048;
 template <int NT>
 __device__ __forceinline__ void stage_twiddles(float2* twl, const float2* __restrict__ tw,
                                                int twLen) {
-    for (int i = (int)threadIdx.x; i < twLen; i += NT) twl[i] = tw[i];
+    constexpr int PER = TW_LDS / NT;
+    float2 v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) v[q] = tw[min((int)threadIdx.x + q * NT, twLen - 1)];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int i = (int)threadIdx.x + q * NT;
+        if (i < twLen) twl[i] = v[q];
+    }
 }
 
 // Mesh row of forward/inverse work row a: all rows on the torus; under walls
@@ -229,13 +239,14 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     // in its mesh row; an item of several lines routes each corner to the
     // line of its mesh row, so each body of the rows is visited once.
     const int line0 = mesh_row(s, first);
+    // The rows of a range are contiguous in the cell segments.
     for (int q = 0; q < nr; q++) {
-        for (int r = ra[q]; r <= rb[q]; r++) {
-            const int t0 = (int)cellPrefix[s.physBase + r * s.pw];
-            const int t1 = (int)cellPrefix[s.physBase + (r + 1) * s.pw];
-            // Four bodies a lane per round, their loads issued together and
-            // unconditional (clamped to the row).
-            constexpr int UNROLL = 4;
+        {
+            const int t0 = (int)cellPrefix[s.physBase + ra[q] * s.pw];
+            const int t1 = (int)cellPrefix[s.physBase + (rb[q] + 1) * s.pw];
+            // Eight bodies a lane per round, their loads issued together and
+            // unconditional (clamped to the range).
+            constexpr int UNROLL = 8;
             for (int t = t0 + (int)threadIdx.x; t < t1; t += UNROLL * ROW_NT) {
                 float4 bds[UNROLL];
 #pragma unroll
@@ -307,14 +318,30 @@ __global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restric
     const int H = s.mh, CB = w.count, c0 = w.first;
     stage_twiddles<COL_NT>(twl, tw, twLen);
     const float2* src = spec + s.meshBase;
-    for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
-        int y = i / CB, c = i - y * CB;
-        bool live = !s.bounded || y <= s.ph || y == H - 1;
-        data[i] = live ? src[y * s.mw + c0 + c] : make_float2(0.0f, 0.0f);
+    // A lane's COL_LDS / COL_NT values load together (a dead row or a lane
+    // past the item reads its column's row 0), then go to LDS.
+    constexpr int PER = COL_LDS / COL_NT;
+    {
+        float2 v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int i = min((int)threadIdx.x + q * COL_NT, CB * H - 1);
+            const int y = i / CB, c = i - y * CB;
+            const bool live = !s.bounded || y <= s.ph || y == H - 1;
+            v[q] = src[(live ? y * s.mw : 0) + c0 + c];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int i = (int)threadIdx.x + q * COL_NT;
+            if (i < CB * H) {
+                const int y = i / CB;
+                const bool live = !s.bounded || y <= s.ph || y == H - 1;
+                data[i] = live ? v[q] : make_float2(0.0f, 0.0f);
+            }
+        }
     }
     // The spectrum values of this workgroup's nodes load now and arrive
     // during the forward transform (COL_LDS / COL_NT of them per lane).
-    constexpr int PER = COL_LDS / COL_NT;
     const float2* G = green + s.meshBase;
     float2 gv[PER];
 #pragma unroll
@@ -355,9 +382,22 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_inverse(const SimDesc* __re
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
     stage_twiddles<ROW_NT>(twl, tw, twLen);
-    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
-        int l = i / L, x = i - l * L;
-        data[i] = spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x];
+    {
+        // A lane's ROW_LDS / ROW_NT values load together, then go to LDS.
+        constexpr int PER = ROW_LDS / ROW_NT;
+        const int n = w.count * L;
+        float2 v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int i = min((int)threadIdx.x + q * ROW_NT, n - 1);
+            const int l = i / L, x = i - l * L;
+            v[q] = spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int i = (int)threadIdx.x + q * ROW_NT;
+            if (i < n) data[i] = v[q];
+        }
     }
     __syncthreads();
     lds_fft<1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, twl, twLen);
