@@ -77,20 +77,37 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int
                                                  const uint32_t* __restrict__ in,
                                                  uint4* __restrict__ rec,
                                                  uint4* __restrict__ near) {
+    // This lane's elements (SCAN_BLOCK / WG of them) and the totals of the
+    // blocks before this one (four a lane per round) load unconditionally,
+    // clamped, so they are in flight together.
+    const int nb = (int)blockIdx.x;
+    const int base = nb * SCAN_BLOCK + (int)threadIdx.x;
+    constexpr int PER = SCAN_BLOCK / WG;
+    uint32_t o[PER], c[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        o[q] = out[min(base + q * WG, len)];
+        c[q] = in[min(base + q * WG, max(len - 1, 0))];
+    }
     uint32_t part = 0u;
-    for (int i = (int)threadIdx.x; i < (int)blockIdx.x; i += WG) part += totals[i];
+    for (int i = (int)threadIdx.x; i < nb; i += 4 * WG) {
+        uint32_t t[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) t[q] = totals[min(i + q * WG, nb - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; q++) part += i + q * WG < nb ? t[q] : 0u;
+    }
     uint32_t add;
     block_exclusive(part, &add);
-    int base = (int)blockIdx.x * SCAN_BLOCK;
-    for (int i = (int)threadIdx.x; i < SCAN_BLOCK; i += WG) {
-        int idx = base + i;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int idx = base + q * WG;
         if (idx <= len) {
-            uint32_t v = out[idx] + add;
+            uint32_t v = o[q] + add;
             out[idx] = v;
             if (rec != nullptr && idx < len) {
-                uint32_t count = in[idx];
-                rec[idx] = make_uint4(v, count, 0u, 0u);
-                near[2 * (size_t)idx] = make_uint4(count, 0u, 0u, 0u);
+                rec[idx] = make_uint4(v, c[q], 0u, 0u);
+                near[2 * (size_t)idx] = make_uint4(c[q], 0u, 0u, 0u);
             }
         }
     }
