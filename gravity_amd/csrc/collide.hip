@@ -6,11 +6,12 @@
 //   count : clears the previous pass's grid cells, walks the 3x3 stencil of
 //           this pass's grid, writes the contact count and the ascending list
 //           of the CONTACT_SLOTS smallest contact indices (kept sorted in
-//           registers while walking; the indices they push out go to the
-//           planes past them);
+//           registers while walking; the indices they push out go to
+//           overflow planes, which a crowded body then orders into the
+//           listed planes, up to CONTACT_SORTED contacts);
 //   solve : Jacobi correction in ascending contact order (a body with more
-//           contacts takes the rest six at a time, in order, from the contact
-//           planes past the listed ones or the contact pool), walls,
+//           contacts than the contact array lists takes the rest six at a
+//           time, in order, from the contact pool), walls,
 //           integration, then, fused, the next pass's drift and its insertion
 //           into the other grid buffer.
 // The broad phase is a linked-cell grid of cells at least diameter + guard
@@ -297,27 +298,58 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, cons
         // Insertion keeping the CONTACT_SLOTS smallest indices, ascending:
         // with s sorted, the new slot k is the median of s[k-1], u, s[k]
         // (one v_med3_u32 each, all independent of one another). The index
-        // it pushes out of the kept ones goes to the next free plane past
-        // them, while there is one (plane indices fit 32 bits: the engine
-        // caps the slots at 2^32 / CONTACT_PLANES).
+        // it pushes out of the kept ones goes to the next free overflow
+        // plane (planes CONTACT_SORTED..), while there is one (plane indices
+        // fit 32 bits: the engine caps the slots at 2^32 / CONTACT_PLANES).
         uint32_t u = (uint32_t)other;
         const uint32_t out = max(u, slot[CONTACT_SLOTS - 1]);
 #pragma unroll
         for (int k = CONTACT_SLOTS - 1; k > 0; k--) slot[k] = med3u(slot[k - 1], u, slot[k]);
         slot[0] = min(u, slot[0]);
-        if (n > (uint32_t)CONTACT_SLOTS && n <= (uint32_t)CONTACT_PLANES)
-            contacts[(n - 1u) * (uint32_t)planeStride + (uint32_t)g] = out;
+        if (n > (uint32_t)CONTACT_SLOTS && n <= (uint32_t)CONTACT_SORTED)
+            contacts[(n - 1u + (uint32_t)(CONTACT_SORTED - CONTACT_SLOTS)) * (uint32_t)planeStride +
+                     (uint32_t)g] = out;
     });
     counts[g] = n;
 #pragma unroll
     for (int k = 0; k < CONTACT_SLOTS; k++)
         if ((uint32_t)k < n) contacts[(size_t)k * planeStride + g] = slot[k];
-    if (n > (uint32_t)CONTACT_SLOTS && n <= (uint32_t)CONTACT_PLANES) {
-        pool.base[g] = IN_PLANES;
-    } else if (n > (uint32_t)CONTACT_PLANES) {
-        // More than the planes hold: the contacts past the listed ones, found
-        // by a second walk, go to the pool, so the solve takes them without
-        // walking the grid again.
+    if (n > (uint32_t)CONTACT_SLOTS && n <= (uint32_t)CONTACT_SORTED) {
+        // The overflow planes, ordered into the listed planes CONTACT_SLOTS
+        // at a time: each round keeps the smallest above the last one listed,
+        // from loads issued CONTACT_SLOTS at a time.
+        const uint32_t m = n - (uint32_t)CONTACT_SLOTS;
+        const uint32_t* over = contacts + (size_t)CONTACT_SORTED * planeStride + g;
+        uint32_t floorId = slot[CONTACT_SLOTS - 1];
+        for (uint32_t c0 = 0; c0 < m; c0 += CONTACT_SLOTS) {
+#pragma unroll
+            for (int k = 0; k < CONTACT_SLOTS; k++) slot[k] = 0xffffffffu;
+            for (uint32_t e = 0; e < m; e += CONTACT_SLOTS) {
+                uint32_t u[CONTACT_SLOTS];
+#pragma unroll
+                for (int q = 0; q < CONTACT_SLOTS; q++)
+                    u[q] = over[(size_t)min(e + (uint32_t)q, m - 1u) * planeStride];
+#pragma unroll
+                for (int q = 0; q < CONTACT_SLOTS; q++) {
+                    if (e + (uint32_t)q >= m || u[q] <= floorId) continue;
+#pragma unroll
+                    for (int k = CONTACT_SLOTS - 1; k > 0; k--) slot[k] = med3u(slot[k - 1], u[q], slot[k]);
+                    slot[0] = min(u[q], slot[0]);
+                }
+            }
+            const uint32_t take = min(m - c0, (uint32_t)CONTACT_SLOTS);
+#pragma unroll
+            for (int k = 0; k < CONTACT_SLOTS; k++)
+                if ((uint32_t)k < take)
+                    contacts[(size_t)(CONTACT_SLOTS + c0 + (uint32_t)k) * planeStride + g] = slot[k];
+#pragma unroll
+            for (int k = 0; k < CONTACT_SLOTS; k++)
+                if ((uint32_t)k + 1u == take) floorId = slot[k];
+        }
+    } else if (n > (uint32_t)CONTACT_SORTED) {
+        // More than the planes list: the contacts past the CONTACT_SLOTS
+        // smallest, found by a second walk, go to the pool, so the solve
+        // takes them without walking the grid again.
         const uint32_t need = n - (uint32_t)CONTACT_SLOTS, floorId = slot[CONTACT_SLOTS - 1];
         uint32_t b = atomicAdd(pool.top, need);
         if (b > pool.capacity || pool.capacity - b < need) {
@@ -589,19 +621,18 @@ __device__ __forceinline__ void next_contacts(const SimDesc& s, int cx, int cy, 
     });
 }
 
-// The same from a body's stored contacts past the listed ones (in any
-// order, item e at items[e * stride]): the OVERFLOW_CHUNK smallest above
-// floorId, ascending. The loads go U at a time, unconditional (clamped).
-__device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, size_t stride,
-                                          uint32_t count, uint32_t floorId,
-                                          uint32_t (&next)[OVERFLOW_CHUNK]) {
+// The same from a body's pool entries (its contacts past the CONTACT_SLOTS
+// smallest, in any order): the OVERFLOW_CHUNK smallest above floorId,
+// ascending. The loads go U at a time, unconditional (clamped).
+__device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, uint32_t count,
+                                          uint32_t floorId, uint32_t (&next)[OVERFLOW_CHUNK]) {
 #pragma unroll
     for (int q = 0; q < OVERFLOW_CHUNK; q++) next[q] = 0xffffffffu;
     constexpr uint32_t U = 8;
     for (uint32_t e = 0; e < count; e += U) {
         uint32_t u[U];
 #pragma unroll
-        for (uint32_t q = 0; q < U; q++) u[q] = items[(size_t)min(e + q, count - 1u) * stride];
+        for (uint32_t q = 0; q < U; q++) u[q] = items[min(e + q, count - 1u)];
 #pragma unroll
         for (uint32_t q = 0; q < U; q++)
             if (e + q >= count) u[q] = 0xffffffffu;
@@ -663,7 +694,9 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     float vpx = me.z * k.vw, vpy = me.w * k.vh;
     Accum acc{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t own = counts[g];
-    int listed = own < (uint32_t)CONTACT_SLOTS ? (int)own : CONTACT_SLOTS;
+    // Up to CONTACT_SORTED contacts are listed in ascending order; a body
+    // with more lists its CONTACT_SLOTS smallest and takes the rest below.
+    int listed = own <= (uint32_t)CONTACT_SORTED ? (int)own : CONTACT_SLOTS;
     uint32_t last = 0u;
     const float2* __restrict__ matXY = reinterpret_cast<const float2*>(mats);
 #pragma unroll 1
@@ -693,23 +726,19 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
             }
         }
     }
-    if (own > (uint32_t)CONTACT_SLOTS) {
-        // More contacts than slots: the rest in ascending index, OVERFLOW_CHUNK
-        // at a time, each chunk the smallest indices above the last one taken,
-        // selected from the contact planes past the listed ones or the body's
-        // pool entries (or, when the pool was full, gathered by one walk of
-        // the grid).
+    if (own > (uint32_t)CONTACT_SORTED) {
+        // More contacts than listed: the rest in ascending index,
+        // OVERFLOW_CHUNK at a time, each chunk the smallest indices above the
+        // last one taken, selected from the body's pool entries (or, when the
+        // pool was full, gathered by one walk of the grid).
         const bool ownFluid = k.liquid && ownMat.z == 0.0f;
         const int cx = cell_x(s, me.x), cy = cell_y(s, me.y);
         const uint32_t pb = pool.base[g];
         uint32_t rest = own - (uint32_t)CONTACT_SLOTS;
         while (rest > 0u) {
             uint32_t next[OVERFLOW_CHUNK];
-            if (pb == IN_PLANES)
-                pool_next(contacts + (size_t)CONTACT_SLOTS * planeStride + g, (size_t)planeStride,
-                          own - (uint32_t)CONTACT_SLOTS, last, next);
-            else if (pb != NO_POOL)
-                pool_next(pool.items + pb, 1, own - (uint32_t)CONTACT_SLOTS, last, next);
+            if (pb != NO_POOL)
+                pool_next(pool.items + pb, own - (uint32_t)CONTACT_SLOTS, last, next);
             else if (k.bounded)
                 next_contacts<true>(s, cx, cy, grid, guardLimit, g, ppx, ppy, last, ownFluid, mats, next);
             else

@@ -21,9 +21,10 @@
 namespace gs {
 
 constexpr int WG = 256;               // workgroup width of every body kernel
-constexpr int CONTACT_SLOTS = 16;     // contacts listed per body in ascending order
-constexpr int CONTACT_PLANES = 64;    // planes of the contact array: the listed contacts,
-                                      // then the rest of a crowded body's, in any order
+constexpr int CONTACT_SLOTS = 16;     // contacts a count keeps in registers, ascending
+constexpr int CONTACT_SORTED = 64;    // contacts a body lists in the contact array, ascending
+constexpr int CONTACT_PLANES = 112;   // planes of the contact array: the listed contacts,
+                                      // then a crowded body's unsorted overflow
 constexpr int XCDS = 8;               // accelerator dies of an MI355X, each with its own L2
 constexpr float MAXIMUM_UNIT = 0.99999994f;  // largest float below 1
 

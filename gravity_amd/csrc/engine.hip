@@ -148,7 +148,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     totalSlots_ = std::max(slot, WG);
     // contact planes are indexed in 32 bits
     if ((uint64_t)totalSlots_ * CONTACT_PLANES > 0xffffffffull)
-        throw ArgumentError("too many bodies for the contact planes (67,108,864 at most)");
+        throw ArgumentError("too many bodies for the contact planes (38,347,922 at most)");
     totalCells_ = cells;
     totalTiles_ = tiles;
     totalPhys_ = phys;

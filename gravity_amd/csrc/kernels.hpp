@@ -12,15 +12,13 @@ struct CollideGrid {
     int* cellOf;       // per body: the cell of its insertion
 };
 
-// Contacts past the listed ones, for bodies with more than CONTACT_SLOTS:
-// up to CONTACT_PLANES contacts, the rest stay in the contact array's planes
-// CONTACT_SLOTS.. (base[g] = IN_PLANES); beyond that the count appends the
-// body's remaining contacts (in any order) to
-// items[base[g], base[g] + count - CONTACT_SLOTS), from a counter `top` that
-// the solve returns to zero; a body that finds the pool full gets NO_POOL and
-// the solve walks the grid for its remaining contacts instead.
+// Contacts past the listed ones, for bodies with more than CONTACT_SORTED:
+// the count appends each such body's contacts past its CONTACT_SLOTS
+// smallest (in any order) to items[base[g], base[g] + count - CONTACT_SLOTS),
+// from a counter `top` that the solve returns to zero; a body that finds the
+// pool full gets NO_POOL and the solve walks the grid for its remaining
+// contacts instead.
 constexpr uint32_t NO_POOL = 0xffffffffu;
-constexpr uint32_t IN_PLANES = 0xfffffffeu;
 struct ContactPool {
     uint32_t* items;
     uint32_t* base;
