@@ -290,8 +290,21 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, cons
     for (int k = 0; k < CONTACT_SLOTS; k++) slot[k] = 0xffffffffu;
     uint32_t n = 0;
     const int cx = (int)(cell & 0xffffu), cy = (int)(cell >> 16);
+    // On the torus, a stencil that does not wrap on a grid of at least 8
+    // cells a side keeps every candidate within about a quarter of the
+    // viewport, where floor(d / w + 0.5) is 0: the minimum image is the
+    // difference itself, and either order gives the same square
+    // (fl(a - b) = -fl(b - a)), so the divisions are not needed.
+    const bool interior = BOUNDED || (cx >= 1 && cx <= s.cgw - 2 && cy >= 1 && cy <= s.cgh - 2 &&
+                                      s.cgw >= 8 && s.cgh >= 8);
     walk_stencil<BOUNDED>(s, cx, cy, grid, guardLimit, [&](int other, float ox, float oy) {
-        float d2 = pair_d2<BOUNDED>(g, px, py, other, ox, oy, vw, vh);
+        float d2;
+        if (interior) {
+            const float dx = px - ox, dy = py - oy;
+            d2 = dx * dx + dy * dy;
+        } else {
+            d2 = pair_d2<BOUNDED>(g, px, py, other, ox, oy, vw, vh);
+        }
         if (!(d2 < D2) || other == g) return;
         if (ownFluid && mats[other].z == 0.0f) return;
         n++;
@@ -527,7 +540,9 @@ __device__ __forceinline__ void accumulate_contact(const SolveK& s, const float*
     if (s.liquid && !(ownMat.z != 0.0f || om.z != 0.0f)) return;
     float dx = ppx - ob.x * s.vw;
     float dy = ppy - ob.y * s.vh;
-    if (!s.bounded) {
+    // On the torus, a difference under a quarter of the viewport is its own
+    // minimum image (floor(d / w + 0.5) is 0 there), with no division.
+    if (!s.bounded && !(fabsf(dx) < 0.25f * s.vw && fabsf(dy) < 0.25f * s.vh)) {
         dx -= floorf(dx / s.vw + 0.5f) * s.vw;
         dy -= floorf(dy / s.vh + 0.5f) * s.vh;
     }

@@ -72,7 +72,9 @@ __device__ __forceinline__ void liquid_delta(const SimDesc& s, float ax, float a
                                              float by, float& dx, float& dy) {
     dx = (ax - bx) * s.vw;
     dy = (ay - by) * s.vh;
-    if (!s.bounded) {
+    // a difference under a quarter of the viewport is its own minimum image
+    // (floor(d / w + 0.5) is 0 there), with no division
+    if (!s.bounded && !(fabsf(dx) < 0.25f * s.vw && fabsf(dy) < 0.25f * s.vh)) {
         dx -= floorf(dx / s.vw + 0.5f) * s.vw;
         dy -= floorf(dy / s.vh + 0.5f) * s.vh;
     }
