@@ -24,12 +24,19 @@ __device__ __forceinline__ int liquid_cell(const SimDesc& s, float x, float y) {
            floor_index(y * (float)s.lgh, s.lgh) * s.lgw;
 }
 
-// Every other body of the 3x3 start cells, in the canonical order. On a grid
-// narrower than three cells a wrapped row or column met twice is taken once.
-template <typename F>
+// Every other body of the 3x3 start cells, in the canonical order, with the
+// data `load` fetches for it. On a grid narrower than three cells a wrapped
+// row or column met twice is taken once. The nine cells' lists are taken as
+// one sequence, LIQ_BATCH entries at a time: a batch's list entries load
+// together, then the neighbours' data together (each clamped to a valid
+// entry), so a batch costs two memory round trips where an entry at a time
+// cost two per neighbour; the visits keep the canonical order.
+constexpr int LIQ_BATCH = 8;
+template <typename Load, typename Visit>
 __device__ __forceinline__ void liquid_neighbours(const SimDesc& s, int g, int home,
                                                   const uint32_t* __restrict__ prefix,
-                                                  const uint32_t* __restrict__ items, F&& visit) {
+                                                  const uint32_t* __restrict__ items, Load&& load,
+                                                  Visit&& visit) {
     int local = home - s.liqCellBase;
     int hx = local % s.lgw, hy = local / s.lgw;
     int ys[3], xs[3];
@@ -49,20 +56,39 @@ __device__ __forceinline__ void liquid_neighbours(const SimDesc& s, int g, int h
         ys[j] = y;
         xs[j] = x;
     }
+    // Cell k's list starts at first[k] and covers entries [cum[k], cum[k+1])
+    // of the sequence; a cell that does not take part reads the home cell's
+    // bounds and counts as empty.
+    uint32_t first[9], cum[10];
+    cum[0] = 0u;
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        if (!oky[j]) continue;
-        const int row = s.liqCellBase + ys[j] * s.lgw;
+    for (int k = 0; k < 9; k++) {
+        const int j = k / 3, i = k % 3;
+        const bool ok = oky[j] && okx[i];
+        const int c = ok ? s.liqCellBase + ys[j] * s.lgw + xs[i] : home;
+        const uint32_t t0 = prefix[c], t1 = prefix[c + 1];
+        first[k] = t0;
+        cum[k + 1] = cum[k] + (ok ? t1 - t0 : 0u);
+    }
+    const uint32_t total = cum[9];
+    using Data = decltype(load(0));
+    for (uint32_t k0 = 0; k0 < total; k0 += LIQ_BATCH) {
+        int id[LIQ_BATCH];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            if (!okx[i]) continue;
-            int c = row + xs[i];
-            uint32_t t0 = prefix[c], t1 = prefix[c + 1];
-            for (uint32_t t = t0; t < t1; t++) {
-                int other = (int)items[t];
-                if (other != g) visit(other);
-            }
+        for (int q = 0; q < LIQ_BATCH; q++) {
+            const uint32_t e = min(k0 + (uint32_t)q, total - 1u);
+            uint32_t pos = first[0] + e;
+#pragma unroll
+            for (int k = 1; k < 9; k++)
+                if (e >= cum[k]) pos = first[k] + (e - cum[k]);
+            id[q] = (int)items[pos];
         }
+        Data d[LIQ_BATCH];
+#pragma unroll
+        for (int q = 0; q < LIQ_BATCH; q++) d[q] = load(id[q]);
+#pragma unroll
+        for (int q = 0; q < LIQ_BATCH; q++)
+            if (k0 + (uint32_t)q < total && id[q] != g) visit(id[q], d[q]);
     }
 }
 
@@ -212,8 +238,12 @@ __global__ void __launch_bounds__(WG) k_liq_lambda(const SimDesc* __restrict__ s
         return;
     }
     float density = 0.0f, ogx = 0.0f, ogy = 0.0f, others = 0.0f;
-    liquid_neighbours(s, g, (int)cellOf[g], prefix, items, [&](int j) {
-        float4 o = body[j];
+    liquid_neighbours(s, g, (int)cellOf[g], prefix, items,
+                      [&](int j) {
+                          const float4 o = body[j];
+                          return make_float3(o.x, o.y, mats[j].z);
+                      },
+                      [&](int j, float3 o) {
         float dx, dy;
         liquid_delta(s, me.x, me.y, o.x, o.y, dx, dy);
         float d2 = dx * dx + dy * dy;
@@ -225,7 +255,7 @@ __global__ void __launch_bounds__(WG) k_liq_lambda(const SimDesc* __restrict__ s
         gy /= s.lrest;
         ogx += gx;
         ogy += gy;
-        if (mats[j].z == 0.0f) others += gx * gx + gy * gy;
+        if (o.z == 0.0f) others += gx * gx + gy * gy;
     });
     density += poly6(s, 0.0f);
     float wr, wgx, wgy;
@@ -266,13 +296,17 @@ __global__ void __launch_bounds__(WG) k_liq_apply(const SimDesc* __restrict__ si
     float ownMass = m.x;
     float ownLambda = lambdas[g] / ownMass;
     float cx = 0.0f, cy = 0.0f;
-    liquid_neighbours(s, g, (int)cellOf[g], prefix, items, [&](int j) {
-        float4 o = body[j];
+    liquid_neighbours(s, g, (int)cellOf[g], prefix, items,
+                      [&](int j) {
+                          const float4 o = body[j];
+                          return make_float3(o.x, o.y, lambdas[j]);
+                      },
+                      [&](int j, float3 o) {
         float dx, dy;
         liquid_delta(s, b.x, b.y, o.x, o.y, dx, dy);
         float d2 = dx * dx + dy * dy;
         if (d2 >= s.lh2) return;
-        float coef = ownLambda + lambdas[j] / ownMass;
+        float coef = ownLambda + o.z / ownMass;
         if (coef == 0.0f) return;
         float gx, gy;
         spiky(s, dx, dy, sqrtf(d2), (uint32_t)g, (uint32_t)j, fallback, gx, gy);
@@ -311,6 +345,11 @@ __global__ void __launch_bounds__(WG) k_liq_apply(const SimDesc* __restrict__ si
     out[g] = o;
 }
 
+struct BodySolid {
+    float4 b;
+    float solid;
+};
+
 // S/liquid_viscosity.comp:27-55: XSPH viscosity between fluid bodies.
 __global__ void __launch_bounds__(WG) k_liq_viscosity(const SimDesc* __restrict__ sims,
                                                       const BodyBlock* __restrict__ blocks,
@@ -331,9 +370,11 @@ __global__ void __launch_bounds__(WG) k_liq_viscosity(const SimDesc* __restrict_
     }
     float vx = b.z * s.vw, vy = b.w * s.vh;
     float bx = 0.0f, by = 0.0f;
-    liquid_neighbours(s, g, (int)cellOf[g], prefix, items, [&](int j) {
-        if (mats[j].z != 0.0f) return;
-        float4 o = body[j];
+    liquid_neighbours(s, g, (int)cellOf[g], prefix, items,
+                      [&](int j) { return BodySolid{body[j], mats[j].z}; },
+                      [&](int j, BodySolid od) {
+        if (od.solid != 0.0f) return;
+        const float4 o = od.b;
         float dx, dy;
         liquid_delta(s, b.x, b.y, o.x, o.y, dx, dy);
         float d2 = dx * dx + dy * dy;
