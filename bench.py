@@ -207,6 +207,46 @@ def adjoint_leg(steps=64, every=8, warmup=5):
             "gradient_finite": bool(np.isfinite(dv0).all())}
 
 
+def liquid_leg(steps=40, warmup=5):
+    """The liquid-and-sand workload: 400,000 bodies in walls on 2880x1440, a
+    uniform field of -500 px/s^2 in y, no mutual gravity; 60% fluid bodies
+    in a layer at one body per px^2 and 40% hard grains in a heap
+    (scenarios.liquid_and_sand, seed 3), contact diameter 1.0746 px, 16
+    substeps, 12 density iterations and XSPH viscosity per step. Reports the
+    graph-replayed step and the liquid stage's share of it."""
+    from gravity_amd import (CollisionConfig, GravityConfig, GravityFieldConfig, GravityPipeline,
+                             LiquidConfig, ParticleConfig, StartConfig, scenarios)
+    n, W, H, d = 400_000, 2880, 1440, 1.0746
+    bodies, mats = scenarios.liquid_and_sand(n, W, H, diameter=d, seed=3)
+    cfg = GravityConfig(particles=ParticleConfig(capacity=n, initialBodies=n, seed=3),
+                        gravity=GravityFieldConfig(gridWidth=1024, gridHeight=512),
+                        collisions=CollisionConfig(contactDiameterPixels=d, maximumSubsteps=SUBSTEPS),
+                        liquid=LiquidConfig(iterations=12, viscosity=0.1),
+                        start=StartConfig(walls=True, mutualGravity=False, liquid=True))
+    with GravityPipeline(cfg, W, H, scenarioState=bodies, bounded=True) as pipe:
+        pipe.upload(bodies, mats)
+        pipe.fieldAccelerationY = -500.0
+        pipe.initializeAcceleration(n)
+        for _ in range(warmup):
+            pipe.step(n)
+        pipe.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pipe.step(n)
+        pipe.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        stats = pipe.profile(2)
+        finite = bool(np.isfinite(pipe.bodies(n)).all())
+    total = sum(ms for _, _, ms in stats)
+    liquid = sum(ms for name, _, ms in stats if name.startswith("liquid"))
+    return {"workload": "liquid-sand-400k: walls, 2880x1440, field -500 px/s^2, 60% PBF fluid "
+                        "layer at 1 body/px^2, 40% grains heap, diameter 1.0746 px, 16 substeps, "
+                        "12 density iterations, seed 3",
+            "bodies": n, "steps": steps, "ms_per_step": t * 1e3, "value": n / t,
+            "unit": "body-steps/s", "liquid_stage_fraction": liquid / total if total else None,
+            "finite": finite}
+
+
 def cpu_baseline(bodies, mats, steps=2):
     """The oracle (single-threaded C restatement) on `steps` steps of the same workload."""
     from oracle import oracle as orc
@@ -232,6 +272,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=20,
                     help="steps of the batched-ensemble leg (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-liquid", action="store_true", help="skip the liquid-and-sand leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel eagerly instead of through captured graphs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -294,6 +335,11 @@ def main():
         print("[bench] adjoint: 64 steps recorded, VJP", file=sys.stderr, flush=True)
         adjoint = adjoint_leg()
 
+    liquid = None
+    if not args.no_liquid:
+        print("[bench] liquid and sand: 40 steps", file=sys.stderr, flush=True)
+        liquid = liquid_leg()
+
     cpu = None
     if not args.no_cpu:
         print(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps", file=sys.stderr, flush=True)
@@ -326,6 +372,7 @@ def main():
         "finite": finite,
         "batched": batched,
         "adjoint": adjoint,
+        "liquid": liquid,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
