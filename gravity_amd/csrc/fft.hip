@@ -81,6 +81,44 @@ __device__ __forceinline__ void dft8(float2* v) {
     v[1] = b[0]; v[3] = b[1]; v[5] = b[2]; v[7] = b[3];
 }
 
+// Radix 16 as 4 x 4: four 4-point DFTs over the inputs 4 apart, the
+// twiddles W16^(n2 k1), four 4-point DFTs across them; outputs in order.
+template <int DIR>
+__device__ __forceinline__ void dft16(float2* v) {
+    const float c1 = 0.92387953251128675613f;  // cos(pi / 8)
+    const float s1 = 0.38268343236508977173f;  // sin(pi / 8)
+    const float c2 = 0.70710678118654752440f;  // cos(pi / 4)
+    float2 y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        float2 a0 = v[n2], a1 = v[4 + n2], a2 = v[8 + n2], a3 = v[12 + n2];
+        dft4<DIR>(a0, a1, a2, a3);
+        y[4 * n2 + 0] = a0;
+        y[4 * n2 + 1] = a1;
+        y[4 * n2 + 2] = a2;
+        y[4 * n2 + 3] = a3;
+    }
+    // W16^m = (cos(2 pi m / 16), DIR sin(2 pi m / 16)) for m = n2 k1
+    y[5] = cmul(y[5], make_float2(c1, DIR * s1));     // m = 1
+    y[6] = cmul(y[6], make_float2(c2, DIR * c2));     // m = 2
+    y[7] = cmul(y[7], make_float2(s1, DIR * c1));     // m = 3
+    y[9] = cmul(y[9], make_float2(c2, DIR * c2));     // m = 2
+    y[10] = mul_w4<DIR>(y[10]);                       // m = 4
+    y[11] = cmul(y[11], make_float2(-c2, DIR * c2));  // m = 6
+    y[13] = cmul(y[13], make_float2(s1, DIR * c1));   // m = 3
+    y[14] = cmul(y[14], make_float2(-c2, DIR * c2));  // m = 6
+    y[15] = cmul(y[15], make_float2(-c1, -DIR * s1)); // m = 9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        float2 b0 = y[k1], b1 = y[4 + k1], b2 = y[8 + k1], b3 = y[12 + k1];
+        dft4<DIR>(b0, b1, b2, b3);
+        v[k1] = b0;
+        v[k1 + 4] = b1;
+        v[k1 + 8] = b2;
+        v[k1 + 12] = b3;
+    }
+}
+
 // One Stockham stage of radix R on nl lines of length L held in LDS;
 // element k of line l lives at buf[l * ls + k * es]. With NT lanes and at
 // most E values in LDS a lane takes Q = E / (R NT) butterflies. jFast:
@@ -118,7 +156,8 @@ __device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int l
                     v[q][r] = cmul(v[q][r], w);
                 }
             }
-            if (R == 8) dft8<DIR>(v[q]);
+            if (R == 16) dft16<DIR>(v[q]);
+            else if (R == 8) dft8<DIR>(v[q]);
             else if (R == 4) dft4<DIR>(v[q][0], v[q][1], v[q][2], v[q][3]);
             else dft2<DIR>(v[q]);
             int base = (j - k) * R + k;
@@ -130,15 +169,21 @@ __device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int l
 }
 
 // Transforms nl lines (nl * L <= E). DIR = -1 forward, +1 inverse
-// (unscaled, like S/fft_shared.comp).
-template <int DIR, int NT, int E>
+// (unscaled, like S/fft_shared.comp). Stages of radix MAXR (8 or 16) first:
+// the column transforms, whose workgroups hold four lines, take radix 16
+// (one stage fewer per direction); the rows keep radix 8, which gives each of
+// their lanes a butterfly per stage.
+template <int DIR, int NT, int E, int MAXR = 8>
 __device__ __forceinline__ void lds_fft(float2* buf, int L, int nl, int ls, int es, const float2* __restrict__ tw,
                         int twLen) {
     const bool jFast = es == 1;
     int Ns = 1;
     while (Ns < L) {
         int rem = L / Ns;
-        if (rem >= 8) {
+        if (MAXR >= 16 && rem >= 16) {
+            stockham_stage<DIR, 16, (E / (16 * NT) > 0 ? E / (16 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
+            Ns *= 16;
+        } else if (rem >= 8) {
             stockham_stage<DIR, 8, (E / (8 * NT) > 0 ? E / (8 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
             Ns *= 8;
         } else if (rem == 4) {
@@ -353,7 +398,7 @@ __global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restric
         }
     }
     __syncthreads();
-    lds_fft<-1, COL_NT, COL_LDS>(data, H, CB, 1, CB, twl, twLen);
+    lds_fft<-1, COL_NT, COL_LDS, 16>(data, H, CB, 1, CB, twl, twLen);
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         int i = (int)threadIdx.x + q * COL_NT;
@@ -363,7 +408,7 @@ __global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restric
         }
     }
     __syncthreads();
-    lds_fft<1, COL_NT, COL_LDS>(data, H, CB, 1, CB, twl, twLen);
+    lds_fft<1, COL_NT, COL_LDS, 16>(data, H, CB, 1, CB, twl, twLen);
     float2* dst = spec + s.meshBase;
     for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
         int y = i / CB, c = i - y * CB;
