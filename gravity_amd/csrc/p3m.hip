@@ -77,13 +77,16 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
     int d = (int)(blockIdx.x * WG + threadIdx.x);
     const bool active = d < slots;
     uint32_t c = 0xffffffffu, units = 0u;
+    int s0 = 0, s1 = 0;
     if (active) {
         uint32_t g = slotBody[d];
+        // the body's own data load with its cell, ahead of the rank
+        const float4 p = body[g];
+        const float m = mats[g].x;
         c = homeOf[g];
-        int s0 = (int)cellPrefix[c], s1 = (int)cellPrefix[c + 1];
+        s0 = (int)cellPrefix[c];
+        s1 = (int)cellPrefix[c + 1];
         const int rank = segment_rank<true>(slotBody, s0, s1, g);
-        float4 p = body[g];
-        float m = mats[g].x;
         cellBody[s0 + rank] = make_float4(p.x, p.y, m, __uint_as_float(g));
         units = (uint32_t)rintf(m * 256.0f);
         if (s1 - s0 <= NEAR_INLINE) {
@@ -106,7 +109,7 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
     if (active && run.rank == run.length - 1) {
         uint32_t* mass = &reinterpret_cast<uint32_t*>(cellRec)[4 * (size_t)c + 2];
         uint32_t* nearMass = &reinterpret_cast<uint32_t*>(cellNear)[8 * (size_t)c + 1];
-        if (cellPrefix[c] + (uint32_t)run.length == cellPrefix[c + 1]) {
+        if (s0 + run.length == s1) {
             // the run is the whole cell: no other wave adds to it
             *mass = units;
             *nearMass = units;

@@ -200,13 +200,17 @@ __global__ void __launch_bounds__(WG) k_sort_canon(
     int d = b.first + (int)threadIdx.x;
     if (d - s.base >= s.n) return;
     uint32_t src = slotSrc[d];
+    // the moved data load with the tile, ahead of the rank
+    const float4 bo = bodyIn[src];
+    const float2 ac = accIn[src];
+    const float4 ma = matIn[src];
     uint32_t t = keyOf[src];
     int s0 = s.base + (int)(tilePrefix[t] - (uint32_t)s.rank0);
     int s1 = s.base + (int)(tilePrefix[t + 1] - (uint32_t)s.rank0);
     int fin = s0 + segment_rank<false>(slotSrc, s0, s1, src);
-    bodyOut[fin] = bodyIn[src];
-    accOut[fin] = accIn[src];
-    matOut[fin] = matIn[src];
+    bodyOut[fin] = bo;
+    accOut[fin] = ac;
+    matOut[fin] = ma;
     dest[src] = (uint32_t)(fin - s.base);
 }
 
