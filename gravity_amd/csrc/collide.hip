@@ -385,16 +385,16 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, cons
 }
 
 __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
-                                              const BodyBlock* __restrict__ blocks,
-                                              const int* __restrict__ order,
+                                              const OrderedBlock* __restrict__ order,
                                               uint32_t* __restrict__ waveWork, GridBuf grid,
                                               GridBuf prev, int clearPrev,
                                               const float4* __restrict__ mats,
                                               uint32_t* __restrict__ counts,
                                               uint32_t* __restrict__ contacts, int planeStride,
                                               int guardLimit, ContactPool pool, SleepBufs sleep) {
-    const int bi = order[blockIdx.x];
-    BodyBlock b = blocks[bi];
+    const OrderedBlock ob = order[blockIdx.x];
+    const int bi = ob.index;
+    const BodyBlock b{ob.sim, ob.first};
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     uint32_t n = 0;
@@ -425,17 +425,24 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
 // last full XCD round, and XCDs with more than ORDER_MAX blocks, keep the
 // xcd_block order.
 constexpr int ORDER_MAX = 512;
-__global__ void __launch_bounds__(WG) k_block_order(int nb, const uint32_t* __restrict__ waveWork,
-                                                    int* __restrict__ order) {
+__device__ __forceinline__ OrderedBlock ordered(const BodyBlock* __restrict__ blocks, int i) {
+    const BodyBlock b = blocks[i];
+    return OrderedBlock{b.sim, b.first, i, 0};
+}
+
+__global__ void __launch_bounds__(WG) k_block_order(int nb, const BodyBlock* __restrict__ blocks,
+                                                    const uint32_t* __restrict__ waveWork,
+                                                    OrderedBlock* __restrict__ order) {
     __shared__ uint32_t key[ORDER_MAX];
     __shared__ int item[ORDER_MAX];
     const int x = (int)blockIdx.x;
     const int full = nb / (XCDS * XCD_CHUNK) * (XCDS * XCD_CHUNK);
     const int m = full / XCDS;
     if (x == 0)
-        for (int i = full + (int)threadIdx.x; i < nb; i += WG) order[i] = i;
+        for (int i = full + (int)threadIdx.x; i < nb; i += WG) order[i] = ordered(blocks, i);
     if (m > ORDER_MAX) {
-        for (int j = (int)threadIdx.x; j < m; j += WG) order[x + XCDS * j] = xcd_block(x + XCDS * j, nb);
+        for (int j = (int)threadIdx.x; j < m; j += WG)
+            order[x + XCDS * j] = ordered(blocks, xcd_block(x + XCDS * j, nb));
         return;
     }
     for (int j = (int)threadIdx.x; j < m; j += WG) {
@@ -450,18 +457,21 @@ __global__ void __launch_bounds__(WG) k_block_order(int nb, const uint32_t* __re
         uint32_t kj = key[j];
         int rank = 0;
         for (int k = 0; k < m; k++) rank += key[k] > kj ? 1 : 0;
-        order[x + XCDS * rank] = item[j];
+        order[x + XCDS * rank] = ordered(blocks, item[j]);
     }
 }
 
-void launch_block_order(hipStream_t st, const LaunchShape& ls, int* order) {
+void launch_block_order(hipStream_t st, const LaunchShape& ls, OrderedBlock* order) {
     if (ls.bodyBlocks == 0) return;
-    launch(k_block_order, dim3(XCDS), dim3(WG), 0, st, ls.bodyBlocks, ls.waveWork,
+    launch(k_block_order, dim3(XCDS), dim3(WG), 0, st, ls.bodyBlocks, ls.blocks, ls.waveWork,
                        order);
 }
 
-void identity_block_order(int blocks, int* order) {
-    for (int i = 0; i < blocks; i++) order[i] = xcd_block(i, blocks);
+void identity_block_order(const BodyBlock* blocks, int count, OrderedBlock* order) {
+    for (int i = 0; i < count; i++) {
+        const int bi = xcd_block(i, count);
+        order[i] = OrderedBlock{blocks[bi].sim, blocks[bi].first, bi, 0};
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -667,8 +677,7 @@ __device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, ui
 constexpr int SOLVE_BATCH = 4;
 
 __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
-                                              const BodyBlock* __restrict__ blocks,
-                                              const int* __restrict__ order,
+                                              const OrderedBlock* __restrict__ order,
                                               const float4* __restrict__ bodyIn,
                                               float4* __restrict__ bodyOut,
                                               const float4* __restrict__ mats,
@@ -682,14 +691,16 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     // The pool's entries were all written by the count; its counter starts
     // the next count from zero (no kernel of this launch reads it).
     if (blockIdx.x == 0 && threadIdx.x == 0) *pool.top = 0u;
+    // the block and its simulation's constants load while the table fills
+    const OrderedBlock ob = order[blockIdx.x];
+    const BodyBlock b{ob.sim, ob.first};
+    const SimDesc& s = sims[b.sim];
+    const SolveK k = solve_k(s);
     __shared__ float recip[RECIP_TABLE];
     if (threadIdx.x < RECIP_TABLE) recip[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 1.0f;
     __syncthreads();
-    BodyBlock b = blocks[order[blockIdx.x]];
-    const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
-    const SolveK k = solve_k(s);
     float4 me = bodyIn[g];
     bool sleeping = false;
     uint32_t stamp = 0u;
@@ -886,7 +897,7 @@ void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid
                   const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
                   uint32_t* contacts) {
     if (ls.bodyBlocks == 0) return;
-    launch(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
+    launch(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.order,
                        ls.waveWork,
                        grid_of(grid), grid_of(prev), clearPrev, mats, counts, contacts,
                        ls.planeStride, ls.guardLimit, ls.pool, ls.sleep);
@@ -897,7 +908,7 @@ void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, f
                   const CollideGrid& grid, const float2* fallback, float* pressures,
                   int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next) {
     if (ls.bodyBlocks == 0) return;
-    launch(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, ls.order,
+    launch(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.order,
                        bodyIn,
                        bodyOut, mats, counts, contacts, ls.planeStride, grid_of(grid),
                        ls.guardLimit, fallback, pressures, writePressure, nextDriftDt, insertNext,

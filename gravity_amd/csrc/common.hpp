@@ -134,6 +134,9 @@ __device__ __forceinline__ int sleep_tile(const SimDesc& s, float x, float y) {
 
 // Global work tables (built on the host whenever the batch shape changes).
 struct BodyBlock { int sim; int first; };
+// A contact workgroup's body block with its index, so that the dispatch
+// order hands a workgroup its block in one load.
+struct OrderedBlock { int sim; int first; int index; int pad; };
 
 // Workgroups are dealt to the XCDs round robin (workgroup i runs on XCD
 // i % 8). Body blocks are spatially sorted, so hand each XCD runs of

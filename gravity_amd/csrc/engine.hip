@@ -224,7 +224,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     dSims_ = dalloc<SimDesc>(sims_.size());
     int maxBlocks = totalSlots_ / WG;
     dBlocks_ = dalloc<BodyBlock>((size_t)maxBlocks);
-    order_ = dalloc<int>((size_t)maxBlocks);
+    order_ = dalloc<OrderedBlock>((size_t)maxBlocks);
     waveWork_ = dalloc<uint32_t>((size_t)maxBlocks * (WG / 64));
     int maxLines = 0;
     for (auto& h : sims_) maxLines += 2 * std::max(h.cfg.grid_width, h.cfg.grid_height) * 2 + 2;
@@ -431,10 +431,10 @@ void Engine::rebuild_tables() {
     if (!blocks.empty()) {
         check(hipMemcpy(dBlocks_, blocks.data(), blocks.size() * sizeof(BodyBlock),
                         hipMemcpyHostToDevice), "blocks");
-        std::vector<int> order(blocks.size());
-        identity_block_order(bodyBlocks_, order.data());
-        check(hipMemcpy(order_, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice),
-              "order");
+        std::vector<OrderedBlock> order(blocks.size());
+        identity_block_order(blocks.data(), bodyBlocks_, order.data());
+        check(hipMemcpy(order_, order.data(), order.size() * sizeof(OrderedBlock),
+                        hipMemcpyHostToDevice), "order");
     }
     if (!rowsF.empty()) {
         check(hipMemcpy(dRowsF_, rowsF.data(), rowsF.size() * sizeof(LineWork), hipMemcpyHostToDevice), "rows");

@@ -156,7 +156,7 @@ private:
     // tables
     SimDesc* dSims_ = nullptr;
     BodyBlock* dBlocks_ = nullptr;
-    int* order_ = nullptr;           // dispatch order of the contact kernels
+    OrderedBlock* order_ = nullptr;  // dispatch order of the contact kernels
     uint32_t* waveWork_ = nullptr;   // contacts per wave of the last count
     LineWork *dRowsF_ = nullptr, *dCols_ = nullptr, *dRowsI_ = nullptr;
     int bodyBlocks_ = 0, rowsF_ = 0, cols_ = 0, rowsI_ = 0, gravitySlots_ = 0, liquidSlots_ = 0;

@@ -37,9 +37,10 @@ struct LaunchShape {
     SleepBufs sleep;     // sleeping tiles of the batch
     // Dispatch order of the contact kernels: workgroup i takes body block
     // order[i] (a permutation keeping every block on the XCD xcd_block gives
-    // it), and waveWork[4 * block + wave] receives the contacts the count
-    // found in that wave, from which the order is rebuilt once per step.
-    const int* order;
+    // it, with the block's data), and waveWork[4 * block + wave] receives the
+    // contacts the count found in that wave, from which the order is rebuilt
+    // once per step.
+    const OrderedBlock* order;
     uint32_t* waveWork;
 };
 
@@ -71,9 +72,9 @@ void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, co
                          const CollideGrid& grid);
 // Rebuilds ls.order: on every XCD, its body blocks by descending contacts of
 // the last count, so the heaviest blocks are dispatched first, one per CU.
-void launch_block_order(hipStream_t st, const LaunchShape& ls, int* order);
+void launch_block_order(hipStream_t st, const LaunchShape& ls, OrderedBlock* order);
 // The order xcd_block alone gives, on the host.
-void identity_block_order(int blocks, int* order);
+void identity_block_order(const BodyBlock* blocks, int count, OrderedBlock* order);
 void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
                   const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
                   uint32_t* contacts);
