@@ -276,12 +276,12 @@ __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__
 // Count (S/collision_cache_count.comp:61-95, 130-285 without RIGID): every
 // hard pair with d^2 < D^2, strictly.
 template <bool BOUNDED>
-__device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, const GridBuf& grid,
+__device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, float4 own,
+                                               const GridBuf& grid,
                                                const float4* __restrict__ mats,
                                                uint32_t* __restrict__ counts,
                                                uint32_t* __restrict__ contacts, int planeStride,
                                                int guardLimit, const ContactPool& pool) {
-    const float4 own = grid.nodes[g];
     const float px = own.x, py = own.y, vw = s.vw, vh = s.vh, D2 = s.diameter2;
     const uint32_t cell = __float_as_uint(own.w);
     const bool ownFluid = s.liquid && mats[g].z == 0.0f;
@@ -399,7 +399,12 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
     int g = b.first + (int)threadIdx.x;
     uint32_t n = 0;
     if (g - s.base < s.n) {
-        if (clearPrev) prev.heads[prev.cellOf[g]] = 0u;
+        // The body's cell in the previous pass's grid and its own node here
+        // load together (cellOf is always allocated, so the first load is
+        // unconditional); the clearing store waits only for the first.
+        const int prevCell = prev.cellOf[g];
+        const float4 own = grid.nodes[g];
+        if (clearPrev) prev.heads[prevCell] = 0u;
         // A sleeping body keeps its last count (S/collision_cache_count.comp:147-160).
         bool asleep = false;
         if (s.sleepOn) {
@@ -407,10 +412,10 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
             asleep = sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp;
         }
         if (!asleep) {
-            n = s.bounded ? count_contacts<true>(s, g, grid, mats, counts, contacts, planeStride,
-                                                 guardLimit, pool)
-                          : count_contacts<false>(s, g, grid, mats, counts, contacts, planeStride,
-                                                  guardLimit, pool);
+            n = s.bounded ? count_contacts<true>(s, g, own, grid, mats, counts, contacts,
+                                                 planeStride, guardLimit, pool)
+                          : count_contacts<false>(s, g, own, grid, mats, counts, contacts,
+                                                  planeStride, guardLimit, pool);
         }
     }
     // The wave's contacts, for the next step's dispatch order.
