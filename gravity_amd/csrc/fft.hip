@@ -200,19 +200,31 @@ __device__ __forceinline__ void lds_fft(float2* buf, int L, int nl, int ls, int 
 // loads of a kernel: the stages then read it without a memory round trip.
 // A lane's TW_LDS / NT loads are issued together (clamped to the table), then
 // stored: one round trip, where a load-store loop waits for every load.
+// A kernel loads its lane's part of the table first and stores it to LDS
+// after issuing its other global loads, so those are in flight together.
 constexpr int TW_LDS = 2048;
+template <int NT>
+struct Twiddles {
+    static constexpr int PER = TW_LDS / NT;
+    float2 v[PER];
+    __device__ __forceinline__ void load(const float2* __restrict__ tw, int twLen) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) v[q] = tw[min((int)threadIdx.x + q * NT, twLen - 1)];
+    }
+    __device__ __forceinline__ void store(float2* twl, int twLen) const {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int i = (int)threadIdx.x + q * NT;
+            if (i < twLen) twl[i] = v[q];
+        }
+    }
+};
 template <int NT>
 __device__ __forceinline__ void stage_twiddles(float2* twl, const float2* __restrict__ tw,
                                                int twLen) {
-    constexpr int PER = TW_LDS / NT;
-    float2 v[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) v[q] = tw[min((int)threadIdx.x + q * NT, twLen - 1)];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int i = (int)threadIdx.x + q * NT;
-        if (i < twLen) twl[i] = v[q];
-    }
+    Twiddles<NT> t;
+    t.load(tw, twLen);
+    t.store(twl, twLen);
 }
 
 // Mesh row of forward/inverse work row a: all rows on the torus; under walls
@@ -235,7 +247,8 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
-    stage_twiddles<ROW_NT>(twl, tw, twLen);
+    Twiddles<ROW_NT> tws;
+    tws.load(tw, twLen);  // stored after the deposit's loads
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) mass[i] = 0u;
     __syncthreads();
     const float scale = s.massScale;
@@ -339,6 +352,7 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
             }
         }
     }
+    tws.store(twl, twLen);
     __syncthreads();
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT)
         data[i] = make_float2((float)mass[i] * s.invMassScale, 0.0f);
@@ -361,7 +375,8 @@ __global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restric
     LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int H = s.mh, CB = w.count, c0 = w.first;
-    stage_twiddles<COL_NT>(twl, tw, twLen);
+    Twiddles<COL_NT> tws;
+    tws.load(tw, twLen);  // stored after the values' loads
     const float2* src = spec + s.meshBase;
     // A lane's COL_LDS / COL_NT values load together (a dead row or a lane
     // past the item reads its column's row 0), then go to LDS.
@@ -375,6 +390,7 @@ __global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restric
             const bool live = !s.bounded || y <= s.ph || y == H - 1;
             v[q] = src[(live ? y * s.mw : 0) + c0 + c];
         }
+        tws.store(twl, twLen);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int i = (int)threadIdx.x + q * COL_NT;
@@ -426,7 +442,8 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_inverse(const SimDesc* __re
     LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
-    stage_twiddles<ROW_NT>(twl, tw, twLen);
+    Twiddles<ROW_NT> tws;
+    tws.load(tw, twLen);  // stored after the values' loads
     {
         // A lane's ROW_LDS / ROW_NT values load together, then go to LDS.
         constexpr int PER = ROW_LDS / ROW_NT;
@@ -438,6 +455,7 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_inverse(const SimDesc* __re
             const int l = i / L, x = i - l * L;
             v[q] = spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x];
         }
+        tws.store(twl, twLen);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int i = (int)threadIdx.x + q * ROW_NT;
