@@ -677,9 +677,9 @@ __device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, ui
 }
 
 // Contacts whose loads are issued together. The batch loop is not
-// unrolled: four contacts' registers keep the kernel at 7 waves per SIMD,
-// which holds every wave of a 400k-body launch resident at once.
-constexpr int SOLVE_BATCH = 4;
+// unrolled; three contacts a batch (73 VGPRs, six waves per SIMD) measured
+// faster than two or four.
+constexpr int SOLVE_BATCH = 3;
 
 __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               const OrderedBlock* __restrict__ order,
