@@ -629,7 +629,9 @@ __device__ __forceinline__ void accumulate_contact(const SolveK& s, const float*
 // The OVERFLOW_CHUNK smallest contact indices above floorId, ascending, by
 // one walk of the stencil (unused entries 0xffffffff).
 constexpr int OVERFLOW_CHUNK = 6;
-constexpr int OVERFLOW_BATCH = 2;   // their loads issued together
+// One contact's loads at a time: the rare path keeps the solve at 71 VGPRs,
+// seven waves per SIMD, so every wave of a 400k-body launch is resident.
+constexpr int OVERFLOW_BATCH = 1;
 template <bool BOUNDED>
 __device__ __forceinline__ void next_contacts(const SimDesc& s, int cx, int cy, const GridBuf& grid,
                                               int guardLimit, int g, float px, float py,
@@ -677,8 +679,7 @@ __device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, ui
 }
 
 // Contacts whose loads are issued together. The batch loop is not
-// unrolled; three contacts a batch (73 VGPRs, six waves per SIMD) measured
-// faster than two or four.
+// unrolled; three contacts a batch measured faster than two or four.
 constexpr int SOLVE_BATCH = 3;
 
 __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
