@@ -187,6 +187,8 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     cellPrefix_ = dalloc<uint32_t>((size_t)totalPhys_ + 1);
     homeOf_ = dalloc<uint32_t>(S);
     slotBody_ = dalloc<uint32_t>(S);
+    slotCell_ = dalloc<uint32_t>(S);
+    slotKey_ = dalloc<uint32_t>(S);
     cellRec_ = dalloc<uint4>((size_t)totalPhys_);
     cellNear_ = dalloc<uint4>(2 * (size_t)totalPhys_);
     cellBody_ = dalloc<float4>(S);
@@ -272,7 +274,7 @@ Engine::~Engine() {
                     headsAlloc_[0], headsAlloc_[1], grid_[0].nodes, grid_[1].nodes,
                     grid_[0].cellOf, grid_[1].cellOf, press_, counts_, contacts_, tileCount_,
                     tilePrefix_, keyOf_, slotSrc_, dest_, scanTmp_, cellCount_, cellPrefix_,
-                    homeOf_, slotBody_, cellRec_, cellNear_, cellBody_, spec_, green_, tw_, fallback_,
+                    homeOf_, slotBody_, slotCell_, slotKey_, cellRec_, cellNear_, cellBody_, spec_, green_, tw_, fallback_,
                     liqCount_, liqPrefix_, liqCellOf_, liqUnordered_, liqItems_, lambdas_,
                     sleepState_, bodyAwake_, tileAwake_, tileActive_, dTileBlocks_,
                     dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_, poolItems_, poolBase_, poolTop_};
@@ -630,9 +632,9 @@ void Engine::enqueue_sort() {
     int o = bodyPar_ ^ 1, a = auxPar_ ^ 1;
     timed("sort_count", [&] { launch_sort_count(st_, shape_, body_[bodyPar_], tileCount_, keyOf_); });
     timed("sort_scan", [&] { launch_scan(st_, tileCount_, tilePrefix_, totalTiles_, scanTmp_); });
-    timed("sort_scatter", [&] { launch_sort_scatter(st_, shape_, keyOf_, tileCount_, tilePrefix_, slotSrc_); });
+    timed("sort_scatter", [&] { launch_sort_scatter(st_, shape_, keyOf_, tileCount_, tilePrefix_, slotSrc_, slotKey_); });
     timed("sort_canon", [&] {
-        launch_sort_canon(st_, shape_, keyOf_, tilePrefix_, slotSrc_, body_[bodyPar_],
+        launch_sort_canon(st_, shape_, slotKey_, tilePrefix_, slotSrc_, body_[bodyPar_],
                           acc_[auxPar_], mat_[auxPar_], body_[o], acc_[a], mat_[a], dest_);
     });
     bodyPar_ = o;
@@ -651,9 +653,9 @@ void Engine::enqueue_force() {
         timed("cell_scan", [&] {
             launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, cellRec_, cellNear_);
         });
-        timed("cell_scatter", [&] { launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_); });
+        timed("cell_scatter", [&] { launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_, slotCell_); });
         timed("cell_canon", [&] {
-            launch_cell_canon(st_, gravitySlots_, homeOf_, cellPrefix_, slotBody_, body_[bodyPar_],
+            launch_cell_canon(st_, gravitySlots_, slotCell_, cellPrefix_, slotBody_, body_[bodyPar_],
                               mat_[auxPar_], cellBody_, cellRec_, cellNear_);
         });
         timed("fft_rows_forward", [&] {

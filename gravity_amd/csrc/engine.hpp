@@ -139,6 +139,8 @@ private:
     // P3M
     uint32_t *cellCount_ = nullptr, *cellPrefix_ = nullptr, *homeOf_ = nullptr;
     uint32_t* slotBody_ = nullptr;
+    uint32_t* slotCell_ = nullptr;   // per cell slot: the cell of the body in it
+    uint32_t* slotKey_ = nullptr;    // per sorted slot: the tile of the body in it
     uint4* cellRec_ = nullptr;  // per physical cell: first slot, count, mass / 256, 0
     uint4* cellNear_ = nullptr;  // per physical cell: the kick's 32-byte near record
     float4* cellBody_ = nullptr;

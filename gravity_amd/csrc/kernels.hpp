@@ -58,9 +58,12 @@ int scan_totals_size(int len);
 // ---- spatial sort ----
 void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float4* body,
                        uint32_t* tileCount, uint32_t* keyOf);
+// slotKey receives the tile of the body in each slot, which the canon reads
+// with the slot instead of through the body.
 void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
-                         uint32_t* tileCount, const uint32_t* tilePrefix, uint32_t* slotSrc);
-void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
+                         uint32_t* tileCount, const uint32_t* tilePrefix, uint32_t* slotSrc,
+                         uint32_t* slotKey);
+void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* slotKey,
                        const uint32_t* tilePrefix, const uint32_t* slotSrc,
                        const float4* bodyIn, const float2* accIn, const float4* matIn,
                        float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest);
@@ -86,13 +89,16 @@ void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, f
 // ---- P3M gravity ----
 void launch_bin(hipStream_t st, const LaunchShape& ls, const float4* body, uint32_t* cellCount,
                 uint32_t* homeOf);
+// slotCell receives the cell of the body in each slot, which the canon reads
+// with the slot instead of through the body.
 void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* homeOf,
-                         uint32_t* cellCount, const uint32_t* cellPrefix, uint32_t* slotBody);
+                         uint32_t* cellCount, const uint32_t* cellPrefix, uint32_t* slotBody,
+                         uint32_t* slotCell);
 // cellNear: per physical cell two uint4, {count, mass in 1/256, x0, y0} and
 // {x1, y1, m1, m0}: a cell of at most NEAR_INLINE bodies carries its list's
 // bodies (x, y normalised, m mass) in the record itself.
 constexpr int NEAR_INLINE = 2;
-void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
+void launch_cell_canon(hipStream_t st, int slots, const uint32_t* slotCell,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
                        const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear);
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,

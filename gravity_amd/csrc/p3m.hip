@@ -45,7 +45,8 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
                                                      const uint32_t* __restrict__ homeOf,
                                                      uint32_t* cellCount,
                                                      const uint32_t* __restrict__ cellPrefix,
-                                                     uint32_t* __restrict__ slotBody) {
+                                                     uint32_t* __restrict__ slotBody,
+                                                     uint32_t* __restrict__ slotCell) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
@@ -57,7 +58,9 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
     top = (uint32_t)__shfl((int)top, run.start, 64);
     if (!active) return;
     uint32_t rank = top - 1u - (uint32_t)run.rank;
-    slotBody[cellPrefix[c] + rank] = (uint32_t)g;
+    const uint32_t slot = cellPrefix[c] + rank;
+    slotBody[slot] = (uint32_t)g;
+    slotCell[slot] = c;
 }
 
 // Orders each cell by descending body index and sums the cell mass in 1/256
@@ -66,7 +69,7 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
 // cell stores its sum, any other accumulates it with one atomic into the
 // cell record and the near record (whose masses the cell scan zeroed). The
 // bodies of a cell of at most NEAR_INLINE go into its near record as well.
-__global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __restrict__ homeOf,
+__global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __restrict__ slotCell,
                                                    const uint32_t* __restrict__ cellPrefix,
                                                    const uint32_t* __restrict__ slotBody,
                                                    const float4* __restrict__ body,
@@ -80,10 +83,10 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
     int s0 = 0, s1 = 0;
     if (active) {
         uint32_t g = slotBody[d];
-        // the body's own data load with its cell, ahead of the rank
+        c = slotCell[d];
+        // the body's own data load with its cell's bounds, ahead of the rank
         const float4 p = body[g];
         const float m = mats[g].x;
-        c = homeOf[g];
         s0 = (int)cellPrefix[c];
         s1 = (int)cellPrefix[c + 1];
         const int rank = segment_rank<true>(slotBody, s0, s1, g);
@@ -376,17 +379,18 @@ void launch_bin(hipStream_t st, const LaunchShape& ls, const float4* body, uint3
 }
 
 void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* homeOf,
-                         uint32_t* cellCount, const uint32_t* cellPrefix, uint32_t* slotBody) {
+                         uint32_t* cellCount, const uint32_t* cellPrefix, uint32_t* slotBody,
+                         uint32_t* slotCell) {
     if (ls.bodyBlocks == 0) return;
     launch(k_cell_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       homeOf, cellCount, cellPrefix, slotBody);
+                       homeOf, cellCount, cellPrefix, slotBody, slotCell);
 }
 
-void launch_cell_canon(hipStream_t st, int slots, const uint32_t* homeOf,
+void launch_cell_canon(hipStream_t st, int slots, const uint32_t* slotCell,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
                        const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear) {
     if (slots <= 0) return;
-    launch(k_cell_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, homeOf,
+    launch(k_cell_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, slotCell,
                        cellPrefix, slotBody, body, mats, cellBody, cellRec, cellNear);
 }
 

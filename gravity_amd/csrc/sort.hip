@@ -171,7 +171,8 @@ __global__ void __launch_bounds__(WG) k_sort_scatter(const SimDesc* __restrict__
                                                      const uint32_t* __restrict__ keyOf,
                                                      uint32_t* tileCount,
                                                      const uint32_t* __restrict__ tilePrefix,
-                                                     uint32_t* __restrict__ slotSrc) {
+                                                     uint32_t* __restrict__ slotSrc,
+                                                     uint32_t* __restrict__ slotKey) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
@@ -185,12 +186,13 @@ __global__ void __launch_bounds__(WG) k_sort_scatter(const SimDesc* __restrict__
     uint32_t rank = top - 1u - (uint32_t)run.rank;
     int slot = s.base + (int)(tilePrefix[t] - (uint32_t)s.rank0 + rank);
     slotSrc[slot] = (uint32_t)g;
+    slotKey[slot] = t;
 }
 
 // Orders every tile by original index and moves the bodies.
 __global__ void __launch_bounds__(WG) k_sort_canon(
     const SimDesc* __restrict__ sims, const BodyBlock* __restrict__ blocks,
-    const uint32_t* __restrict__ keyOf, const uint32_t* __restrict__ tilePrefix,
+    const uint32_t* __restrict__ slotKey, const uint32_t* __restrict__ tilePrefix,
     const uint32_t* __restrict__ slotSrc, const float4* __restrict__ bodyIn,
     const float2* __restrict__ accIn, const float4* __restrict__ matIn,
     float4* __restrict__ bodyOut, float2* __restrict__ accOut, float4* __restrict__ matOut,
@@ -200,11 +202,11 @@ __global__ void __launch_bounds__(WG) k_sort_canon(
     int d = b.first + (int)threadIdx.x;
     if (d - s.base >= s.n) return;
     uint32_t src = slotSrc[d];
-    // the moved data load with the tile, ahead of the rank
+    uint32_t t = slotKey[d];
+    // the moved data load with the tile's bounds, ahead of the rank
     const float4 bo = bodyIn[src];
     const float2 ac = accIn[src];
     const float4 ma = matIn[src];
-    uint32_t t = keyOf[src];
     int s0 = s.base + (int)(tilePrefix[t] - (uint32_t)s.rank0);
     int s1 = s.base + (int)(tilePrefix[t + 1] - (uint32_t)s.rank0);
     int fin = s0 + segment_rank<false>(slotSrc, s0, s1, src);
@@ -222,19 +224,20 @@ void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float4* body
 }
 
 void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
-                         uint32_t* tileCount, const uint32_t* tilePrefix, uint32_t* slotSrc) {
+                         uint32_t* tileCount, const uint32_t* tilePrefix, uint32_t* slotSrc,
+                         uint32_t* slotKey) {
     if (ls.bodyBlocks == 0) return;
     launch(k_sort_scatter, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       keyOf, tileCount, tilePrefix, slotSrc);
+                       keyOf, tileCount, tilePrefix, slotSrc, slotKey);
 }
 
-void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* keyOf,
+void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* slotKey,
                        const uint32_t* tilePrefix, const uint32_t* slotSrc,
                        const float4* bodyIn, const float2* accIn, const float4* matIn,
                        float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest) {
     if (ls.bodyBlocks == 0) return;
     launch(k_sort_canon, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       keyOf, tilePrefix, slotSrc, bodyIn, accIn, matIn, bodyOut, accOut, matOut,
+                       slotKey, tilePrefix, slotSrc, bodyIn, accIn, matIn, bodyOut, accOut, matOut,
                        dest);
 }
 
