@@ -419,10 +419,17 @@ void Engine::rebuild_tables() {
         const SimDesc& d = h.desc;
         int active = h.bounded ? d.ph + 2 : d.mh;
         int nl = fft_row_lines_per_group(d.mw);
-        for (int a = 0; a < active; a += nl) {
-            rowsF.push_back({s, a, std::min(nl, active - a), 0});
-            rowsI.push_back({s, a, std::min(nl, active - a), 0});
-        }
+        // A forward item deposits every body of its rows first: a crowded
+        // simulation (many bodies per short row) takes fewer lines per item,
+        // about FORWARD_ITEM_BODIES bodies each, so its deposits run side by
+        // side (the grouping changes no result: the sums are integer and each
+        // line is transformed alone).
+        constexpr long FORWARD_ITEM_BODIES = 4096;
+        int nlF = nl;
+        if (h.n > 0)
+            nlF = (int)std::max<long>(1, std::min<long>(nl, FORWARD_ITEM_BODIES * active / h.n));
+        for (int a = 0; a < active; a += nlF) rowsF.push_back({s, a, std::min(nlF, active - a), 0});
+        for (int a = 0; a < active; a += nl) rowsI.push_back({s, a, std::min(nl, active - a), 0});
         int cb = fft_columns_per_group(d.mh, d.mw);
         for (int c = 0; c < d.mw; c += cb) cols.push_back({s, c, cb, 0});
     }
