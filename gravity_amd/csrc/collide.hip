@@ -702,9 +702,17 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     const BodyBlock b{ob.sim, ob.first};
     const SimDesc& s = sims[b.sim];
     const SolveK k = solve_k(s);
-    __shared__ float recip[RECIP_TABLE];
-    if (threadIdx.x < RECIP_TABLE) recip[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 1.0f;
-    __syncthreads();
+    // Each wave fills its own copy of the table (one entry a lane), so no
+    // workgroup barrier holds the waves: a wave's LDS writes are complete
+    // for its own later reads once the wave has waited for them.
+    static_assert(RECIP_TABLE == 64, "one table entry per lane of a wave");
+    __shared__ float recipWaves[WG / 64][RECIP_TABLE];
+    const int lane = (int)(threadIdx.x & 63u);
+    float* recip = recipWaves[threadIdx.x >> 6];
+    recip[lane] = lane ? 1.0f / (float)lane : 1.0f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
     float4 me = bodyIn[g];
