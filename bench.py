@@ -101,10 +101,11 @@ def ensemble_configs():
     return members, cfgs
 
 
-def batched_leg(steps, warmup):
+def batched_leg(steps, warmup, plain=True):
     """The ensemble of 64 torus simulations stepped as one batch (one graph
-    launch per step) and, for comparison, as 64 plain pipelines stepped one
-    after another. Both rates are whole-ensemble body-steps/s."""
+    launch per step) and, for comparison (unless plain is False), as 64 plain
+    pipelines stepped one after another. Both rates are whole-ensemble
+    body-steps/s."""
     from gravity_amd import GravityBatch, GravityPipeline
     members, cfgs = ensemble_configs()
     total = sum(m[0] for m in members)
@@ -122,6 +123,13 @@ def batched_leg(steps, warmup):
             batch.step()
         batch.synchronize()
         t_batch = time.perf_counter() - t0
+    rate = total * steps / t_batch
+    record = {"workload": "ensemble-64: 64 torus simulations on 1024x1024, N = floor(10^U(3,5)), "
+                          "grids 2^U{6..9} squared, Plummer spheres a = 50 px, seeds 100..163",
+              "sims": len(members), "bodies": total, "steps": steps, "warmup": warmup,
+              "value": rate, "unit": "body-steps/s", "ms_per_step": t_batch * 1e3 / steps}
+    if not plain:
+        return record
     pipes = []
     try:
         for cfg, (n, _, b, mat) in zip(cfgs, members):
@@ -144,14 +152,10 @@ def batched_leg(steps, warmup):
     finally:
         for p in pipes:
             p.close()
-    rate = total * steps / t_batch
-    plain = total * steps / t_plain
-    return {"workload": "ensemble-64: 64 torus simulations on 1024x1024, N = floor(10^U(3,5)), "
-                        "grids 2^U{6..9} squared, Plummer spheres a = 50 px, seeds 100..163",
-            "sims": len(members), "bodies": total, "steps": steps, "warmup": warmup,
-            "value": rate, "unit": "body-steps/s", "ms_per_step": t_batch * 1e3 / steps,
-            "plain_loop": plain, "plain_ms_per_step": t_plain * 1e3 / steps,
-            "speedup_vs_plain": rate / plain}
+    loop = total * steps / t_plain
+    record.update({"plain_loop": loop, "plain_ms_per_step": t_plain * 1e3 / steps,
+                   "speedup_vs_plain": rate / loop})
+    return record
 
 
 def adjoint_leg(steps=64, every=8, warmup=5):
@@ -275,6 +279,9 @@ def main():
     ap.add_argument("--no-liquid", action="store_true", help="skip the liquid-and-sand leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel eagerly instead of through captured graphs")
+    ap.add_argument("--leg", choices=("batched", "batch", "adjoint", "liquid"),
+                    help="run only this leg and print its record (batch: the batched ensemble "
+                         "without the plain loop), for kernel profiles of one workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -292,6 +299,14 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": src.numel() * 4}))
+        return
+
+    if args.leg:
+        bw = max(2, args.warmup // 4)
+        legs = {"batched": lambda: batched_leg(args.batched_steps, bw),
+                "batch": lambda: batched_leg(args.batched_steps, bw, plain=False),
+                "adjoint": adjoint_leg, "liquid": liquid_leg}
+        print(json.dumps({args.leg: legs[args.leg]()}))
         return
 
     pipe, bodies, mats = build_pipeline()
