@@ -71,18 +71,18 @@ def units_per_launch(name, pipe, n):
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_galaxy400k.md")
-KERNEL_SYMBOL = {"kick": "k_kick<3>"}
 
 
 def measured_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the committed counter summary
-    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy of known size), or None."""
+    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy of known size), or None.
+    A templated kernel's row may carry its template argument (k_kick<3>)."""
     if not os.path.exists(TRAFFIC_SUMMARY):
         return None
-    symbol = KERNEL_SYMBOL.get(kernel, "k_" + kernel)
+    symbol = "k_" + kernel
     for line in open(TRAFFIC_SUMMARY):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == symbol:
+        if len(cells) == 5 and cells[0].split("<")[0] == symbol:
             return float(cells[4])
     return None
 
