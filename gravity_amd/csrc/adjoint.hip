@@ -126,7 +126,8 @@ template <int R>
 __device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __restrict__ cellRec,
                                               const float4* __restrict__ cellBody,
                                               const float2* __restrict__ xbar,
-                                              const float2* __restrict__ vbar, float dt, int g,
+                                              const float2* __restrict__ vbar,
+                                              const float2* __restrict__ wCell, float dt, int g,
                                               float4 p, float mass, float2 w, int myCount,
                                               int myRank, float& gx, float& gy) {
 #pragma clang fp contract(fast)
@@ -194,12 +195,10 @@ __device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __r
             float4 o[D];
 #pragma unroll
             for (int k = 0; k < D; k++) o[k] = cellBody[need[k] > t ? rec[k].x + (uint32_t)t : 0u];
+            // The entries' adjoint weights by slot, loaded with the entries.
             float2 wk[D];
 #pragma unroll
-            for (int k = 0; k < D; k++) {
-                const bool use = need[k] > t && ((seenMask >> k) & 1u);
-                wk[k] = adjoint_w(xbar, vbar, use ? (int)__float_as_uint(o[k].w) : g, dt);
-            }
+            for (int k = 0; k < D; k++) wk[k] = wCell[need[k] > t ? rec[k].x + (uint32_t)t : 0u];
 #pragma unroll
             for (int k = 0; k < D; k++) {
                 if (need[k] <= t || __float_as_uint(o[k].w) == (uint32_t)g) continue;
@@ -231,12 +230,11 @@ __device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __r
                 float4 o[FURTHER_BATCH];
                 float2 wo[FURTHER_BATCH];
 #pragma unroll
-                for (int q = 0; q < FURTHER_BATCH; q++)
-                    o[q] = cellBody[rec[k].x + (uint32_t)min(t0 + q, need[k] - 1)];
-#pragma unroll
-                for (int q = 0; q < FURTHER_BATCH; q++)
-                    wo[q] = adjoint_w(xbar, vbar, (seenMask >> k) & 1u ? (int)__float_as_uint(o[q].w) : g,
-                                      dt);
+                for (int q = 0; q < FURTHER_BATCH; q++) {
+                    const uint32_t slot = rec[k].x + (uint32_t)min(t0 + q, need[k] - 1);
+                    o[q] = cellBody[slot];
+                    wo[q] = wCell[slot];
+                }
 #pragma unroll
                 for (int q = 0; q < FURTHER_BATCH; q++) {
                     const int t = t0 + q;
@@ -263,6 +261,26 @@ __device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __r
 
 // Per body: the gather's own-position term and the near-field terms of
 // J_A^T w; also the value the transposed gather spreads, (wx, -wy).
+// The adjoint weight w = vbar + dt * xbar of the body in every cell slot, so
+// the near-field VJP loads a neighbour's weight with its entry (by slot)
+// instead of through the entry's body index.
+__global__ void __launch_bounds__(WG) k_adj_cell_weights(const SimDesc* __restrict__ sims,
+                                                         const BodyBlock* __restrict__ blocks,
+                                                         const float4* __restrict__ cellBody,
+                                                         const float2* __restrict__ xbar,
+                                                         const float2* __restrict__ vbar, float dt,
+                                                         float2* __restrict__ wCell) {
+#pragma clang fp contract(fast)
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[b.sim];
+    int slot = b.first + (int)threadIdx.x;
+    if (slot - s.base >= s.n || !s.mutual) return;
+    // (the id is clamped to the simulation's bodies: every slot of a
+    // simulation with mutual gravity holds one of them)
+    const uint32_t id = min(__float_as_uint(cellBody[slot].w), (uint32_t)(s.base + s.n - 1));
+    wCell[slot] = adjoint_w(xbar, vbar, (int)id, dt);
+}
+
 __global__ void __launch_bounds__(WG) k_adj_direct(const SimDesc* __restrict__ sims,
                                                    const BodyBlock* __restrict__ blocks,
                                                    const float4* __restrict__ body,
@@ -271,7 +289,8 @@ __global__ void __launch_bounds__(WG) k_adj_direct(const SimDesc* __restrict__ s
                                                    const uint4* __restrict__ cellRec,
                                                    const float4* __restrict__ cellBody,
                                                    const float2* __restrict__ xbar,
-                                                   const float2* __restrict__ vbar, float dt,
+                                                   const float2* __restrict__ vbar,
+                                                   const float2* __restrict__ wCell, float dt,
                                                    float2* __restrict__ gbar,
                                                    float2* __restrict__ spread) {
 #pragma clang fp contract(fast)
@@ -318,8 +337,8 @@ __global__ void __launch_bounds__(WG) k_adj_direct(const SimDesc* __restrict__ s
                 if (t0 + q < myCount && id[q] == (uint32_t)g) myRank = t0 + q;
         }
         if (s.rx == 3 && s.ry == 3)
-            near_vjp_rows<3>(s, cellRec, cellBody, xbar, vbar, dt, g, p, mass, w, myCount, myRank,
-                             gx, gy);
+            near_vjp_rows<3>(s, cellRec, cellBody, xbar, vbar, wCell, dt, g, p, mass, w, myCount,
+                             myRank, gx, gy);
         else
             near_vjp_any(s, cellRec, cellBody, xbar, vbar, dt, g, p, mass, w, myCount, myRank, gx,
                          gy);
@@ -399,10 +418,12 @@ __global__ void __launch_bounds__(WG) k_adj_compose(const SimDesc* __restrict__ 
 void launch_adj_direct(hipStream_t st, const LaunchShape& ls, const float4* body,
                        const float4* mats, const float2* force, const uint4* cellRec,
                        const float4* cellBody, const float2* xbar, const float2* vbar, float dt,
-                       float2* gbar, float2* spread) {
+                       float2* wCell, float2* gbar, float2* spread) {
     if (ls.bodyBlocks == 0) return;
+    launch(k_adj_cell_weights, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, cellBody,
+           xbar, vbar, dt, wCell);
     launch(k_adj_direct, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       body, mats, force, cellRec, cellBody, xbar, vbar, dt, gbar, spread);
+                       body, mats, force, cellRec, cellBody, xbar, vbar, wCell, dt, gbar, spread);
 }
 
 void launch_adj_finish(hipStream_t st, const LaunchShape& ls, const float4* body,

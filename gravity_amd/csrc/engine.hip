@@ -823,7 +823,7 @@ void Engine::adjoint_free() {
     adjAllocs_.clear();
     adjCk_.clear();
     adjSlot_.clear();
-    adjX_ = adjV_ = adjX2_ = adjV2_ = adjG_ = adjSpread_ = nullptr;
+    adjX_ = adjV_ = adjX2_ = adjV2_ = adjG_ = adjSpread_ = adjW_ = nullptr;
     adjWhere_ = nullptr;
     adjFinal_ = AdjCheckpoint{};
 }
@@ -880,7 +880,7 @@ void Engine::adjoint_record(int steps, int every) {
     adjFinal_.body = (float4*)alloc(n * sizeof(float4));
     adjFinal_.acc = (float2*)alloc(n * sizeof(float2));
     adjFinal_.mats = (float4*)alloc(n * sizeof(float4));
-    for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_})
+    for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_, &adjW_})
         *p = (float2*)alloc(n * sizeof(float2));
     adjWhere_ = (uint32_t*)alloc(n * sizeof(uint32_t));
     std::vector<uint32_t> iota(n);
@@ -915,7 +915,7 @@ void Engine::debug_accel_vjp(const float* w, float* out) {
     const HostSim& h = sims_[0];
     const size_t cap = (size_t)h.cap, n = (size_t)h.n;
     if (!adjX_) {
-        for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_}) {
+        for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_, &adjW_}) {
             void* q = nullptr;
             if (hipMalloc(&q, std::max<size_t>(cap * sizeof(float2), 16)) != hipSuccess)
                 throw StateError("hipMalloc failed");
@@ -928,7 +928,7 @@ void Engine::debug_accel_vjp(const float* w, float* out) {
     check(hipMemcpyAsync(adjV_, w, n * sizeof(float2), hipMemcpyHostToDevice, st_), "vjp");
     enqueue_force();
     launch_adj_direct(st_, shape_, body_[bodyPar_], mat_[auxPar_], spec_, cellRec_, cellBody_,
-                      adjX_, adjV_, 0.0f, adjG_, adjSpread_);
+                      adjX_, adjV_, 0.0f, adjW_, adjG_, adjSpread_);
     launch_fft_rows_forward_values(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, adjSpread_,
                                    spec_, tw_, twLen_);
     launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_);
@@ -1006,7 +1006,7 @@ void Engine::adjoint_vjp(const float* dLdx, float* dLdx0, float* dLdv0) {
             }
             launch_adj_direct(st_, shape_, sl.body - h.base, sl.mats - h.base,
                               sl.force - h.desc.meshBase, sl.rec - h.desc.physBase,
-                              sl.cellBody, adjX_, adjV_, dt, adjG_, adjSpread_);
+                              sl.cellBody, adjX_, adjV_, dt, adjW_, adjG_, adjSpread_);
             launch_fft_rows_forward_values(st_, dSims_, dRowsF_, rowsF_, sl.prefix - h.desc.physBase,
                                            sl.cellBody, adjSpread_, spec_, tw_, twLen_);
             launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_);

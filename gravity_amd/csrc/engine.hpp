@@ -193,7 +193,7 @@ private:
     std::vector<char> adjSortDue_;
     int adjSteps_ = 0, adjEvery_ = 0;
     float2 *adjX_ = nullptr, *adjV_ = nullptr, *adjX2_ = nullptr, *adjV2_ = nullptr;
-    float2 *adjG_ = nullptr, *adjSpread_ = nullptr;
+    float2 *adjG_ = nullptr, *adjSpread_ = nullptr, *adjW_ = nullptr;
     uint32_t* adjWhere_ = nullptr;
     AdjCheckpoint adjFinal_;
     std::vector<void*> adjAllocs_;

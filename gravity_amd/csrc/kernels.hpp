@@ -124,10 +124,12 @@ void launch_fft_rows_forward_values(hipStream_t st, const SimDesc* sims, const L
                                     int items, const uint32_t* cellPrefix, const float4* cellBody,
                                     const float2* value, float2* spec, const float2* twiddles,
                                     int twiddleLength);
+// wCell receives the adjoint weight of the body in every cell slot (scratch
+// of the near-field VJP, one float2 per body slot).
 void launch_adj_direct(hipStream_t st, const LaunchShape& ls, const float4* body,
                        const float4* mats, const float2* force, const uint4* cellRec,
                        const float4* cellBody, const float2* xbar, const float2* vbar, float dt,
-                       float2* gbar, float2* spread);
+                       float2* wCell, float2* gbar, float2* spread);
 void launch_adj_finish(hipStream_t st, const LaunchShape& ls, const float4* body,
                        const float4* mats, const float2* result, const float2* gbar, float2* xbar,
                        float2* vbar, float dt, float coef);
