@@ -17,7 +17,7 @@ class GravityBatch:
     """Simulation i has configs[i], viewports[i] = (width, height) in px,
     walls if bounded[i], and starts from scenarioStates[i] (an (n, 4) array of
     normalised x, y, vx, vy, or None). All members share dt, the substep
-    count and the sort interval."""
+    count, the sort interval and collisionsEnabled."""
 
     def __init__(self, configs: Sequence[GravityConfig], viewports: Sequence[tuple[int, int]],
                  scenarioStates: Sequence[np.ndarray | None] | None = None,

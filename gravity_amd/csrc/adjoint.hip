@@ -263,7 +263,10 @@ __device__ __forceinline__ void near_vjp_rows(const SimDesc& s, const uint4* __r
 // J_A^T w; also the value the transposed gather spreads, (wx, -wy).
 // The adjoint weight w = vbar + dt * xbar of the body in every cell slot, so
 // the near-field VJP loads a neighbour's weight with its entry (by slot)
-// instead of through the entry's body index.
+// instead of through the entry's body index. Cell slots count the bodies of
+// the simulations with mutual gravity only while body indices count every
+// simulation's capacity: the two agree for the one simulation starting at
+// body 0 that the engine's adjoint entry points require.
 __global__ void __launch_bounds__(WG) k_adj_cell_weights(const SimDesc* __restrict__ sims,
                                                          const BodyBlock* __restrict__ blocks,
                                                          const float4* __restrict__ cellBody,
@@ -402,6 +405,17 @@ __global__ void __launch_bounds__(WG) k_adj_unsort(const SimDesc* __restrict__ s
     vout[g] = vin[d];
 }
 
+// The identity permutation of the recording's start: where[g] = g.
+__global__ void __launch_bounds__(WG) k_adj_iota(const SimDesc* __restrict__ sims,
+                                                 const BodyBlock* __restrict__ blocks,
+                                                 uint32_t* __restrict__ where) {
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[b.sim];
+    int g = b.first + (int)threadIdx.x;
+    if (g - s.base >= s.n) return;
+    where[g] = (uint32_t)g;
+}
+
 // Composes a sort into the running permutation: where[i] is the index now
 // holding the body that started at index i.
 __global__ void __launch_bounds__(WG) k_adj_compose(const SimDesc* __restrict__ sims,
@@ -439,6 +453,11 @@ void launch_adj_unsort(hipStream_t st, const LaunchShape& ls, const uint32_t* de
     if (ls.bodyBlocks == 0) return;
     launch(k_adj_unsort, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        dest, xin, vin, xout, vout);
+}
+
+void launch_adj_iota(hipStream_t st, const LaunchShape& ls, uint32_t* where) {
+    if (ls.bodyBlocks == 0) return;
+    launch(k_adj_iota, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, where);
 }
 
 void launch_adj_compose(hipStream_t st, const LaunchShape& ls, const uint32_t* dest,

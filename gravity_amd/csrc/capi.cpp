@@ -230,16 +230,16 @@ int gs_adjoint_record(gs_sim* sim, int active, int steps, int checkpoint_every) 
     });
 }
 
-int gs_adjoint_vjp(gs_sim* sim, const float* dL_dx, float* dL_dx0, float* dL_dv0) {
+int gs_adjoint_vjp(gs_sim* sim, int count, const float* dL_dx, float* dL_dx0, float* dL_dv0) {
     SIM_OR_FAIL
     if (!dL_dx) return fail(GS_EINVAL, "null gradient");
-    return guarded([&] { e.adjoint_vjp(dL_dx, dL_dx0, dL_dv0); });
+    return guarded([&] { e.adjoint_vjp(count, dL_dx, dL_dx0, dL_dv0); });
 }
 
-int gs_adjoint_order(gs_sim* sim, uint32_t* where) {
+int gs_adjoint_order(gs_sim* sim, int count, uint32_t* where) {
     SIM_OR_FAIL
     if (!where) return fail(GS_EINVAL, "null destination");
-    return guarded([&] { e.adjoint_order(where); });
+    return guarded([&] { e.adjoint_order(count, where); });
 }
 
 int gs_vjp_initial_velocity(gs_sim* sim, int active, int steps, int checkpoint_every,
@@ -254,7 +254,7 @@ int gs_vjp_initial_velocity(gs_sim* sim, int active, int steps, int checkpoint_e
         std::vector<float> xyvv(4 * n), grad(2 * n, 0.0f);
         std::vector<uint32_t> where(n);
         e.download(0, GS_BODIES_XYVV, 0, (int)n, xyvv.data());
-        e.adjoint_order(where.data());
+        e.adjoint_order((int)n, where.data());
         // L = mean |minimum image((x_T - target) * viewport)|^2, px^2
         const double W = h.cfg.viewport_width, H = h.cfg.viewport_height;
         double sum = 0.0;
@@ -271,7 +271,7 @@ int gs_vjp_initial_velocity(gs_sim* sim, int active, int steps, int checkpoint_e
             grad[2 * f + 1] = (float)(2.0 * dy * H / (double)n);
         }
         if (loss) *loss = n ? (float)(sum / (double)n) : 0.0f;
-        e.adjoint_vjp(grad.data(), nullptr, dL_dv0);
+        e.adjoint_vjp((int)n, grad.data(), nullptr, dL_dv0);
     });
 }
 

@@ -92,6 +92,10 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
         if (c.maximum_substeps != substeps || c.time_step != cfgs[0].time_step ||
             c.spatial_sort_interval != cfgs[0].spatial_sort_interval)
             throw ArgumentError("a batch shares dt, substeps and sort interval");
+        // The step schedule is either contact passes or plain drifts for the
+        // whole batch (no kernel reads a member's own choice).
+        if ((c.collisions_enabled != 0) != (cfgs[0].collisions_enabled != 0))
+            throw ArgumentError("a batch shares collisionsEnabled");
     }
     for (const auto& c : cfgs)
         for (const auto& o : cfgs)
@@ -266,6 +270,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
 Engine::~Engine() {
     invalidate_graphs();
     adjoint_free();
+    for (void* p : adjWork_) hipFree(p);
     for (auto& e : events_) {
         hipEventDestroy(e.second.first);
         hipEventDestroy(e.second.second);
@@ -823,9 +828,24 @@ void Engine::adjoint_free() {
     adjAllocs_.clear();
     adjCk_.clear();
     adjSlot_.clear();
-    adjX_ = adjV_ = adjX2_ = adjV2_ = adjG_ = adjSpread_ = adjW_ = nullptr;
     adjWhere_ = nullptr;
     adjFinal_ = AdjCheckpoint{};
+    adjN_ = -1;
+    for (long long& v : adjShape_) v = -1;
+}
+
+// The per-body work arrays of the reversed leapfrog and of the VJP test hook,
+// allocated on first use and kept until the engine is destroyed.
+void Engine::adjoint_work() {
+    if (adjX_) return;
+    const size_t cap = (size_t)sims_[0].cap;
+    for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_, &adjW_}) {
+        void* q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(cap * sizeof(float2), 16)) != hipSuccess)
+            throw StateError("hipMalloc failed");
+        adjWork_.push_back(q);
+        *p = (float2*)q;
+    }
 }
 
 void Engine::adjoint_save_slot(int j) {
@@ -843,8 +863,15 @@ void Engine::adjoint_save_slot(int j) {
     copy(sl.cellBody, cellBody_, (size_t)gravitySlots_ * sizeof(float4));
 }
 
+// The adjoint kernels index their per-body arrays like the bodies and read
+// the cell tables of the simulation's first cell slot: they serve one
+// simulation whose bodies start at global index 0.
+static void require_single_base0(const std::vector<HostSim>& sims, const char* what) {
+    if (sims.size() != 1 || sims[0].base != 0) throw ArgumentError(what);
+}
+
 void Engine::adjoint_record(int steps, int every) {
-    if (sims_.size() != 1) throw ArgumentError("the adjoint runs on a single simulation");
+    require_single_base0(sims_, "the adjoint runs on a single simulation");
     const HostSim& h = sims_[0];
     if (h.cfg.collisions_enabled) throw ArgumentError("the adjoint differentiates the gravity step without contact passes");
     if (!h.mutual) throw ArgumentError("the adjoint needs mutual gravity");
@@ -852,43 +879,47 @@ void Engine::adjoint_record(int steps, int every) {
     if (steps < 1) throw ArgumentError("steps must be positive");
     if (every < 1) throw ArgumentError("checkpoint interval must be positive");
     normalize();
-    adjoint_free();
+    adjoint_work();
     size_t n = (size_t)h.cap;
-    auto alloc = [&](size_t bytes) {
-        void* p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) throw StateError("hipMalloc failed");
-        adjAllocs_.push_back(p);
-        return p;
-    };
-    int segments = (steps + every - 1) / every;
-    adjCk_.resize((size_t)segments);
-    for (auto& c : adjCk_) {
-        c.body = (float4*)alloc(n * sizeof(float4));
-        c.acc = (float2*)alloc(n * sizeof(float2));
-        c.mats = (float4*)alloc(n * sizeof(float4));
+    const long long shape[4] = {(long long)h.cap, steps, every, gravitySlots_};
+    if (!std::equal(shape, shape + 4, adjShape_)) {
+        // A new shape: the checkpoints, the per-step force builds of one
+        // interval and the final state are (re)allocated once for it.
+        adjoint_free();
+        auto alloc = [&](size_t bytes) {
+            void* p = nullptr;
+            if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) throw StateError("hipMalloc failed");
+            adjAllocs_.push_back(p);
+            return p;
+        };
+        int segments = (steps + every - 1) / every;
+        adjCk_.resize((size_t)segments);
+        for (auto& c : adjCk_) {
+            c.body = (float4*)alloc(n * sizeof(float4));
+            c.acc = (float2*)alloc(n * sizeof(float2));
+            c.mats = (float4*)alloc(n * sizeof(float4));
+        }
+        adjSlot_.resize((size_t)every);
+        for (auto& sl : adjSlot_) {
+            sl.body = (float4*)alloc(n * sizeof(float4));
+            sl.mats = (float4*)alloc(n * sizeof(float4));
+            sl.force = (float2*)alloc((size_t)h.meshMax * sizeof(float2));
+            sl.rec = (uint4*)alloc((size_t)h.phys * sizeof(uint4));
+            sl.prefix = (uint32_t*)alloc(((size_t)h.phys + 1) * sizeof(uint32_t));
+            sl.cellBody = (float4*)alloc((size_t)std::max(gravitySlots_, 1) * sizeof(float4));
+            sl.dest = (uint32_t*)alloc(n * sizeof(uint32_t));
+        }
+        adjFinal_.body = (float4*)alloc(n * sizeof(float4));
+        adjFinal_.acc = (float2*)alloc(n * sizeof(float2));
+        adjFinal_.mats = (float4*)alloc(n * sizeof(float4));
+        adjWhere_ = (uint32_t*)alloc(n * sizeof(uint32_t));
+        std::copy(shape, shape + 4, adjShape_);
     }
-    adjSlot_.resize((size_t)every);
-    for (auto& sl : adjSlot_) {
-        sl.body = (float4*)alloc(n * sizeof(float4));
-        sl.mats = (float4*)alloc(n * sizeof(float4));
-        sl.force = (float2*)alloc((size_t)h.meshMax * sizeof(float2));
-        sl.rec = (uint4*)alloc((size_t)h.phys * sizeof(uint4));
-        sl.prefix = (uint32_t*)alloc(((size_t)h.phys + 1) * sizeof(uint32_t));
-        sl.cellBody = (float4*)alloc((size_t)std::max(gravitySlots_, 1) * sizeof(float4));
-        sl.dest = (uint32_t*)alloc(n * sizeof(uint32_t));
-    }
-    adjFinal_.body = (float4*)alloc(n * sizeof(float4));
-    adjFinal_.acc = (float2*)alloc(n * sizeof(float2));
-    adjFinal_.mats = (float4*)alloc(n * sizeof(float4));
-    for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_, &adjW_})
-        *p = (float2*)alloc(n * sizeof(float2));
-    adjWhere_ = (uint32_t*)alloc(n * sizeof(uint32_t));
-    std::vector<uint32_t> iota(n);
-    for (size_t i = 0; i < n; i++) iota[i] = (uint32_t)i;
-    check(hipMemcpy(adjWhere_, iota.data(), n * 4, hipMemcpyHostToDevice), "adjoint order");
+    launch_adj_iota(st_, shape_, adjWhere_);
     adjSortDue_.assign((size_t)steps, 0);
     adjSteps_ = steps;
     adjEvery_ = every;
+    adjN_ = h.n;
     int interval = h.cfg.spatial_sort_interval;
     for (int k = 0; k < steps; k++) {
         if (k % every == 0) {
@@ -910,19 +941,12 @@ void Engine::adjoint_record(int steps, int every) {
 }
 
 void Engine::debug_accel_vjp(const float* w, float* out) {
-    if (sims_.size() != 1 || !sims_[0].mutual) throw ArgumentError("one simulation with mutual gravity");
+    require_single_base0(sims_, "one simulation with mutual gravity");
+    if (!sims_[0].mutual) throw ArgumentError("one simulation with mutual gravity");
     normalize();
     const HostSim& h = sims_[0];
     const size_t cap = (size_t)h.cap, n = (size_t)h.n;
-    if (!adjX_) {
-        for (float2** p : {&adjX_, &adjV_, &adjX2_, &adjV2_, &adjG_, &adjSpread_, &adjW_}) {
-            void* q = nullptr;
-            if (hipMalloc(&q, std::max<size_t>(cap * sizeof(float2), 16)) != hipSuccess)
-                throw StateError("hipMalloc failed");
-            adjAllocs_.push_back(q);
-            *p = (float2*)q;
-        }
-    }
+    adjoint_work();
     check(hipMemsetAsync(adjX_, 0, cap * sizeof(float2), st_), "vjp");
     check(hipMemsetAsync(adjV_, 0, cap * sizeof(float2), st_), "vjp");
     check(hipMemcpyAsync(adjV_, w, n * sizeof(float2), hipMemcpyHostToDevice, st_), "vjp");
@@ -939,16 +963,19 @@ void Engine::debug_accel_vjp(const float* w, float* out) {
     check(hipStreamSynchronize(st_), "vjp");
 }
 
-void Engine::adjoint_order(uint32_t* where) {
-    if (!adjWhere_) throw StateError("no adjoint recording");
-    check(hipMemcpyAsync(where, adjWhere_, (size_t)sims_[0].n * 4, hipMemcpyDeviceToHost, st_),
+void Engine::adjoint_order(int count, uint32_t* where) {
+    if (!adjWhere_ || adjN_ < 0) throw StateError("no adjoint recording");
+    if (count != adjN_) throw ArgumentError("count differs from the recording's body count");
+    check(hipMemcpyAsync(where, adjWhere_, (size_t)adjN_ * 4, hipMemcpyDeviceToHost, st_),
           "adjoint order");
     check(hipStreamSynchronize(st_), "adjoint order");
 }
 
-void Engine::adjoint_vjp(const float* dLdx, float* dLdx0, float* dLdv0) {
-    if (adjCk_.empty()) throw StateError("no adjoint recording");
+void Engine::adjoint_vjp(int count, const float* dLdx, float* dLdx0, float* dLdv0) {
+    if (adjCk_.empty() || adjN_ < 0) throw StateError("no adjoint recording");
+    if (count != adjN_) throw ArgumentError("count differs from the recording's body count");
     HostSim& h = sims_[0];
+    if (h.n != adjN_) throw StateError("the active body count changed since the recording");
     const size_t n = (size_t)h.n, cap = (size_t)h.cap;
     const float dt = h.cfg.time_step;
     const int S = h.substeps;

@@ -80,10 +80,14 @@ public:
     // Adjoint of the gravity step (one simulation, drift-only passes):
     // `steps` steps like step(), keeping the state every `every` steps, then
     // the vector-Jacobian product of those steps by checkpointed recomputation.
+    // Buffers are allocated on the first recording of a shape (body
+    // capacity, steps, interval) and kept for the next ones.
     void adjoint_record(int steps, int every);
-    void adjoint_vjp(const float* dLdx, float* dLdx0, float* dLdv0);
+    // count must be the recording's body count (dLdx and the outputs hold
+    // 2 * count floats).
+    void adjoint_vjp(int count, const float* dLdx, float* dLdx0, float* dLdv0);
     // where[i]: the index now holding the body that started the recording at i
-    void adjoint_order(uint32_t* where);
+    void adjoint_order(int count, uint32_t* where);
     // J_A(x)^T w of the acceleration field at the current positions
     void debug_accel_vjp(const float* w, float* out);
     void debug_collide(float kickDt, float driftDt, bool writePressure);
@@ -192,12 +196,16 @@ private:
     std::vector<AdjSlot> adjSlot_;
     std::vector<char> adjSortDue_;
     int adjSteps_ = 0, adjEvery_ = 0;
+    int adjN_ = -1;              // bodies of the last recording (-1: none)
+    long long adjShape_[4] = {-1, -1, -1, -1};  // what the recording buffers were sized for
     float2 *adjX_ = nullptr, *adjV_ = nullptr, *adjX2_ = nullptr, *adjV2_ = nullptr;
     float2 *adjG_ = nullptr, *adjSpread_ = nullptr, *adjW_ = nullptr;
     uint32_t* adjWhere_ = nullptr;
     AdjCheckpoint adjFinal_;
-    std::vector<void*> adjAllocs_;
+    std::vector<void*> adjAllocs_;   // recording buffers (checkpoints, slots, final state)
+    std::vector<void*> adjWork_;     // per-body work arrays, allocated once
     void adjoint_free();
+    void adjoint_work();
     void adjoint_save_slot(int j);
 
     // profiling

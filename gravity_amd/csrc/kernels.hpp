@@ -137,6 +137,7 @@ void launch_adj_unsort(hipStream_t st, const LaunchShape& ls, const uint32_t* de
                        const float2* xin, const float2* vin, float2* xout, float2* vout);
 void launch_adj_compose(hipStream_t st, const LaunchShape& ls, const uint32_t* dest,
                         uint32_t* where);
+void launch_adj_iota(hipStream_t st, const LaunchShape& ls, uint32_t* where);
 
 // ---- position-based liquid ----
 void launch_liquid_bin(hipStream_t st, const LaunchShape& ls, const float4* body,

@@ -99,8 +99,8 @@ SIGNATURES = {
     "gs_batch_initialize_acceleration": (_I, [_P]),
     "gs_batch_step": (_I, [_P]),
     "gs_adjoint_record": (_I, [_P, _I, _I, _I]),
-    "gs_adjoint_vjp": (_I, [_P, _P, _P, _P]),
-    "gs_adjoint_order": (_I, [_P, _P]),
+    "gs_adjoint_vjp": (_I, [_P, _I, _P, _P, _P]),
+    "gs_adjoint_order": (_I, [_P, _I, _P]),
     "gs_vjp_initial_velocity": (_I, [_P, _I, _I, _I, _P, ctypes.POINTER(_F), _P]),
     "gs_debug_acceleration_vjp": (_I, [_P, _I, _P, _P]),
     "gs_last_error": (ctypes.c_char_p, []),

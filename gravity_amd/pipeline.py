@@ -275,23 +275,26 @@ class GravityPipeline:
                                                  int(checkpointEvery)))
 
     def adjointVjp(self, dLdx: np.ndarray, count: int) -> tuple[np.ndarray, np.ndarray]:
-        """(dL/dx0, dL/dv0) of the recorded steps for dL/dx at their end."""
-        g = np.ascontiguousarray(dLdx, dtype=np.float32).reshape(-1)
+        """(dL/dx0, dL/dv0) of the recorded steps for dL/dx at their end;
+        count is the recording's active body count."""
+        g = self._floats(dLdx, 2 * count, "dLdx")
         x0 = np.empty(2 * count, np.float32)
         v0 = np.empty(2 * count, np.float32)
-        native.check(self._lib.gs_adjoint_vjp(self._h, native.ptr(g), native.ptr(x0), native.ptr(v0)))
+        native.check(self._lib.gs_adjoint_vjp(self._h, int(count), native.ptr(g), native.ptr(x0),
+                                              native.ptr(v0)))
         return x0.reshape(count, 2), v0.reshape(count, 2)
 
     def adjointOrder(self, count: int) -> np.ndarray:
-        out = np.empty(count, np.uint32)
-        native.check(self._lib.gs_adjoint_order(self._h, native.ptr(out)))
+        out = np.empty(max(int(count), 0), np.uint32)
+        native.check(self._lib.gs_adjoint_order(self._h, int(count), native.ptr(out)))
         return out
 
     def vjpInitialVelocity(self, activeBodies: int, steps: int, checkpointEvery: int,
                            targets: np.ndarray) -> tuple[float, np.ndarray]:
         """Loss mean |x_T - target|^2 (px^2, minimum image) and its gradient
         with respect to the initial normalised velocities."""
-        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        self._check_active(activeBodies)
+        t = self._floats(targets, 2 * activeBodies, "targets")
         grad = np.empty(2 * activeBodies, np.float32)
         loss = ctypes.c_float()
         native.check(self._lib.gs_vjp_initial_velocity(self._h, int(activeBodies), int(steps),
@@ -301,11 +304,23 @@ class GravityPipeline:
 
     def accelerationVjp(self, activeBodies: int, w: np.ndarray) -> np.ndarray:
         """J_A(x)^T w at the current positions (test hook of the adjoint)."""
-        wv = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+        self._check_active(activeBodies)
+        wv = self._floats(w, 2 * activeBodies, "w")
         out = np.empty(2 * activeBodies, np.float32)
         native.check(self._lib.gs_debug_acceleration_vjp(self._h, int(activeBodies), native.ptr(wv),
                                                          native.ptr(out)))
         return out.reshape(activeBodies, 2)
+
+    def _check_active(self, activeBodies: int) -> None:
+        if not 0 <= int(activeBodies) <= self.config.particles.capacity:
+            raise ValueError("active body count out of range")
+
+    @staticmethod
+    def _floats(a: np.ndarray, need: int, name: str) -> np.ndarray:
+        v = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+        if v.size < need:
+            raise ValueError(f"{name} holds {v.size} floats, {need} needed")
+        return v
 
     def resolveLiquid(self, activeBodies: int) -> None:
         """The liquid solve of a step on its own (the reflection hook

@@ -205,10 +205,15 @@ int gs_batch_step(gs_sim *sim);
  * respect to the normalised positions and velocities of the recording's
  * start, in its body order; the handle keeps the final state. The spatial
  * sorts of the recording are undone on the way back. gs_adjoint_order:
- * where[i] = the index that now holds the body that started at index i. */
+ * where[i] = the index that now holds the body that started at index i.
+ * count must equal the recording's active body count (GS_EINVAL otherwise),
+ * and every array holds 2 * count floats (where: count entries); a VJP after
+ * the active count has changed fails with GS_ESTATE. The first recording of
+ * a shape (capacity, steps, interval) allocates its checkpoint buffers; later
+ * recordings of the same shape reuse them. */
 int gs_adjoint_record(gs_sim *sim, int active, int steps, int checkpoint_every);
-int gs_adjoint_vjp(gs_sim *sim, const float *dL_dx, float *dL_dx0, float *dL_dv0);
-int gs_adjoint_order(gs_sim *sim, uint32_t *where);
+int gs_adjoint_vjp(gs_sim *sim, int count, const float *dL_dx, float *dL_dx0, float *dL_dv0);
+int gs_adjoint_order(gs_sim *sim, int count, uint32_t *where);
 /* Record + loss L = mean |minimum image((x_T - target) * viewport)|^2 (px^2,
  * targets float[2 * n] normalised, in the start order) + VJP: writes L and
  * dL/dv_0 (float[2 * n], normalised units, start order). */

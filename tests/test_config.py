@@ -130,3 +130,13 @@ def test_batch_rejects_mixed_time_steps_before_touching_the_gpu():
                       collisions=CollisionConfig(timeStep=1.0 / 60.0))
     with pytest.raises(ValueError, match="shares dt"):
         GravityBatch([a, b], [(256, 256), (256, 256)])
+
+
+def test_batch_rejects_mixed_collision_schedules_before_touching_the_gpu():
+    # Every member steps through the same schedule: contact passes for all
+    # or plain drifts for all.
+    from gravity_amd import GravityBatch, GravityConfig, ParticleConfig
+    a = GravityConfig(particles=ParticleConfig(capacity=100, initialBodies=1))
+    b = GravityConfig(particles=ParticleConfig(capacity=100, initialBodies=1), collisionsEnabled=False)
+    with pytest.raises(ValueError, match="shares collisionsEnabled"):
+        GravityBatch([a, b], [(256, 256), (256, 256)])
