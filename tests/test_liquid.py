@@ -129,6 +129,66 @@ def test_coincident_bodies_are_pushed_apart_keeping_momentum(oracle):
     assert np.abs(v.sum(axis=0)).max() < 1e-3 * np.abs(v).sum()
 
 
+def rest_density_fp64(diameter, radius_diameters):
+    """rho_0 = sum of poly6 over the hexagonal packing, self included
+    (K/LiquidKernel.kt:14-60), summed here in FP64 over a lattice large
+    enough to cover the kernel."""
+    h = diameter * radius_diameters
+    poly6 = 4.0 / (math.pi * h ** 8)
+    total = 0.0
+    for r in range(-12, 13):
+        shift = 0.0 if r % 2 == 0 else 0.5 * diameter
+        for c in range(-12, 13):
+            x, y = c * diameter + shift, r * diameter * math.sqrt(3.0) * 0.5
+            q = h * h - (x * x + y * y)
+            total += poly6 * q ** 3 if q > 0 else 0.0
+    return total
+
+
+@pytest.mark.parametrize("r_over_h", [0.3, 0.62, 0.95])
+def test_xsph_viscosity_of_a_pair_is_the_closed_form(oracle, r_over_h):
+    # S/liquid_viscosity.comp:27-55: two fluid bodies closer than h, too
+    # sparse for any density correction (lambda = 0), exchange velocity by
+    # v1' = v1 + c (v2 - v1) W(r) / rho_0 and symmetrically, so the pair's
+    # momentum is kept and the relative velocity shrinks by 1 - 2 c W / rho_0.
+    W = H = 200
+    c = 0.1
+    h = 2.0 * D
+    r = r_over_h * h
+    xy = np.array([[100.0, 100.0], [100.0 + r * 0.6, 100.0 + r * 0.8]])
+    vel = np.array([[30.0, -10.0], [-20.0, 25.0]])
+    b = pack(xy, W, H, False, vel)
+    got, _ = solve(oracle, liquid_params(oracle, W, H, False, viscosity=c), b, fluid(2))
+    np.testing.assert_array_equal(got[:, :2], b[:, :2])  # no density correction moved them
+    # the separation the FP32 positions actually hold
+    dx = (float(b[1, 0]) - float(b[0, 0])) * W
+    dy = (float(b[1, 1]) - float(b[0, 1])) * H
+    w = 4.0 / (math.pi * h ** 8) * (h * h - (dx * dx + dy * dy)) ** 3
+    k = c * w / rest_density_fp64(D, 2.0)
+    v = np.stack([got[:, 2] * W, got[:, 3] * H], axis=1).astype(np.float64)
+    v0 = np.stack([b[:, 2] * np.float32(W), b[:, 3] * np.float32(H)], axis=1).astype(np.float64)
+    want = v0 + k * (v0[::-1] - v0)
+    change = np.abs(want - v0).max()
+    assert change > 1e-3
+    assert np.abs(v - want).max() < 2e-4 * change + 1e-5
+    assert np.abs(v.sum(0) - v0.sum(0)).max() < 1e-4
+
+
+def test_xsph_skips_grains_and_bodies_beyond_the_kernel(oracle):
+    # A fluid body next to a solid grain and a fluid body just outside h keeps
+    # its velocity; two fluid bodies inside h still exchange.
+    W = H = 200
+    h = 2.0 * D
+    xy = np.array([[50.0, 50.0], [50.0 + 0.5 * h, 50.0], [120.0, 50.0], [120.0 + 1.01 * h, 50.0]])
+    vel = np.array([[10.0, 0.0], [-10.0, 0.0], [5.0, 5.0], [-5.0, -5.0]])
+    b = pack(xy, W, H, False, vel)
+    m = fluid(4)
+    m[1, 2] = 1.0   # a grain
+    m[1, 3] = -1.0
+    got, _ = solve(oracle, liquid_params(oracle, W, H, False, viscosity=0.1), b, m)
+    np.testing.assert_array_equal(got[:, 2:], b[:, 2:])
+
+
 # --------------------------------------------------------------------------
 # HIP passes against the oracle (bit-exact)
 

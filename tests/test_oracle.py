@@ -189,16 +189,6 @@ def test_fft_convolution_is_direct_circular_convolution(oracle):
     assert np.abs(conv.real - direct).max() < 1e-9
 
 
-def test_fp32_density_spectrum_error(oracle):
-    # The FP32 radix-2 path on a density-like field: the reference measured
-    # 6.35e-8 relative (docs/performance-fft.md:49) at 512 x 256.
-    rng = np.random.default_rng(5)
-    dens = rng.poisson(0.4, (256, 512)).astype(np.float64)
-    got = oracle.fft2d(dens + 0j, f64=False)
-    ref = np.fft.fft2(dens)
-    assert np.abs(got - ref).max() / np.abs(ref).max() < 2e-7
-
-
 def test_twiddles_are_rounded_unit_roots(oracle):
     # S/twiddle_init.comp:15-24 forms the angle -TAU * id / (2 h) in FP32, so
     # the table is cos/sin of that FP32 angle (within 1 ulp) and deviates from
