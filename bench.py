@@ -33,15 +33,15 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per launch of each kernel, per unit it processes: the
 # logical FP32 SoA arrays a launch must read and write once (neighbour
 # gathers and list traffic excluded).
+#   solve : one contact pass of the byte model, read x, y, vx, vy, mass
+#           (20 B), write x, y, vx, vy (16 B)                       36 B/body
 #   count : read x, y (8 B), write the contact count (4 B)            12 B/body
-#   solve : read x, y, vx, vy, mass, friction, count (28 B),
-#           write x, y, vx, vy (16 B)                               44 B/body
 #   kick  : read x, y, vx, vy (16 B), write vx, vy, ax, ay (16 B)   32 B/body
 #   fft_columns : read + write one complex mesh value (16 B) and
 #                 read the spectral kernel (8 B)                    24 B/node
 #   fft_rows_forward : write one complex mesh value (8 B)            8 B/node
 #   fft_rows_inverse : read + write one complex mesh value (16 B)   16 B/node
-BYTES_PER_UNIT = {"count": 12, "solve": 44, "kick": 32, "drift_insert": 32, "fft_columns": 24,
+BYTES_PER_UNIT = {"count": 12, "solve": 36, "kick": 32, "drift_insert": 32, "fft_columns": 24,
                   "fft_rows_forward": 8, "fft_rows_inverse": 16}
 
 
@@ -158,6 +158,78 @@ def batched_leg(steps, warmup, plain=True):
     return record
 
 
+def p3m_only_leg(steps=100, warmup=10):
+    """The P3M-only workload: 50,000 bodies uniform on a 1440x720 torus
+    (velocities 5 px/s Gaussian, seed 1), grid 512x256, mutual gravity,
+    no contact passes, dt 1/120 s; `steps` graph-replayed steps."""
+    from gravity_amd import scenarios
+    from gravity_amd import CollisionConfig, GravityConfig, GravityFieldConfig, GravityPipeline, ParticleConfig
+    n, W, H = 50_000, 1440, 720
+    bodies, mats = scenarios.uniform(n, W, H, seed=1)
+    cfg = GravityConfig(particles=ParticleConfig(capacity=n, initialBodies=n, seed=1),
+                        gravity=GravityFieldConfig(gridWidth=512, gridHeight=256),
+                        collisions=CollisionConfig(maximumSubsteps=1), collisionsEnabled=False)
+    with GravityPipeline(cfg, W, H, scenarioState=bodies) as pipe:
+        pipe.upload(bodies, mats)
+        pipe.initializeAcceleration(n)
+        for _ in range(warmup):
+            pipe.step(n)
+        pipe.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pipe.step(n)
+        pipe.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        finite = bool(np.isfinite(pipe.bodies(n)).all())
+    return {"workload": "p3m-50k: 50,000 bodies uniform on a 1440x720 torus, grid 512x256, "
+                        "gravity only, dt 1/120, seed 1",
+            "bodies": n, "steps": steps, "ms_per_step": t * 1e3, "value": n / t,
+            "unit": "body-steps/s", "finite": finite}
+
+
+def force_error_leg(sample=1000):
+    """FP64 direct-summation force error of the HIP path and of the oracle on
+    the galaxy workload's initial state (walls: the isolated 2D sum
+    a_i = sum_j G m_j d / (M max(r^2, r_min^2)), x scaled by the aspect),
+    over `sample` seeded bodies: RMS of |a - a_ref| over RMS |a_ref|."""
+    from oracle import oracle as orc
+    pipe, bodies, mats = build_pipeline()
+    n = len(bodies)
+    gpu = pipe.acceleration(n).astype(np.float64)
+    pipe.close()
+    sim = orc.Sim(orc.params(WIDTH, HEIGHT, bounded=True, grid=GRID, diameter=1.0, substeps=SUBSTEPS),
+                  bodies, mats)
+    sim.initialize_acceleration()
+    cpu = sim.state()[1].astype(np.float64)
+    sim.close()
+    aspect = WIDTH / HEIGHT
+    cw, ch = aspect / GRID[0], 1.0 / GRID[1]
+    min_d2 = (0.05 * min(cw, ch)) ** 2
+    pts = bodies[:, :2].astype(np.float64) * np.array([aspect, 1.0])
+    m = mats[:, 0].astype(np.float64)
+    pick = np.random.default_rng(11).choice(n, sample, replace=False)
+    ref = np.zeros((sample, 2))
+    for a in range(0, sample, 50):
+        rows = pick[a:a + 50]
+        d = pts[None, :, :] - pts[rows, None, :]
+        r2 = np.maximum((d * d).sum(-1), min_d2)
+        r2[np.arange(len(rows)), rows] = np.inf
+        ref[a:a + 50] = 0.035 / n * ((m / r2)[..., None] * d).sum(1)
+    ref /= np.array([aspect, 1.0])
+
+    def rms(a):
+        e = (a[pick] - ref) * np.array([aspect, 1.0])
+        r = ref * np.array([aspect, 1.0])
+        return float(np.sqrt((e ** 2).sum(1).mean() / (r ** 2).sum(1).mean()))
+
+    diff = (gpu - cpu) * np.array([aspect, 1.0])
+    return {"workload": "galaxy-400k initial state, walls, grid 1024x512",
+            "reference": "FP64 direct sum (isolated, minimum-distance floor)", "sample": sample,
+            "hip_rms_relative": rms(gpu), "oracle_rms_relative": rms(cpu),
+            "hip_vs_oracle_max_relative": float(np.abs(diff).max() /
+                                                np.abs(cpu * np.array([aspect, 1.0])).max())}
+
+
 def adjoint_leg(steps=64, every=8, warmup=5):
     """The adjoint workload: 100,000 bodies uniform on a 1440x720 torus, P3M
     grid 512x256, gravity only; 64 leapfrog steps recorded with a checkpoint
@@ -184,6 +256,15 @@ def adjoint_leg(steps=64, every=8, warmup=5):
             pipe.step(n)
         pipe.synchronize()
         t_fwd = (time.perf_counter() - t0) / steps
+        # The first recording of this shape allocates the checkpoints; the
+        # timed one reuses them.
+        pipe.upload(bodies, mats)
+        pipe.initializeAcceleration(n)
+        pipe.synchronize()
+        t0 = time.perf_counter()
+        pipe.adjointRecord(n, steps, every)
+        pipe.synchronize()
+        t_setup = time.perf_counter() - t0
         pipe.upload(bodies, mats)
         pipe.initializeAcceleration(n)
         pipe.synchronize()
@@ -205,6 +286,7 @@ def adjoint_leg(steps=64, every=8, warmup=5):
                         "gravity only, 64 steps, checkpoint every 8, loss mean |x_T - target|^2",
             "bodies": n, "steps": steps, "checkpoint_every": every,
             "forward_ms_per_step": t_fwd * 1e3, "record_ms_per_step": t_rec * 1e3 / steps,
+            "first_record_ms": t_setup * 1e3,
             "vjp_ms_per_step": t_vjp * 1e3 / steps,
             "cost_vs_forward": (t_vjp / steps) / t_fwd,
             "value": n * steps / (t_rec + t_vjp), "unit": "body-steps/s (record + VJP)",
@@ -277,9 +359,11 @@ def main():
                     help="steps of the batched-ensemble leg (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-liquid", action="store_true", help="skip the liquid-and-sand leg")
+    ap.add_argument("--no-p3m", action="store_true", help="skip the P3M-only leg")
+    ap.add_argument("--no-force-error", action="store_true", help="skip the FP64 force-error leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel eagerly instead of through captured graphs")
-    ap.add_argument("--leg", choices=("batched", "batch", "adjoint", "liquid"),
+    ap.add_argument("--leg", choices=("batched", "batch", "adjoint", "liquid", "p3m", "force"),
                     help="run only this leg and print its record (batch: the batched ensemble "
                          "without the plain loop), for kernel profiles of one workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -305,7 +389,8 @@ def main():
         bw = max(2, args.warmup // 4)
         legs = {"batched": lambda: batched_leg(args.batched_steps, bw),
                 "batch": lambda: batched_leg(args.batched_steps, bw, plain=False),
-                "adjoint": adjoint_leg, "liquid": liquid_leg}
+                "adjoint": adjoint_leg, "liquid": liquid_leg, "p3m": p3m_only_leg,
+                "force": force_error_leg}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
 
@@ -328,6 +413,17 @@ def main():
 
     print(f"[bench] profiling {args.profile_steps} steps with per-kernel events", file=sys.stderr, flush=True)
     stats = pipe.profile(args.profile_steps)
+    eager = {name: (launches, ms) for name, launches, ms in stats}
+    # The roofline's kernel time: graph-replayed steps with event records
+    # around every stage, as the timed steps run.
+    timing = "graph-replayed"
+    try:
+        stats = pipe.profile(args.profile_steps, graphs=not args.eager)
+        if args.eager:
+            timing = "eager"
+    except Exception as err:  # a runtime without event records in captures
+        print(f"[bench] graph profiling unavailable ({err}); eager timing", file=sys.stderr)
+        timing = "eager"
     per_kernel = {name: (launches, ms) for name, launches, ms in stats}
     dominant = max(per_kernel, key=lambda k: per_kernel[k][1])
     launches, total_ms = per_kernel[dominant]
@@ -344,6 +440,16 @@ def main():
         print(f"[bench] batched ensemble: {args.batched_steps} steps, batch and plain loop",
               file=sys.stderr, flush=True)
         batched = batched_leg(args.batched_steps, max(2, args.warmup // 4))
+
+    p3m = None
+    if not args.no_p3m:
+        print("[bench] P3M only: 50k bodies, 100 steps", file=sys.stderr, flush=True)
+        p3m = p3m_only_leg()
+
+    force = None
+    if not args.no_force_error:
+        print("[bench] FP64 force error of the galaxy's initial state", file=sys.stderr, flush=True)
+        force = force_error_leg()
 
     adjoint = None
     if not args.no_adjoint:
@@ -382,12 +488,16 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dominant),
-                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": alg_bytes},
+                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": alg_bytes, "timing": timing,
+                     "eager_avg_launch_us": (eager[dominant][1] / eager[dominant][0] * 1e3
+                                             if dominant in eager else None)},
         "kernels_ms_per_step": {k: v[1] / args.profile_steps for k, v in per_kernel.items()},
         "finite": finite,
         "batched": batched,
         "adjoint": adjoint,
         "liquid": liquid,
+        "p3m_only": p3m,
+        "force_error": force,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

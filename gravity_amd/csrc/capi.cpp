@@ -315,6 +315,23 @@ int gs_debug_resolve_liquid(gs_sim* sim, int count) {
     });
 }
 
+int gs_debug_density(gs_sim* sim, int count, uint32_t* density, float* spectrum, uint32_t* cells) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        e.set_active(0, count);
+        e.debug_density(density, spectrum, cells);
+    });
+}
+
+int gs_validate_fft(gs_sim* sim, int active, double* out) {
+    SIM_OR_FAIL
+    if (!out) return fail(GS_EINVAL, "null output");
+    return guarded([&] {
+        e.set_active(0, active);
+        e.validate_fft(out);
+    });
+}
+
 int gs_synchronize(gs_sim* sim) {
     SIM_OR_FAIL
     return guarded([&] { e.synchronize(); });
@@ -334,10 +351,15 @@ int gs_query(gs_sim* sim, int* out) {
 }
 
 int gs_profile_steps(gs_sim* sim, int steps, gs_kernel_stat* out, int capacity, int* written) {
+    return gs_profile_steps_mode(sim, steps, 0, out, capacity, written);
+}
+
+int gs_profile_steps_mode(gs_sim* sim, int steps, int graphs, gs_kernel_stat* out, int capacity,
+                          int* written) {
     SIM_OR_FAIL
     if (steps < 0 || (capacity > 0 && !out)) return fail(GS_EINVAL, "bad profile arguments");
     return guarded([&] {
-        auto stats = e.profile(steps);
+        auto stats = e.profile(steps, graphs != 0);
         int k = 0;
         for (auto& s : stats) {
             if (k >= capacity) break;

@@ -271,6 +271,7 @@ Engine::~Engine() {
     invalidate_graphs();
     adjoint_free();
     for (void* p : adjWork_) hipFree(p);
+    if (debugDensity_) hipFree(debugDensity_);
     for (auto& e : events_) {
         hipEventDestroy(e.second.first);
         hipEventDestroy(e.second.second);
@@ -629,6 +630,20 @@ void Engine::synchronize() { check(hipStreamSynchronize(st_), "synchronize"); }
 template <typename F>
 void Engine::timed(const char* name, F&& f) {
     if (!profiling_) { f(); return; }
+    if (graphProfiling_) {
+        // Inside a capture: the event records become graph nodes around the
+        // stage's launches, so the replay is timed as it runs.
+        hipEvent_t a = nullptr, b = nullptr;
+        check(hipEventCreate(&a), "profile event");
+        check(hipEventCreate(&b), "profile event");
+        // external records: timed event nodes, not capture-internal
+        // dependencies
+        check(hipEventRecordWithFlags(a, st_, hipEventRecordExternal), "profile event");
+        f();
+        check(hipEventRecordWithFlags(b, st_, hipEventRecordExternal), "profile event");
+        events_.push_back({name, {a, b}});
+        return;
+    }
     std::vector<LaunchEvents> launched;
     launch_sink() = &launched;
     f();
@@ -1131,22 +1146,131 @@ void Engine::debug_collide(float kickDt, float driftDt, bool writePressure) {
     check(hipGetLastError(), "collide");
 }
 
+void Engine::debug_density(uint32_t* density, float* spectrum, uint32_t* cells) {
+    const HostSim& h = sims_[0];
+    if (!h.mutual) throw ArgumentError("the density mesh needs mutual gravity");
+    normalize();
+    const size_t nodes = (size_t)h.desc.mw * h.desc.mh;
+    if (!debugDensity_) debugDensity_ = dalloc<uint32_t>((size_t)totalMesh_);
+    // rows the walls leave empty are not written by the deposit
+    check(hipMemsetAsync(debugDensity_, 0, (size_t)totalMesh_ * 4, st_), "density");
+    timed("bin", [&] { launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_); });
+    launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, cellRec_, cellNear_);
+    launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_, slotCell_);
+    launch_cell_canon(st_, gravitySlots_, slotCell_, cellPrefix_, slotBody_, body_[bodyPar_],
+                      mat_[auxPar_], cellBody_, cellRec_, cellNear_);
+    launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_,
+                            debugDensity_);
+    launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_, true);
+    check(hipGetLastError(), "density");
+    check(hipStreamSynchronize(st_), "density");
+    if (density)
+        check(hipMemcpy(density, debugDensity_ + h.desc.meshBase, nodes * 4, hipMemcpyDeviceToHost),
+              "density");
+    if (spectrum)
+        check(hipMemcpy(spectrum, spec_ + h.desc.meshBase, nodes * 8, hipMemcpyDeviceToHost), "spectrum");
+    if (cells) {
+        std::vector<uint4> rec((size_t)h.phys);
+        check(hipMemcpy(rec.data(), cellRec_ + h.desc.physBase, rec.size() * sizeof(uint4),
+                        hipMemcpyDeviceToHost), "cells");
+        for (size_t c = 0; c < rec.size(); c++) {
+            cells[2 * c] = rec[c].y;
+            cells[2 * c + 1] = rec[c].z;
+        }
+    }
+}
+
+void Engine::validate_fft(double* out) {
+    const HostSim& h = sims_[0];
+    const int mw = h.desc.mw, mh = h.desc.mh;
+    const size_t nodes = (size_t)mw * mh;
+    std::vector<uint32_t> dens(nodes);
+    std::vector<float> spec(2 * nodes);
+    debug_density(dens.data(), spec.data(), nullptr);
+    std::vector<std::complex<double>> ref(nodes);
+    const double inv = 1.0 / (double)h.desc.massScale;
+    for (size_t i = 0; i < nodes; i++) ref[i] = (double)dens[i] * inv;
+    fft2d_f64(ref, mw, mh);
+    double maxAbs = 0.0, maxRef = 0.0, sq = 0.0;
+    for (size_t i = 0; i < nodes; i++) {
+        for (int c = 0; c < 2; c++) {
+            double e = c == 0 ? ref[i].real() : ref[i].imag();
+            double d = std::fabs(e - (double)spec[2 * i + c]);
+            maxAbs = std::max(maxAbs, d);
+            maxRef = std::max(maxRef, std::fabs(e));
+            sq += d * d;
+        }
+    }
+    out[0] = maxAbs;
+    out[1] = maxAbs / std::max(maxRef, 1e-30);
+    out[2] = std::sqrt(sq / (double)(2 * nodes));
+}
+
 void Engine::debug_sort() {
     if (bodyBlocks_ == 0) return;
     enqueue_sort();
     check(hipGetLastError(), "sort");
 }
 
-std::vector<std::tuple<std::string, int, double>> Engine::profile(int steps) {
+std::vector<std::tuple<std::string, int, double>> Engine::profile(int steps, bool graphs) {
     profiling_ = true;
-    for (int i = 0; i < steps; i++) step();
-    profiling_ = false;
+    graphProfiling_ = graphs;
+    try {
+        for (int i = 0; i < steps; i++) {
+            if (!graphs) {
+                step();
+                continue;
+            }
+            // One step captured with event records around every stage, then
+            // replayed: kernels run back to back as in the timed steps.
+            normalize();
+            const HostSim& h0 = sims_[0];
+            int interval = h0.cfg.spatial_sort_interval;
+            bool sortDue = interval > 0 && (sims_[0].steps % interval) == 0;
+            for (auto& h : sims_) h.steps++;
+            if (bodyBlocks_ == 0) continue;
+            hipGraph_t graph;
+            check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
+            enqueue_step(sortDue);
+            check(hipStreamEndCapture(st_, &graph), "capture");
+            hipGraphExec_t exec;
+            check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+            hipGraphDestroy(graph);
+            check(hipGraphLaunch(exec, st_), "graph launch");
+            check(hipStreamSynchronize(st_), "profile");
+            hipGraphExecDestroy(exec);
+        }
+    } catch (...) {
+        profiling_ = graphProfiling_ = false;
+        hipStreamCaptureStatus status;
+        if (hipStreamIsCapturing(st_, &status) == hipSuccess && status != hipStreamCaptureStatusNone) {
+            hipGraph_t broken = nullptr;
+            if (hipStreamEndCapture(st_, &broken) == hipSuccess && broken) hipGraphDestroy(broken);
+        }
+        for (auto& e : events_) {
+            hipEventDestroy(e.second.first);
+            hipEventDestroy(e.second.second);
+        }
+        events_.clear();
+        (void)hipGetLastError();  // leave no error for the next call to find
+        throw;
+    }
+    profiling_ = graphProfiling_ = false;
     check(hipStreamSynchronize(st_), "profile");
     std::map<std::string, std::pair<int, double>> acc;
     std::vector<std::string> order;
     for (auto& e : events_) {
         float ms = 0.0f;
-        check(hipEventElapsedTime(&ms, e.second.first, e.second.second), "elapsed");
+        hipError_t err = hipEventElapsedTime(&ms, e.second.first, e.second.second);
+        if (err != hipSuccess) {
+            for (auto& x : events_) {
+                hipEventDestroy(x.second.first);
+                hipEventDestroy(x.second.second);
+            }
+            events_.clear();
+            (void)hipGetLastError();
+            throw StateError(std::string("elapsed: ") + hipGetErrorString(err));
+        }
         if (!acc.count(e.first)) order.push_back(e.first);
         acc[e.first].first += 1;
         acc[e.first].second += ms;

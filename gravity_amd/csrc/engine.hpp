@@ -90,12 +90,23 @@ public:
     void adjoint_order(int count, uint32_t* where);
     // J_A(x)^T w of the acceleration field at the current positions
     void debug_accel_vjp(const float* w, float* out);
+    // The fixed-point density mesh, its FP32 forward spectrum (both mesh
+    // nodes, row-major) and the physical cell table (count, mass / 256 per
+    // cell) of the current positions; each output nullable.
+    void debug_density(uint32_t* density, float* spectrum, uint32_t* cells);
+    // validateFft (K/GravityPipeline.kt:378-419): the density spectrum
+    // against an FP64 transform of the same deposit: out = {max abs error,
+    // max abs error / max |reference|, RMS error}.
+    void validate_fft(double* out);
     void debug_collide(float kickDt, float driftDt, bool writePressure);
     void debug_sort();
     void debug_liquid();
 
     // Eager steps with an event pair around every launch.
-    std::vector<std::tuple<std::string, int, double>> profile(int steps);
+    // graphs: each step captured with event records around every stage and
+    // replayed (the launches run back to back, as in graph-replayed steps);
+    // otherwise eager launches stamped by their own dispatch.
+    std::vector<std::tuple<std::string, int, double>> profile(int steps, bool graphs = false);
     int mesh_w(int s) const { return sims_.at(s).desc.mw; }
     int mesh_h(int s) const { return sims_.at(s).desc.mh; }
 
@@ -204,12 +215,14 @@ private:
     AdjCheckpoint adjFinal_;
     std::vector<void*> adjAllocs_;   // recording buffers (checkpoints, slots, final state)
     std::vector<void*> adjWork_;     // per-body work arrays, allocated once
+    uint32_t* debugDensity_ = nullptr;   // density readback (diagnostics only)
     void adjoint_free();
     void adjoint_work();
     void adjoint_save_slot(int j);
 
     // profiling
     bool profiling_ = false;
+    bool graphProfiling_ = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events_;
 };
 

@@ -235,12 +235,16 @@ __device__ __forceinline__ int mesh_row(const SimDesc& s, int a) {
 }
 
 // ---------------------------------------------------------------------------
+// DENSITY: also store the fixed-point deposit of the item's rows to density
+// (the readback of the density mesh, S/build_grid.comp's buffer).
+template <bool DENSITY>
 __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __restrict__ sims,
                                                              const LineWork* __restrict__ work,
                                                              const uint32_t* __restrict__ cellPrefix,
                                                              const float4* __restrict__ cellBody,
                                                              float2* __restrict__ spec,
-                                                             const float2* __restrict__ tw, int twLen) {
+                                                             const float2* __restrict__ tw, int twLen,
+                                                             uint32_t* __restrict__ density) {
     __shared__ float2 data[ROW_LDS];
     __shared__ uint32_t mass[ROW_LDS];
     __shared__ float2 twl[TW_LDS];
@@ -354,6 +358,12 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     }
     tws.store(twl, twLen);
     __syncthreads();
+    if (DENSITY) {
+        for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
+            int l = i / L, x = i - l * L;
+            density[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = mass[i];
+        }
+    }
     for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT)
         data[i] = make_float2((float)mass[i] * s.invMassScale, 0.0f);
     __syncthreads();
@@ -365,6 +375,9 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     }
 }
 
+// FORWARD_ONLY: stop after the forward transform and store every row (the
+// density spectrum of the reference's validateFft, K/GravityPipeline.kt:378-419).
+template <bool FORWARD_ONLY>
 __global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restrict__ sims,
                                                         const LineWork* __restrict__ work,
                                                         float2* __restrict__ spec,
@@ -415,6 +428,14 @@ __global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restric
     }
     __syncthreads();
     lds_fft<-1, COL_NT, COL_LDS, 16>(data, H, CB, 1, CB, twl, twLen);
+    if (FORWARD_ONLY) {
+        float2* out = spec + s.meshBase;
+        for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
+            int y = i / CB, c = i - y * CB;
+            out[y * s.mw + c0 + c] = data[i];
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         int i = (int)threadIdx.x + q * COL_NT;
@@ -541,18 +562,26 @@ void launch_fft_rows_forward_values(hipStream_t st, const SimDesc* sims, const L
 
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
-                             const float2* twiddles, int twiddleLength) {
+                             const float2* twiddles, int twiddleLength, uint32_t* density) {
     if (items == 0) return;
-    launch(k_fft_rows_forward, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
-                       cellBody, spec, twiddles, twiddleLength);
+    if (density)
+        launch(k_fft_rows_forward<true>, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
+               cellBody, spec, twiddles, twiddleLength, density);
+    else
+        launch(k_fft_rows_forward<false>, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
+               cellBody, spec, twiddles, twiddleLength, density);
 }
 
 void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                         float2* spec, const float2* green, const float2* twiddles,
-                        int twiddleLength) {
+                        int twiddleLength, bool forwardOnly) {
     if (items == 0) return;
-    launch(k_fft_columns, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green,
-                       twiddles, twiddleLength);
+    if (forwardOnly)
+        launch(k_fft_columns<true>, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green,
+               twiddles, twiddleLength);
+    else
+        launch(k_fft_columns<false>, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green,
+               twiddles, twiddleLength);
 }
 
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,

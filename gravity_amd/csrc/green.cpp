@@ -48,6 +48,8 @@ static void fft2d(std::vector<cd>& m, int w, int h) {
     for (int x = 0; x < w; x++) fft_inplace(m.data() + x, h, w);
 }
 
+void fft2d_f64(std::vector<std::complex<double>>& values, int w, int h) { fft2d(values, w, h); }
+
 static double sinc(double v) { return std::fabs(v) < 1e-12 ? 1.0 : std::sin(v) / v; }
 
 void green_spectrum(const GreenKey& k, std::vector<float>& out) {

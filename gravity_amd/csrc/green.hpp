@@ -2,6 +2,7 @@
 // coincident-pair normals and the liquid kernel constants.
 #pragma once
 
+#include <complex>
 #include <vector>
 
 namespace gs {
@@ -19,6 +20,8 @@ struct GreenKey {
 void green_spectrum(const GreenKey& key, std::vector<float>& out);
 // exp(-2 pi i m / length), m < length, interleaved complex float.
 void fft_twiddles(int length, std::vector<float>& out);
+// Forward FP64 2D DFT (exp(-i ...), unscaled) of a w x h row-major mesh.
+void fft2d_f64(std::vector<std::complex<double>>& values, int w, int h);
 // cos/sin of the 65536 fallback angles, interleaved.
 void fallback_normals(std::vector<float>& out);
 

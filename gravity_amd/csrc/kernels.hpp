@@ -101,12 +101,15 @@ constexpr int NEAR_INLINE = 2;
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* slotCell,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
                        const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear);
+// density (nullable) receives the fixed-point deposit of every live mesh row.
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
-                             const float2* twiddles, int twiddleLength);
+                             const float2* twiddles, int twiddleLength, uint32_t* density = nullptr);
+// forwardOnly: leave the 2D density spectrum in spec (every row) instead of
+// multiplying and transforming back.
 void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                         float2* spec, const float2* green, const float2* twiddles,
-                        int twiddleLength);
+                        int twiddleLength, bool forwardOnly = false);
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              float2* spec, const float2* twiddles, int twiddleLength);
 // cellRec: per physical cell {first slot, body count, mass in 1/256, 0}.

@@ -86,9 +86,12 @@ SIGNATURES = {
     "gs_debug_resolve_collisions": (_I, [_P, _I, _F, _F, _I, _I]),
     "gs_debug_sort": (_I, [_P, _I]),
     "gs_debug_resolve_liquid": (_I, [_P, _I]),
+    "gs_debug_density": (_I, [_P, _I, _P, _P, _P]),
+    "gs_validate_fft": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double)]),
     "gs_synchronize": (_I, [_P]),
     "gs_query": (_I, [_P, ctypes.POINTER(_I)]),
     "gs_profile_steps": (_I, [_P, _I, ctypes.POINTER(GsKernelStat), _I, ctypes.POINTER(_I)]),
+    "gs_profile_steps_mode": (_I, [_P, _I, _I, ctypes.POINTER(GsKernelStat), _I, ctypes.POINTER(_I)]),
     "gs_set_graphs": (_I, [_P, _I]),
     "gs_batch_create": (_I, [ctypes.POINTER(GsConfig), _I, ctypes.POINTER(_P), ctypes.POINTER(_I),
                              ctypes.POINTER(_P)]),

@@ -259,6 +259,27 @@ class GravityPipeline:
         native.check(self._lib.gs_download(self._h, native.GS_FORCE_MESH, 0, mw * mh, native.ptr(out)))
         return out.reshape(mh, mw, 2)
 
+    def densityMesh(self, activeBodies: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(fixed-point density (mh, mw) uint32, its FP32 spectrum (mh, mw)
+        complex, cell table (gh, gw, 2) uint32 of bodies and mass / 256) of
+        the current positions."""
+        q = self.query()
+        mw, mh, pw, ph = q["meshWidth"], q["meshHeight"], q["gridWidth"], q["gridHeight"]
+        dens = np.empty(mw * mh, np.uint32)
+        spec = np.empty(2 * mw * mh, np.float32)
+        cells = np.empty(2 * pw * ph, np.uint32)
+        native.check(self._lib.gs_debug_density(self._h, int(activeBodies), native.ptr(dens),
+                                                native.ptr(spec), native.ptr(cells)))
+        spec = spec.reshape(mh, mw, 2)
+        return dens.reshape(mh, mw), spec[..., 0] + 1j * spec[..., 1], cells.reshape(ph, pw, 2)
+
+    def validateFft(self, activeBodies: int) -> dict:
+        """K/GravityPipeline.kt:378-419 (FftValidationResult)."""
+        out = (ctypes.c_double * 3)()
+        native.check(self._lib.gs_validate_fft(self._h, int(activeBodies), out))
+        return {"maximumAbsoluteError": out[0], "maximumRelativeError": out[1],
+                "rootMeanSquareError": out[2]}
+
     def resolveCollisions(self, activeBodies: int, kickDt: float, driftDt: float, writePressure: bool,
                           substepIndex: int | None) -> None:
         native.check(self._lib.gs_debug_resolve_collisions(
@@ -330,12 +351,15 @@ class GravityPipeline:
     def setGraphs(self, enabled: bool) -> None:
         native.check(self._lib.gs_set_graphs(self._h, int(bool(enabled))))
 
-    def profile(self, steps: int) -> list[tuple[str, int, float]]:
-        """Per-kernel (name, launches, total ms) over `steps` eager steps."""
+    def profile(self, steps: int, graphs: bool = False) -> list[tuple[str, int, float]]:
+        """Per-stage (name, launches, total ms) over `steps` steps: eager
+        launches stamped by their own dispatch, or (graphs) captured steps
+        with event records around every stage, replayed back to back."""
         cap = 64
         stats = (native.GsKernelStat * cap)()
         written = ctypes.c_int(0)
-        native.check(self._lib.gs_profile_steps(self._h, int(steps), stats, cap, ctypes.byref(written)))
+        native.check(self._lib.gs_profile_steps_mode(self._h, int(steps), int(bool(graphs)), stats, cap,
+                                                     ctypes.byref(written)))
         return [(stats[i].name.decode(), stats[i].launches, stats[i].total_ms) for i in range(written.value)]
 
     # -- diagnostics (K/GravityPipeline.kt:793-905) -------------------------

@@ -148,6 +148,22 @@ int gs_debug_resolve_collisions(gs_sim *sim, int count, float kick_dt, float dri
 int gs_debug_sort(gs_sim *sim, int count);
 int gs_debug_resolve_liquid(gs_sim *sim, int count);
 
+/* Readback of the force build's first half at the current positions of
+ * `count` bodies (the density buffer of S/build_grid.comp:49-76 and the
+ * spectrum of S/fft_init.comp + the forward transform): density receives
+ * the fixed-point CIC deposit (uint32 per mesh node, row-major; rows the
+ * walls leave empty are 0), spectrum its FP32 forward 2D DFT (2 floats per
+ * mesh node), cells the physical cell table (2 uint32 per cell: bodies,
+ * mass in 1/256). Each output is nullable. Blocks. */
+int gs_debug_density(gs_sim *sim, int count, uint32_t *density, float *spectrum, uint32_t *cells);
+
+/* validateFft(activeBodies) (K/GravityPipeline.kt:378-419): the density
+ * spectrum against an FP64 transform of the same fixed-point deposit;
+ * out[0] maximum absolute error, out[1] that over the largest reference
+ * value, out[2] RMS error. Works on walls too (the reference checks the
+ * periodic mesh only). Blocks. */
+int gs_validate_fft(gs_sim *sim, int active, double *out);
+
 /* Waits for the handle's stream. */
 int gs_synchronize(gs_sim *sim);
 
@@ -163,6 +179,12 @@ typedef struct gs_kernel_stat {
     double total_ms;
 } gs_kernel_stat;
 int gs_profile_steps(gs_sim *sim, int steps, gs_kernel_stat *out, int capacity, int *written);
+/* The same with graphs != 0: each step is captured with event records
+ * around every stage and replayed, so the stages run back to back as in
+ * gs_step's graph replays (a stage's time then includes the gap to the
+ * launch before it). */
+int gs_profile_steps_mode(gs_sim *sim, int steps, int graphs, gs_kernel_stat *out, int capacity,
+                          int *written);
 
 /* Steps through one captured HIP graph per step (default on); 0 launches
  * every kernel eagerly. */
