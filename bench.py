@@ -366,6 +366,8 @@ def main():
     ap.add_argument("--leg", choices=("batched", "batch", "adjoint", "liquid", "p3m", "force"),
                     help="run only this leg and print its record (batch: the batched ensemble "
                          "without the plain loop), for kernel profiles of one workload")
+    ap.add_argument("--leg-steps", type=int, default=0,
+                    help="timed steps of the liquid or P3M-only leg run with --leg (0: its default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -387,9 +389,12 @@ def main():
 
     if args.leg:
         bw = max(2, args.warmup // 4)
+        k = args.leg_steps
         legs = {"batched": lambda: batched_leg(args.batched_steps, bw),
                 "batch": lambda: batched_leg(args.batched_steps, bw, plain=False),
-                "adjoint": adjoint_leg, "liquid": liquid_leg, "p3m": p3m_only_leg,
+                "adjoint": adjoint_leg,
+                "liquid": (lambda: liquid_leg(k, 2)) if k else liquid_leg,
+                "p3m": (lambda: p3m_only_leg(k, 2)) if k else p3m_only_leg,
                 "force": force_error_leg}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
@@ -413,21 +418,19 @@ def main():
 
     print(f"[bench] profiling {args.profile_steps} steps with per-kernel events", file=sys.stderr, flush=True)
     stats = pipe.profile(args.profile_steps)
-    eager = {name: (launches, ms) for name, launches, ms in stats}
-    # The roofline's kernel time: graph-replayed steps with event records
-    # around every stage, as the timed steps run.
-    timing = "graph-replayed"
-    try:
-        stats = pipe.profile(args.profile_steps, graphs=not args.eager)
-        if args.eager:
-            timing = "eager"
-    except Exception as err:  # a runtime without event records in captures
-        print(f"[bench] graph profiling unavailable ({err}); eager timing", file=sys.stderr)
-        timing = "eager"
     per_kernel = {name: (launches, ms) for name, launches, ms in stats}
     dominant = max(per_kernel, key=lambda k: per_kernel[k][1])
     launches, total_ms = per_kernel[dominant]
-    avg_s = total_ms / launches / 1e3
+    eager_s = total_ms / launches / 1e3
+    # The timed steps are graph replays, whose kernels run back to back: the
+    # step time beyond the kernels' own (dispatch-stamped) time is the
+    # launches' turnaround, shared evenly over the step's launches. A kernel's
+    # graph-replayed time is its own plus that share, which is what a kernel
+    # trace of the replays reports (each launch starts as the one before ends).
+    launches_per_step = sum(c for c, _ in per_kernel.values()) / args.profile_steps
+    kernel_ms_per_step = sum(ms for _, ms in per_kernel.values()) / args.profile_steps
+    turnaround_s = max(ms_per_step - kernel_ms_per_step, 0.0) / 1e3 / launches_per_step
+    avg_s = eager_s + (0.0 if args.eager else turnaround_s)
     units = units_per_launch(dominant, pipe, n)
     alg_bytes = BYTES_PER_UNIT.get(dominant, 0) * units
     achieved = alg_bytes / avg_s / 1e9
@@ -488,9 +491,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dominant),
-                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": alg_bytes, "timing": timing,
-                     "eager_avg_launch_us": (eager[dominant][1] / eager[dominant][0] * 1e3
-                                             if dominant in eager else None)},
+                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": alg_bytes,
+                     "timing": "eager" if args.eager else "graph-replayed",
+                     "dispatch_stamped_us": eager_s * 1e6, "turnaround_us": turnaround_s * 1e6},
         "kernels_ms_per_step": {k: v[1] / args.profile_steps for k, v in per_kernel.items()},
         "finite": finite,
         "batched": batched,

@@ -351,15 +351,10 @@ int gs_query(gs_sim* sim, int* out) {
 }
 
 int gs_profile_steps(gs_sim* sim, int steps, gs_kernel_stat* out, int capacity, int* written) {
-    return gs_profile_steps_mode(sim, steps, 0, out, capacity, written);
-}
-
-int gs_profile_steps_mode(gs_sim* sim, int steps, int graphs, gs_kernel_stat* out, int capacity,
-                          int* written) {
     SIM_OR_FAIL
     if (steps < 0 || (capacity > 0 && !out)) return fail(GS_EINVAL, "bad profile arguments");
     return guarded([&] {
-        auto stats = e.profile(steps, graphs != 0);
+        auto stats = e.profile(steps);
         int k = 0;
         for (auto& s : stats) {
             if (k >= capacity) break;

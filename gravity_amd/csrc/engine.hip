@@ -630,20 +630,6 @@ void Engine::synchronize() { check(hipStreamSynchronize(st_), "synchronize"); }
 template <typename F>
 void Engine::timed(const char* name, F&& f) {
     if (!profiling_) { f(); return; }
-    if (graphProfiling_) {
-        // Inside a capture: the event records become graph nodes around the
-        // stage's launches, so the replay is timed as it runs.
-        hipEvent_t a = nullptr, b = nullptr;
-        check(hipEventCreate(&a), "profile event");
-        check(hipEventCreate(&b), "profile event");
-        // external records: timed event nodes, not capture-internal
-        // dependencies
-        check(hipEventRecordWithFlags(a, st_, hipEventRecordExternal), "profile event");
-        f();
-        check(hipEventRecordWithFlags(b, st_, hipEventRecordExternal), "profile event");
-        events_.push_back({name, {a, b}});
-        return;
-    }
     std::vector<LaunchEvents> launched;
     launch_sink() = &launched;
     f();
@@ -1212,41 +1198,12 @@ void Engine::debug_sort() {
     check(hipGetLastError(), "sort");
 }
 
-std::vector<std::tuple<std::string, int, double>> Engine::profile(int steps, bool graphs) {
+std::vector<std::tuple<std::string, int, double>> Engine::profile(int steps) {
     profiling_ = true;
-    graphProfiling_ = graphs;
     try {
-        for (int i = 0; i < steps; i++) {
-            if (!graphs) {
-                step();
-                continue;
-            }
-            // One step captured with event records around every stage, then
-            // replayed: kernels run back to back as in the timed steps.
-            normalize();
-            const HostSim& h0 = sims_[0];
-            int interval = h0.cfg.spatial_sort_interval;
-            bool sortDue = interval > 0 && (sims_[0].steps % interval) == 0;
-            for (auto& h : sims_) h.steps++;
-            if (bodyBlocks_ == 0) continue;
-            hipGraph_t graph;
-            check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
-            enqueue_step(sortDue);
-            check(hipStreamEndCapture(st_, &graph), "capture");
-            hipGraphExec_t exec;
-            check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
-            hipGraphDestroy(graph);
-            check(hipGraphLaunch(exec, st_), "graph launch");
-            check(hipStreamSynchronize(st_), "profile");
-            hipGraphExecDestroy(exec);
-        }
+        for (int i = 0; i < steps; i++) step();
     } catch (...) {
-        profiling_ = graphProfiling_ = false;
-        hipStreamCaptureStatus status;
-        if (hipStreamIsCapturing(st_, &status) == hipSuccess && status != hipStreamCaptureStatusNone) {
-            hipGraph_t broken = nullptr;
-            if (hipStreamEndCapture(st_, &broken) == hipSuccess && broken) hipGraphDestroy(broken);
-        }
+        profiling_ = false;
         for (auto& e : events_) {
             hipEventDestroy(e.second.first);
             hipEventDestroy(e.second.second);
@@ -1255,7 +1212,7 @@ std::vector<std::tuple<std::string, int, double>> Engine::profile(int steps, boo
         (void)hipGetLastError();  // leave no error for the next call to find
         throw;
     }
-    profiling_ = graphProfiling_ = false;
+    profiling_ = false;
     check(hipStreamSynchronize(st_), "profile");
     std::map<std::string, std::pair<int, double>> acc;
     std::vector<std::string> order;

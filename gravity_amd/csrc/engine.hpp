@@ -103,10 +103,7 @@ public:
     void debug_liquid();
 
     // Eager steps with an event pair around every launch.
-    // graphs: each step captured with event records around every stage and
-    // replayed (the launches run back to back, as in graph-replayed steps);
-    // otherwise eager launches stamped by their own dispatch.
-    std::vector<std::tuple<std::string, int, double>> profile(int steps, bool graphs = false);
+    std::vector<std::tuple<std::string, int, double>> profile(int steps);
     int mesh_w(int s) const { return sims_.at(s).desc.mw; }
     int mesh_h(int s) const { return sims_.at(s).desc.mh; }
 
@@ -222,7 +219,6 @@ private:
 
     // profiling
     bool profiling_ = false;
-    bool graphProfiling_ = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events_;
 };
 

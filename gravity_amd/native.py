@@ -91,7 +91,6 @@ SIGNATURES = {
     "gs_synchronize": (_I, [_P]),
     "gs_query": (_I, [_P, ctypes.POINTER(_I)]),
     "gs_profile_steps": (_I, [_P, _I, ctypes.POINTER(GsKernelStat), _I, ctypes.POINTER(_I)]),
-    "gs_profile_steps_mode": (_I, [_P, _I, _I, ctypes.POINTER(GsKernelStat), _I, ctypes.POINTER(_I)]),
     "gs_set_graphs": (_I, [_P, _I]),
     "gs_batch_create": (_I, [ctypes.POINTER(GsConfig), _I, ctypes.POINTER(_P), ctypes.POINTER(_I),
                              ctypes.POINTER(_P)]),

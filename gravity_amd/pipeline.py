@@ -351,15 +351,13 @@ class GravityPipeline:
     def setGraphs(self, enabled: bool) -> None:
         native.check(self._lib.gs_set_graphs(self._h, int(bool(enabled))))
 
-    def profile(self, steps: int, graphs: bool = False) -> list[tuple[str, int, float]]:
-        """Per-stage (name, launches, total ms) over `steps` steps: eager
-        launches stamped by their own dispatch, or (graphs) captured steps
-        with event records around every stage, replayed back to back."""
+    def profile(self, steps: int) -> list[tuple[str, int, float]]:
+        """Per-stage (name, launches, total ms) over `steps` eager steps, each
+        launch stamped by its own dispatch."""
         cap = 64
         stats = (native.GsKernelStat * cap)()
         written = ctypes.c_int(0)
-        native.check(self._lib.gs_profile_steps_mode(self._h, int(steps), int(bool(graphs)), stats, cap,
-                                                     ctypes.byref(written)))
+        native.check(self._lib.gs_profile_steps(self._h, int(steps), stats, cap, ctypes.byref(written)))
         return [(stats[i].name.decode(), stats[i].launches, stats[i].total_ms) for i in range(written.value)]
 
     # -- diagnostics (K/GravityPipeline.kt:793-905) -------------------------

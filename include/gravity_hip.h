@@ -179,12 +179,6 @@ typedef struct gs_kernel_stat {
     double total_ms;
 } gs_kernel_stat;
 int gs_profile_steps(gs_sim *sim, int steps, gs_kernel_stat *out, int capacity, int *written);
-/* The same with graphs != 0: each step is captured with event records
- * around every stage and replayed, so the stages run back to back as in
- * gs_step's graph replays (a stage's time then includes the gap to the
- * launch before it). */
-int gs_profile_steps_mode(gs_sim *sim, int steps, int graphs, gs_kernel_stat *out, int capacity,
-                          int *written);
 
 /* Steps through one captured HIP graph per step (default on); 0 launches
  * every kernel eagerly. */
