@@ -323,6 +323,16 @@ int gs_debug_density(gs_sim* sim, int count, uint32_t* density, float* spectrum,
     });
 }
 
+int gs_debug_density_noalloc(gs_sim* sim, int count, uint32_t* density, float* spectrum,
+                             uint32_t* cells) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        // Changing the active count would rebuild the handle's tables.
+        if (count != e.sim(0).n) throw gs::ArgumentError("count must equal the active body count");
+        e.debug_density_noalloc(density, spectrum, cells);
+    });
+}
+
 int gs_validate_fft(gs_sim* sim, int active, double* out) {
     SIM_OR_FAIL
     if (!out) return fail(GS_EINVAL, "null output");

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace gs {
 
@@ -62,6 +64,30 @@ static void validate(const gs_config& c) {
     req(c.liquid_viscosity >= 0.0f && c.liquid_viscosity <= 1.0f, "liquid viscosity must be in 0..1");
     req(!c.bounded || 2 * std::max(c.grid_width, c.grid_height) <= 2048,
         "Walls need an FFT grid of at most 1024 cells per axis");
+}
+
+// Scratch of gs_debug_density_noalloc: the fixed-point density staging of the
+// largest mesh it accepts (2048 x 1024 nodes, 8 MiB), one copy per GPU for
+// the life of the process. Host code never uses the array's name as an
+// address: scratch_density() returns the device address that
+// hipGetSymbolAddress gives for the current device.
+constexpr size_t DENSITY_SCRATCH_NODES = (size_t)2048 * 1024;
+static __device__ __align__(256) uint32_t scratch[DENSITY_SCRATCH_NODES];
+
+static uint32_t* scratch_density() {
+    static std::mutex lock;
+    static std::map<int, uint32_t*> byDevice;
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess) throw StateError("hipGetDevice failed");
+    std::lock_guard<std::mutex> guard(lock);
+    auto it = byDevice.find(device);
+    if (it != byDevice.end()) return it->second;
+    void* ptr = nullptr;
+    hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || ptr == nullptr)
+        throw StateError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+    byDevice[device] = (uint32_t*)ptr;
+    return (uint32_t*)ptr;
 }
 
 std::vector<LaunchEvents>*& launch_sink() {
@@ -1135,34 +1161,47 @@ void Engine::debug_collide(float kickDt, float driftDt, bool writePressure) {
 void Engine::debug_density(uint32_t* density, float* spectrum, uint32_t* cells) {
     const HostSim& h = sims_[0];
     if (!h.mutual) throw ArgumentError("the density mesh needs mutual gravity");
+    if (!debugDensity_) debugDensity_ = dalloc<uint32_t>((size_t)totalMesh_);
+    density_readback(debugDensity_, density, spectrum, cells);
+}
+
+void Engine::debug_density_noalloc(uint32_t* density, float* spectrum, uint32_t* cells) {
+    const HostSim& h = sims_[0];
+    if (!h.mutual) throw ArgumentError("the density mesh needs mutual gravity");
+    if ((size_t)totalMesh_ > DENSITY_SCRATCH_NODES)
+        throw ArgumentError("the density scratch holds 2048 x 1024 mesh nodes at most");
+    density_readback(scratch_density(), density, spectrum, cells);
+}
+
+// staging holds totalMesh_ uint32 (indexed by the simulations' mesh bases).
+void Engine::density_readback(uint32_t* staging, uint32_t* density, float* spectrum, uint32_t* cells) {
+    const HostSim& h = sims_[0];
     normalize();
     const size_t nodes = (size_t)h.desc.mw * h.desc.mh;
-    if (!debugDensity_) debugDensity_ = dalloc<uint32_t>((size_t)totalMesh_);
-    // rows the walls leave empty are not written by the deposit
-    check(hipMemsetAsync(debugDensity_, 0, (size_t)totalMesh_ * 4, st_), "density");
-    timed("bin", [&] { launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_); });
+    // rows the walls leave empty are not written by the deposit, and the
+    // staging keeps whatever an earlier call left in it
+    check(hipMemsetAsync(staging, 0, (size_t)totalMesh_ * 4, st_), "density");
+    launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_);
     launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, cellRec_, cellNear_);
     launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_, slotCell_);
     launch_cell_canon(st_, gravitySlots_, slotCell_, cellPrefix_, slotBody_, body_[bodyPar_],
                       mat_[auxPar_], cellBody_, cellRec_, cellNear_);
     launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_,
-                            debugDensity_);
+                            staging);
     launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_, true);
     check(hipGetLastError(), "density");
     check(hipStreamSynchronize(st_), "density");
     if (density)
-        check(hipMemcpy(density, debugDensity_ + h.desc.meshBase, nodes * 4, hipMemcpyDeviceToHost),
+        check(hipMemcpy(density, staging + h.desc.meshBase, nodes * 4, hipMemcpyDeviceToHost),
               "density");
     if (spectrum)
         check(hipMemcpy(spectrum, spec_ + h.desc.meshBase, nodes * 8, hipMemcpyDeviceToHost), "spectrum");
     if (cells) {
-        std::vector<uint4> rec((size_t)h.phys);
-        check(hipMemcpy(rec.data(), cellRec_ + h.desc.physBase, rec.size() * sizeof(uint4),
-                        hipMemcpyDeviceToHost), "cells");
-        for (size_t c = 0; c < rec.size(); c++) {
-            cells[2 * c] = rec[c].y;
-            cells[2 * c + 1] = rec[c].z;
-        }
+        // the count and mass words (.y, .z) of each cell's 16-byte record
+        check(hipMemcpy2D(cells, 2 * sizeof(uint32_t),
+                          reinterpret_cast<const char*>(cellRec_ + h.desc.physBase) + sizeof(uint32_t),
+                          sizeof(uint4), 2 * sizeof(uint32_t), (size_t)h.phys, hipMemcpyDeviceToHost),
+              "cells");
     }
 }
 

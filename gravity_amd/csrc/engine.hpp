@@ -94,6 +94,9 @@ public:
     // nodes, row-major) and the physical cell table (count, mass / 256 per
     // cell) of the current positions; each output nullable.
     void debug_density(uint32_t* density, float* spectrum, uint32_t* cells);
+    // The same with the density staged in the library's static scratch array
+    // instead of a buffer allocated on first use: no device allocation.
+    void debug_density_noalloc(uint32_t* density, float* spectrum, uint32_t* cells);
     // validateFft (K/GravityPipeline.kt:378-419): the density spectrum
     // against an FP64 transform of the same deposit: out = {max abs error,
     // max abs error / max |reference|, RMS error}.
@@ -213,6 +216,7 @@ private:
     std::vector<void*> adjAllocs_;   // recording buffers (checkpoints, slots, final state)
     std::vector<void*> adjWork_;     // per-body work arrays, allocated once
     uint32_t* debugDensity_ = nullptr;   // density readback (diagnostics only)
+    void density_readback(uint32_t* staging, uint32_t* density, float* spectrum, uint32_t* cells);
     void adjoint_free();
     void adjoint_work();
     void adjoint_save_slot(int j);

@@ -87,6 +87,7 @@ SIGNATURES = {
     "gs_debug_sort": (_I, [_P, _I]),
     "gs_debug_resolve_liquid": (_I, [_P, _I]),
     "gs_debug_density": (_I, [_P, _I, _P, _P, _P]),
+    "gs_debug_density_noalloc": (_I, [_P, _I, _P, _P, _P]),
     "gs_validate_fft": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double)]),
     "gs_synchronize": (_I, [_P]),
     "gs_query": (_I, [_P, ctypes.POINTER(_I)]),

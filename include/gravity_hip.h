@@ -157,6 +157,18 @@ int gs_debug_resolve_liquid(gs_sim *sim, int count);
  * mass in 1/256). Each output is nullable. Blocks. */
 int gs_debug_density(gs_sim *sim, int count, uint32_t *density, float *spectrum, uint32_t *cells);
 
+/* gs_debug_density without any device allocation in the call. No
+ * counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The fixed-point density is staged in a static device array of the
+ * library, 2048 x 1024 mesh nodes (8 MiB), one per GPU, held until the
+ * process ends: a larger mesh fails with GS_EINVAL, and so does a count
+ * other than the handle's active body count (changing it would rebuild the
+ * handle's tables). Every call on a GPU shares that GPU's array and clears
+ * it first; calls that overlap on one GPU race. Blocks. */
+int gs_debug_density_noalloc(gs_sim *sim, int count, uint32_t *density, float *spectrum,
+                             uint32_t *cells);
+
 /* validateFft(activeBodies) (K/GravityPipeline.kt:378-419): the density
  * spectrum against an FP64 transform of the same fixed-point deposit;
  * out[0] maximum absolute error, out[1] that over the largest reference
