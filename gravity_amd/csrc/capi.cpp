@@ -342,6 +342,58 @@ int gs_validate_fft(gs_sim* sim, int active, double* out) {
     });
 }
 
+int gs_projectile_body_count(gs_sim* sim, float diameter, int* count) {
+    SIM_OR_FAIL
+    if (!count) return fail(GS_EINVAL, "null output");
+    return guarded([&] { *count = e.projectile_body_count(0, diameter); });
+}
+
+int gs_spawn_projectile(gs_sim* sim, int first, float cx, float cy, float vx, float vy, float diameter,
+                        int* spawned) {
+    SIM_OR_FAIL
+    if (!spawned) return fail(GS_EINVAL, "null output");
+    *spawned = 0;
+    return guarded([&] { *spawned = e.spawn_projectile(0, first, cx, cy, vx, vy, diameter); });
+}
+
+int gs_projectile_offsets(float contact, float diameter, float* xy, int capacity, int* count) {
+    if (!count) return fail(GS_EINVAL, "null output");
+    return guarded([&] {
+        std::vector<float> v;
+        gs::projectile_offsets(contact, diameter, v);
+        *count = (int)(v.size() / 2);
+        if (xy && capacity >= *count) std::memcpy(xy, v.data(), v.size() * sizeof(float));
+    });
+}
+
+int gs_projectile_diameter_for_body_count(float contact, int bodies, float* diameter) {
+    if (!diameter) return fail(GS_EINVAL, "null output");
+    return guarded([&] { *diameter = gs::projectile_diameter_for(contact, bodies); });
+}
+
+int gs_measure_cluster(gs_sim* sim, int active, gs_cluster_statistics* out) {
+    SIM_OR_FAIL
+    if (!out) return fail(GS_EINVAL, "null output");
+    return guarded([&] {
+        e.set_active(0, active);
+        *out = e.measure_cluster(0);
+    });
+}
+
+int gs_supports_walls(int grid_width, int grid_height) {
+    return gs::supports_walls(grid_width, grid_height) ? 1 : 0;
+}
+
+int gs_render_view(gs_sim* sim, void** bodies, void** pressures) {
+    SIM_OR_FAIL
+    return guarded([&] { e.render_view(0, bodies, pressures); });
+}
+
+int gs_render_copy(gs_sim* sim, int count, void* bodies, void* pressures) {
+    SIM_OR_FAIL
+    return guarded([&] { e.render_copy(0, count, bodies, pressures); });
+}
+
 int gs_synchronize(gs_sim* sim) {
     SIM_OR_FAIL
     return guarded([&] { e.synchronize(); });

@@ -48,6 +48,12 @@ struct HostSim {
     SleepScene scene;                     // of the last step
 };
 
+// CircularProjectile (K/CircularProjectile.kt:10-60) and the walls check
+// (GravityPipeline.supportsWalls, K/GravityPipeline.kt:1628), host only.
+void projectile_offsets(float contactDiameter, float projectileDiameter, std::vector<float>& xy);
+float projectile_diameter_for(float contactDiameter, int bodies);
+bool supports_walls(int gridWidth, int gridHeight);
+
 class Engine {
 public:
     Engine(const std::vector<gs_config>& cfgs, const std::vector<const float*>& states,
@@ -71,6 +77,14 @@ public:
     void upload_accel(int s, int first, int count, const float* accel);
     void download(int s, int what, int first, int count, void* dst);
     void synchronize();
+
+    // K/GravityPipeline.kt:621-646, :180, :793-905, :509-512 (members.cpp)
+    int spawn_projectile(int s, int firstBody, float centerX, float centerY, float velocityX,
+                         float velocityY, float diameter);
+    int projectile_body_count(int s, float diameter) const;
+    gs_cluster_statistics measure_cluster(int s);
+    void render_view(int s, void** bodies, void** pressures);
+    void render_copy(int s, int count, void* bodies, void* pressures);
 
     void initialize_acceleration();
     void step();

@@ -61,6 +61,15 @@ class GsConfig(ctypes.Structure):
     ]
 
 
+class GsClusterStatistics(ctypes.Structure):
+    """Mirror of gs_cluster_statistics (ClusterStatistics, K/GravityPipeline.kt:47-57)."""
+    _fields_ = [("center_x", ctypes.c_double), ("center_y", ctypes.c_double),
+                ("covariance_area_pixels", ctypes.c_double), ("equivalent_diameter_pixels", ctypes.c_double),
+                ("core90_area_pixels", ctypes.c_double), ("core99_area_pixels", ctypes.c_double),
+                ("max_speed_pixels_per_second", ctypes.c_double),
+                ("average_speed_pixels_per_second", ctypes.c_double), ("non_finite_bodies", ctypes.c_int)]
+
+
 class GsKernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 40), ("launches", ctypes.c_int), ("total_ms", ctypes.c_double)]
 
@@ -89,6 +98,14 @@ SIGNATURES = {
     "gs_debug_density": (_I, [_P, _I, _P, _P, _P]),
     "gs_debug_density_noalloc": (_I, [_P, _I, _P, _P, _P]),
     "gs_validate_fft": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double)]),
+    "gs_projectile_body_count": (_I, [_P, _F, ctypes.POINTER(_I)]),
+    "gs_spawn_projectile": (_I, [_P, _I, _F, _F, _F, _F, _F, ctypes.POINTER(_I)]),
+    "gs_projectile_offsets": (_I, [_F, _F, _P, _I, ctypes.POINTER(_I)]),
+    "gs_projectile_diameter_for_body_count": (_I, [_F, _I, ctypes.POINTER(_F)]),
+    "gs_measure_cluster": (_I, [_P, _I, ctypes.POINTER(GsClusterStatistics)]),
+    "gs_supports_walls": (_I, [_I, _I]),
+    "gs_render_view": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "gs_render_copy": (_I, [_P, _I, _P, _P]),
     "gs_synchronize": (_I, [_P]),
     "gs_query": (_I, [_P, ctypes.POINTER(_I)]),
     "gs_profile_steps": (_I, [_P, _I, ctypes.POINTER(GsKernelStat), _I, ctypes.POINTER(_I)]),

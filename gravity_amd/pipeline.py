@@ -34,26 +34,24 @@ class ClusterStatistics:
 
 
 def projectile_offsets(contact_diameter: float, projectile_diameter: float) -> np.ndarray:
-    """CircularProjectile.offsets (K/CircularProjectile.kt:10-37), FP32 as Kotlin."""
-    f = np.float32
-    d = f(contact_diameter)
-    if not (d > 0 and f(projectile_diameter) >= d):
-        raise ValueError("projectile must fit one body")
-    radius = f(0.5) * d
-    available = f(0.5) * f(projectile_diameter) - radius
-    row_step = d * f(math.sqrt(3.0) * 0.5)
-    out = []
-    row = 0
-    y = -available
-    while y <= available:
-        x = -available + (f(0.0) if row % 2 == 0 else f(0.5) * d)
-        while x <= available:
-            if x * x + y * y <= available * available:
-                out.append((x, y))
-            x = f(x + d)
-        y = f(y + row_step)
-        row += 1
-    return np.array(out, dtype=np.float32).reshape(-1, 2)
+    """CircularProjectile.offsets (K/CircularProjectile.kt:10-37): the packing's
+    centre offsets in px, shape (count, 2), from the library (host only)."""
+    lib = native.load()
+    count = ctypes.c_int(0)
+    native.check(lib.gs_projectile_offsets(float(contact_diameter), float(projectile_diameter), None, 0,
+                                           ctypes.byref(count)))
+    out = np.empty(2 * count.value, np.float32)
+    native.check(lib.gs_projectile_offsets(float(contact_diameter), float(projectile_diameter),
+                                           native.ptr(out), count.value, ctypes.byref(count)))
+    return out.reshape(-1, 2)
+
+
+def projectile_diameter_for_body_count(contact_diameter: float, bodies: int) -> float:
+    """CircularProjectile.diameterForBodyCount (K/CircularProjectile.kt:39-60)."""
+    d = ctypes.c_float(0.0)
+    native.check(native.load().gs_projectile_diameter_for_body_count(float(contact_diameter), int(bodies),
+                                                                     ctypes.byref(d)))
+    return float(d.value)
 
 
 class GravityPipeline:
@@ -82,7 +80,7 @@ class GravityPipeline:
         self._liquid = config.start.liquid
         self._field = (0.0, 0.0)
         self._substeps = config.collisions.maximumSubsteps
-        self.boundedSpaceSupported = 2 * max(config.gravity.gridWidth, config.gravity.gridHeight) <= MAXIMUM_FFT_AXIS
+        self.boundedSpaceSupported = GravityPipeline.supportsWalls(config.gravity)
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:
@@ -174,40 +172,33 @@ class GravityPipeline:
         native.check(self._lib.gs_synchronize(self._h))
 
     def projectileBodyCount(self, diameterPixels: float) -> int:
-        return len(projectile_offsets(self.config.collisions.contactDiameterPixels, diameterPixels))
+        """K/GravityPipeline.kt:180"""
+        count = ctypes.c_int(0)
+        native.check(self._lib.gs_projectile_body_count(self._h, float(diameterPixels), ctypes.byref(count)))
+        return count.value
 
     def spawnProjectile(self, firstBody: int, centerX: float, centerY: float, velocityX: float,
                         velocityY: float, diameterPixels: float) -> int:
-        """K/GravityPipeline.kt:621-637: packed disc of loose unit-mass bodies."""
-        offsets = projectile_offsets(self.config.collisions.contactDiameterPixels, diameterPixels)
-        cap = self.config.particles.capacity
-        if not (0 <= firstBody <= cap and len(offsets) <= cap - firstBody):
-            return 0
-        f = np.float32
-        W, H = f(self.viewportWidth), f(self.viewportHeight)
-        radius = f(np.hypot(offsets[:, 0], offsets[:, 1]).max()) + f(0.5) * f(self.config.collisions.contactDiameterPixels)
-        cx, cy = f(centerX), f(centerY)
-        if self._bounded:
-            cx = min(max(cx, radius), W - radius)
-            cy = min(max(cy, radius), H - radius)
-        px = cx + offsets[:, 0]
-        py = cy + offsets[:, 1]
-        if self._bounded:
-            px = np.clip(px, f(0), W * MAXIMUM_UNIT)
-            py = np.clip(py, f(0), H * MAXIMUM_UNIT)
-        else:
-            px = px - np.floor(px / W) * W
-            py = py - np.floor(py / H) * H
-        bodies = np.empty((len(offsets), 4), dtype=np.float32)
-        bodies[:, 0] = px / W
-        bodies[:, 1] = py / H
-        bodies[:, 2] = f(velocityX) / W
-        bodies[:, 3] = f(velocityY) / H
-        mats = np.zeros((len(offsets), 4), dtype=np.float32)
-        mats[:, 0] = 1.0
-        mats[:, 1] = self.config.collisions.friction
-        self.upload(bodies, mats, first=firstBody)
-        return len(offsets)
+        """K/GravityPipeline.kt:621-646: packed disc of loose unit-mass bodies;
+        returns the bodies written (0 when they do not fit the capacity)."""
+        spawned = ctypes.c_int(0)
+        native.check(self._lib.gs_spawn_projectile(self._h, int(firstBody), float(centerX), float(centerY),
+                                                   float(velocityX), float(velocityY), float(diameterPixels),
+                                                   ctypes.byref(spawned)))
+        return spawned.value
+
+    # -- render view (bindBodiesForRender, K/GravityPipeline.kt:509-512) -----
+    def renderView(self) -> tuple[int, int]:
+        """Device addresses of the current bodies (vec4) and pressures (float)
+        buffers, valid until the next step, upload or sort."""
+        b, p = ctypes.c_void_p(), ctypes.c_void_p()
+        native.check(self._lib.gs_render_view(self._h, ctypes.byref(b), ctypes.byref(p)))
+        return int(b.value or 0), int(p.value or 0)
+
+    def renderCopy(self, count: int, bodies_device: int | None, pressures_device: int | None) -> None:
+        """Queues device-to-device copies of `count` bodies and pressures into
+        caller device memory (asynchronous on the pipeline's stream)."""
+        native.check(self._lib.gs_render_copy(self._h, int(count), bodies_device, pressures_device))
 
     # -- test hooks (T/CollisionGpuSupport.kt:43-156) -----------------------
     def upload(self, bodies: np.ndarray, materials: np.ndarray | None = None, first: int = 0) -> None:
@@ -362,40 +353,15 @@ class GravityPipeline:
 
     # -- diagnostics (K/GravityPipeline.kt:793-905) -------------------------
     def measureCluster(self, activeBodies: int) -> ClusterStatistics:
-        b = self.bodies(activeBodies).astype(np.float64)
-        W, H = self.viewportWidth, self.viewportHeight
-        finite = np.isfinite(b).all(axis=1)
-        nonfinite = int((~finite).sum())
-        x = np.where(finite, b[:, 0], 0.0)
-        y = np.where(finite, b[:, 1], 0.0)
-        speed = np.where(finite, np.hypot(b[:, 2] * W, b[:, 3] * H), 0.0)
-        ax, ay = 2 * np.pi * x, 2 * np.pi * y
-        cx = math.atan2(np.sin(ax).sum(), np.cos(ax).sum()) / (2 * np.pi)
-        cy = math.atan2(np.sin(ay).sum(), np.cos(ay).sum()) / (2 * np.pi)
-        cx -= math.floor(cx)
-        cy -= math.floor(cy)
-        ux = (cx + ((x - cx) - np.floor(x - cx + 0.5))) * W
-        uy = (cy + ((y - cy) - np.floor(y - cy + 0.5))) * H
-        mx, my = ux.mean(), uy.mean()
-        xx = ((ux - mx) ** 2).mean()
-        yy = ((uy - my) ** 2).mean()
-        xy = ((ux - mx) * (uy - my)).mean()
-        area = 4 * math.pi * math.sqrt(max(xx * yy - xy * xy, 0.0))
-        medx = np.sort(ux)[activeBodies // 2]
-        medy = np.sort(uy)[activeBodies // 2]
-        r2 = np.sort((ux - medx) ** 2 + (uy - medy) ** 2)
-
-        def core(fraction):
-            idx = min(max(int(fraction * activeBodies), 1), activeBodies) - 1
-            return math.pi * r2[idx] / fraction
-
+        out = native.GsClusterStatistics()
+        native.check(self._lib.gs_measure_cluster(self._h, int(activeBodies), ctypes.byref(out)))
         return ClusterStatistics(
-            centerX=float(medx - math.floor(medx / W) * W), centerY=float(medy - math.floor(medy / H) * H),
-            covarianceAreaPixels=area, equivalentDiameterPixels=2 * math.sqrt(area / math.pi),
-            core90AreaPixels=core(0.90), core99AreaPixels=core(0.99),
-            maxSpeedPixelsPerSecond=float(speed.max()), averageSpeedPixelsPerSecond=float(speed.mean()),
-            nonFiniteBodies=nonfinite)
+            centerX=out.center_x, centerY=out.center_y, covarianceAreaPixels=out.covariance_area_pixels,
+            equivalentDiameterPixels=out.equivalent_diameter_pixels, core90AreaPixels=out.core90_area_pixels,
+            core99AreaPixels=out.core99_area_pixels, maxSpeedPixelsPerSecond=out.max_speed_pixels_per_second,
+            averageSpeedPixelsPerSecond=out.average_speed_pixels_per_second, nonFiniteBodies=out.non_finite_bodies)
 
     @staticmethod
     def supportsWalls(gravity) -> bool:
-        return 2 * max(gravity.gridWidth, gravity.gridHeight) <= MAXIMUM_FFT_AXIS
+        """K/GravityPipeline.kt:1628"""
+        return bool(native.load().gs_supports_walls(int(gravity.gridWidth), int(gravity.gridHeight)))

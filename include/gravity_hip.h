@@ -139,6 +139,58 @@ int gs_stop_bodies(gs_sim *sim, int active);
 /* wake() (K/GravityPipeline.kt:610). */
 int gs_wake(gs_sim *sim);
 
+/* projectileBodyCount(diameterPixels) (K/GravityPipeline.kt:180): the
+ * bodies a projectile of that diameter holds, in *count. */
+int gs_projectile_body_count(gs_sim *sim, float diameter_px, int *count);
+
+/* spawnProjectile(firstBody, centerX, centerY, velocityX, velocityY,
+ * diameterPixels) (K/GravityPipeline.kt:621-646): a hexagonally packed disc
+ * of loose unit-mass bodies written at firstBody (pixels and px/s; under
+ * walls the centre is kept a radius inside them). *spawned receives the
+ * bodies written, 0 when they do not fit the capacity (fits, :720-722). */
+int gs_spawn_projectile(gs_sim *sim, int first_body, float center_x, float center_y,
+                        float velocity_x, float velocity_y, float diameter_px, int *spawned);
+
+/* CircularProjectile.offsets / diameterForBodyCount (K/CircularProjectile.kt:
+ * 10-60), host only (no handle, no GPU): the packing's centre offsets in px
+ * (2 floats each; written when capacity allows, *count always) and the
+ * smallest diameter on a 0.05 px grid holding `bodies` centres. */
+int gs_projectile_offsets(float contact_diameter_px, float projectile_diameter_px, float *xy,
+                          int capacity, int *count);
+int gs_projectile_diameter_for_body_count(float contact_diameter_px, int bodies, float *diameter_px);
+
+/* ClusterStatistics (K/GravityPipeline.kt:47-57) and measureCluster
+ * (:793-905): a blocking readback of the active bodies and the reference's
+ * physics-quality metric (circular mean centre, covariance area, core
+ * areas, speeds, non-finite bodies), computed on the host in FP64. */
+typedef struct gs_cluster_statistics {
+    double center_x;
+    double center_y;
+    double covariance_area_pixels;
+    double equivalent_diameter_pixels;
+    double core90_area_pixels;
+    double core99_area_pixels;
+    double max_speed_pixels_per_second;
+    double average_speed_pixels_per_second;
+    int non_finite_bodies;
+} gs_cluster_statistics;
+int gs_measure_cluster(gs_sim *sim, int active, gs_cluster_statistics *out);
+
+/* GravityPipeline.supportsWalls(gravity) (K/GravityPipeline.kt:1628): 1 when
+ * the doubled mesh fits the 2048-value FFT line, else 0. Host only. */
+int gs_supports_walls(int grid_width, int grid_height);
+
+/* bindBodiesForRender (K/GravityPipeline.kt:509-512, S/particles.vert:3-4):
+ * the renderer reads the bodies (binding 0, vec4 x, y, vx, vy normalised)
+ * and the pressures (binding 1, float). gs_render_view gives the device
+ * addresses of the current buffers (valid until the next call that steps,
+ * uploads or sorts; either output nullable). gs_render_copy copies `count`
+ * bodies and pressures, asynchronously on the handle's stream after the work
+ * already queued, into caller device memory (for example a GL buffer mapped
+ * through hipGraphicsGLRegisterBuffer); either destination nullable. */
+int gs_render_view(gs_sim *sim, void **bodies, void **pressures);
+int gs_render_copy(gs_sim *sim, int count, void *bodies, void *pressures);
+
 /* Test hooks mirroring the reflection hooks of T/CollisionGpuSupport.kt:68-90:
  * resolveCollisions(count, kickDt, driftDt, writePressure, substep) (one
  * pass; substep < 0 is the final pass), sortBodies(count), buildForce +
