@@ -9,7 +9,7 @@ from typing import Sequence
 
 import numpy as np
 
-from . import native
+from . import native, tuning
 from .config import GravityConfig
 
 
@@ -44,6 +44,8 @@ class GravityBatch:
         self._h = handle
         self.size = k
         self.viewports = [tuple(v) for v in viewports]
+        self._cfgs = [cfgs[i] for i in range(k)]
+        self._active = [int(counts[i]) for i in range(k)]
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -64,6 +66,22 @@ class GravityBatch:
 
     def setActive(self, index: int, activeBodies: int) -> None:
         native.check(self._lib.gs_batch_set_active(self._h, int(index), int(activeBodies)))
+        self._active[int(index)] = int(activeBodies)
+
+    @property
+    def tuning(self) -> tuning.Tuning:
+        """The launch shapes of the batch's step (gs_tuning)."""
+        return tuning.get(self._lib, self._h)
+
+    @tuning.setter
+    def tuning(self, value: tuning.Tuning) -> None:
+        tuning.set(self._lib, self._h, value)
+
+    def autotune(self, steps: int = 3, cache=None):
+        """Chooses the launch shapes for the batch's current workload (see
+        gravity_amd.tuning.autotune); the members' states are unchanged."""
+        return tuning.autotune(self._lib, self._h, tuning.workload_key(self._cfgs, self._active),
+                               steps, cache)
 
     def upload(self, index: int, bodies: np.ndarray, materials: np.ndarray | None = None,
                first: int = 0) -> None:

@@ -19,7 +19,7 @@ CSRC = PACKAGE / "csrc"
 BUILD = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libgravity_hip.so"
 SOURCES = ["sort.hip", "collide.hip", "p3m.hip", "fft.hip", "liquid.hip", "sleep.hip", "adjoint.hip",
-           "engine.hip", "green.cpp", "members.cpp", "capi.cpp"]
+           "engine.hip", "green.cpp", "members.cpp", "tuning.cpp", "capi.cpp"]
 ARCH = os.environ.get("GRAVITY_OFFLOAD_ARCH", "gfx950")
 
 COMMON_FLAGS = [

@@ -436,6 +436,42 @@ int gs_set_graphs(gs_sim* sim, int enabled) {
     return GS_OK;
 }
 
+static void tuning_out(const gs::Tuning& t, gs_tuning* out) {
+    out->big_segment_limit = t.bigLimit;
+    out->forward_item_bodies = t.forwardItemBodies;
+    out->deposit_runs = t.depositRuns;
+    out->scan_direct_blocks = t.scanDirectBlocks;
+}
+
+int gs_get_tuning(gs_sim* sim, gs_tuning* out) {
+    SIM_OR_FAIL
+    if (!out) return fail(GS_EINVAL, "null argument");
+    tuning_out(e.tuning(), out);
+    return GS_OK;
+}
+
+int gs_set_tuning(gs_sim* sim, const gs_tuning* tuning) {
+    SIM_OR_FAIL
+    if (!tuning) return fail(GS_EINVAL, "null argument");
+    return guarded([&] {
+        gs::Tuning t;
+        t.bigLimit = tuning->big_segment_limit;
+        t.forwardItemBodies = tuning->forward_item_bodies;
+        t.depositRuns = tuning->deposit_runs;
+        t.scanDirectBlocks = tuning->scan_direct_blocks;
+        e.set_tuning(t);
+    });
+}
+
+int gs_autotune(gs_sim* sim, int steps, gs_tuning* chosen, double* ms) {
+    SIM_OR_FAIL
+    if (steps < 1) return fail(GS_EINVAL, "autotune needs at least one timed step");
+    return guarded([&] {
+        gs::Tuning t = e.autotune(steps, ms);
+        if (chosen) tuning_out(t, chosen);
+    });
+}
+
 const char* gs_last_error(void) { return g_error.c_str(); }
 
 }  // extern "C"

@@ -226,6 +226,91 @@ __device__ __forceinline__ int segment_rank(const uint32_t* __restrict__ list, i
     return rank;
 }
 
+// Segments too long for segment_rank: a cell or tile holding more than a
+// launch's bigLimit bodies (the dense core of a cluster can put thousands in
+// one) is ordered by a workgroup instead, with a bitonic sort in LDS, so its
+// cost grows as c log^2 c on one CU rather than c^2 loads spread over its
+// lanes. The canon kernels list such segments {first slot, end slot, the
+// simulation's first body} in `list`, counted by counters[0]; the segment
+// kernel's last workgroup returns both counters to zero (counters[1] counts
+// the workgroups that are done), so the next launch starts from an empty list.
+struct BigSegment { uint32_t s0, s1, base, pad; };
+struct BigSegments {
+    BigSegment* list;
+    uint32_t* counters;
+    int limit;        // segments longer than this are big; 0: none is
+    int pad;
+};
+constexpr int BIG_NT = 256;        // lanes of a segment workgroup
+constexpr int BIG_LDS = 8192;      // longest segment sorted in LDS (32 KB)
+constexpr int BIG_GRID = 1024;     // segment workgroups per launch (four per CU)
+constexpr int BIG_LIMIT_MIN = 32;  // smallest limit the engine accepts
+
+// Bitonic sort of keys[0, P) in LDS (P a power of two), ascending or, with
+// DESC, descending; every lane of the workgroup calls it.
+template <bool DESC>
+__device__ __forceinline__ void lds_bitonic(uint32_t* keys, int P) {
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = (int)threadIdx.x; i < P / 2; i += BIG_NT) {
+                const int a = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const int b = a + j;
+                const uint32_t x = keys[a], y = keys[b];
+                const bool up = (a & k) == 0;  // this pair's run goes up
+                const bool swap = DESC ? (up ? x < y : x > y) : (up ? x > y : x < y);
+                if (swap) {
+                    keys[a] = y;
+                    keys[b] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Every listed big segment, its distinct values list[s0, s1) in order
+// (descending with DESC), as write(segment, position, value); one workgroup
+// per segment, the grid striding over the list. Only segments of at most
+// BIG_LDS values are listed (is_big_segment).
+template <bool DESC, typename W>
+__device__ __forceinline__ void order_big_segments(const BigSegments& big,
+                                                   const uint32_t* __restrict__ list, W&& write) {
+    __shared__ uint32_t keys[BIG_LDS];
+    __shared__ uint32_t listed;
+    if (threadIdx.x == 0) listed = atomicAdd(&big.counters[0], 0u);
+    __syncthreads();
+    for (uint32_t i = blockIdx.x; i < listed; i += gridDim.x) {
+        const BigSegment seg = big.list[i];
+        const int s0 = (int)seg.s0, n = (int)(seg.s1 - seg.s0);
+        int P = 64;
+        while (P < n) P <<= 1;
+        for (int e = (int)threadIdx.x; e < P; e += BIG_NT)
+            keys[e] = e < n ? list[s0 + e] : (DESC ? 0u : 0xffffffffu);
+        __syncthreads();
+        lds_bitonic<DESC>(keys, P);
+        for (int e = (int)threadIdx.x; e < n; e += BIG_NT) write(seg, e, keys[e]);
+        __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(&big.counters[1], 1u) == gridDim.x - 1u) {
+        atomicExch(&big.counters[0], 0u);
+        atomicExch(&big.counters[1], 0u);
+    }
+}
+
+// Whether a segment of n values goes to the workgroup sort: longer than the
+// limit, and short enough for LDS (a longer one, rare, keeps the per-lane
+// ranks, spread over as many lanes as it has values).
+__device__ __forceinline__ bool is_big_segment(const BigSegments& big, int n) {
+    return big.limit != 0 && n > big.limit && n <= BIG_LDS;
+}
+
+// Appends the segment [s0, s1) to the big list (one lane per segment).
+__device__ __forceinline__ void list_big_segment(const BigSegments& big, int s0, int s1, int base) {
+    const uint32_t i = atomicAdd(&big.counters[0], 1u);
+    big.list[i] = BigSegment{(uint32_t)s0, (uint32_t)s1, (uint32_t)base, 0u};
+}
+
 // Every launch goes through launch(). While a profiling sink is set (the
 // engine's eager profiling steps), each launch carries its own event pair,
 // stamped by the kernel's dispatch itself: a kernel's time then excludes the

@@ -213,6 +213,8 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     int scanMax = std::max({scan_totals_size(totalTiles_), scan_totals_size(totalPhys_),
                             scan_totals_size(totalLiq_)});
     scanTmp_ = dalloc<uint32_t>((size_t)scanMax);
+    bigList_ = dalloc<BigSegment>(S / (BIG_LIMIT_MIN + 1) + 1);
+    bigCounters_ = dalloc<uint32_t>(2);
     cellCount_ = dalloc<uint32_t>((size_t)totalPhys_ + 1);
     cellPrefix_ = dalloc<uint32_t>((size_t)totalPhys_ + 1);
     homeOf_ = dalloc<uint32_t>(S);
@@ -309,7 +311,8 @@ Engine::~Engine() {
                     homeOf_, slotBody_, slotCell_, slotKey_, cellRec_, cellNear_, cellBody_, spec_, green_, tw_, fallback_,
                     liqCount_, liqPrefix_, liqCellOf_, liqUnordered_, liqItems_, lambdas_,
                     sleepState_, bodyAwake_, tileAwake_, tileActive_, dTileBlocks_,
-                    dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_, poolItems_, poolBase_, poolTop_};
+                    dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_, poolItems_, poolBase_, poolTop_,
+                    bigList_, bigCounters_};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (st_) hipStreamDestroy(st_);
@@ -453,13 +456,12 @@ void Engine::rebuild_tables() {
         int nl = fft_row_lines_per_group(d.mw);
         // A forward item deposits every body of its rows first: a crowded
         // simulation (many bodies per short row) takes fewer lines per item,
-        // about FORWARD_ITEM_BODIES bodies each, so its deposits run side by
+        // about tuning_.forwardItemBodies bodies each, so its deposits run side by
         // side (the grouping changes no result: the sums are integer and each
         // line is transformed alone).
-        constexpr long FORWARD_ITEM_BODIES = 4096;
+        const long itemBodies = tuning_.forwardItemBodies;
         int nlF = nl;
-        if (h.n > 0)
-            nlF = (int)std::max<long>(1, std::min<long>(nl, FORWARD_ITEM_BODIES * active / h.n));
+        if (h.n > 0) nlF = (int)std::max<long>(1, std::min<long>(nl, itemBodies * active / h.n));
         for (int a = 0; a < active; a += nlF) rowsF.push_back({s, a, std::min(nlF, active - a), 0});
         for (int a = 0; a < active; a += nl) rowsI.push_back({s, a, std::min(nl, active - a), 0});
         int cb = fft_columns_per_group(d.mh, d.mw);
@@ -670,11 +672,11 @@ void Engine::enqueue_sort() {
     // K/GravityPipeline.kt:1269-1319
     int o = bodyPar_ ^ 1, a = auxPar_ ^ 1;
     timed("sort_count", [&] { launch_sort_count(st_, shape_, body_[bodyPar_], tileCount_, keyOf_); });
-    timed("sort_scan", [&] { launch_scan(st_, tileCount_, tilePrefix_, totalTiles_, scanTmp_); });
+    timed("sort_scan", [&] { launch_scan(st_, tileCount_, tilePrefix_, totalTiles_, scanTmp_, tuning_.scanDirectBlocks); });
     timed("sort_scatter", [&] { launch_sort_scatter(st_, shape_, keyOf_, tileCount_, tilePrefix_, slotSrc_, slotKey_); });
     timed("sort_canon", [&] {
         launch_sort_canon(st_, shape_, slotKey_, tilePrefix_, slotSrc_, body_[bodyPar_],
-                          acc_[auxPar_], mat_[auxPar_], body_[o], acc_[a], mat_[a], dest_);
+                          acc_[auxPar_], mat_[auxPar_], body_[o], acc_[a], mat_[a], dest_, big());
     });
     bodyPar_ = o;
     auxPar_ = a;
@@ -690,15 +692,17 @@ void Engine::enqueue_force() {
     if (anyMutual_) {
         timed("bin", [&] { launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_); });
         timed("cell_scan", [&] {
-            launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, cellRec_, cellNear_);
+            launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, tuning_.scanDirectBlocks, cellRec_,
+                cellNear_);
         });
         timed("cell_scatter", [&] { launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_, slotCell_); });
         timed("cell_canon", [&] {
             launch_cell_canon(st_, gravitySlots_, slotCell_, cellPrefix_, slotBody_, body_[bodyPar_],
-                              mat_[auxPar_], cellBody_, cellRec_, cellNear_);
+                              mat_[auxPar_], cellBody_, cellRec_, cellNear_, big());
         });
         timed("fft_rows_forward", [&] {
-            launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_);
+            launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_,
+                                    tuning_.depositRuns != 0);
         });
         timed("fft_columns", [&] { launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_); });
         timed("fft_rows_inverse", [&] { launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_); });
@@ -805,7 +809,7 @@ void Engine::enqueue_liquid() {
     timed("liquid_bin", [&] {
         launch_liquid_bin(st_, shape_, body_[bodyPar_], liqCount_, liqCellOf_);
     });
-    timed("liquid_scan", [&] { launch_scan(st_, liqCount_, liqPrefix_, totalLiq_, scanTmp_); });
+    timed("liquid_scan", [&] { launch_scan(st_, liqCount_, liqPrefix_, totalLiq_, scanTmp_, tuning_.scanDirectBlocks); });
     timed("liquid_scatter", [&] {
         launch_liquid_scatter(st_, shape_, liqCellOf_, liqCount_, liqPrefix_, liqUnordered_);
     });
@@ -1182,12 +1186,13 @@ void Engine::density_readback(uint32_t* staging, uint32_t* density, float* spect
     // staging keeps whatever an earlier call left in it
     check(hipMemsetAsync(staging, 0, (size_t)totalMesh_ * 4, st_), "density");
     launch_bin(st_, shape_, body_[bodyPar_], cellCount_, homeOf_);
-    launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, cellRec_, cellNear_);
+    launch_scan(st_, cellCount_, cellPrefix_, totalPhys_, scanTmp_, tuning_.scanDirectBlocks, cellRec_,
+                cellNear_);
     launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_, slotCell_);
     launch_cell_canon(st_, gravitySlots_, slotCell_, cellPrefix_, slotBody_, body_[bodyPar_],
-                      mat_[auxPar_], cellBody_, cellRec_, cellNear_);
+                      mat_[auxPar_], cellBody_, cellRec_, cellNear_, big());
     launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_,
-                            staging);
+                            tuning_.depositRuns != 0, staging);
     launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_, true);
     check(hipGetLastError(), "density");
     check(hipStreamSynchronize(st_), "density");

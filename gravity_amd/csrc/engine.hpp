@@ -54,6 +54,15 @@ void projectile_offsets(float contactDiameter, float projectileDiameter, std::ve
 float projectile_diameter_for(float contactDiameter, int bodies);
 bool supports_walls(int gridWidth, int gridHeight);
 
+// Launch shapes of the step that the autotuner chooses among; no choice
+// changes a result (each is checked bit for bit against the others).
+struct Tuning {
+    int bigLimit = 128;            // cells and tiles longer than this: workgroup sort (0: never)
+    int forwardItemBodies = 4096;  // bodies per forward row item, on average
+    int depositRuns = 0;           // register runs in the deposit (0: an atomic per corner)
+    int scanDirectBlocks = 1024;   // scans of more blocks scan their totals apart
+};
+
 class Engine {
 public:
     Engine(const std::vector<gs_config>& cfgs, const std::vector<const float*>& states,
@@ -72,6 +81,13 @@ public:
     void set_field(int s, float ax, float ay);
     void set_substeps(int substeps);
     void set_graphs(bool on) { useGraphs_ = on; }
+    const Tuning& tuning() const { return tuning_; }
+    void set_tuning(const Tuning& t);
+    // Times `steps` graph-replayed steps of each candidate (coordinate
+    // descent over the Tuning fields from the current choice, the state
+    // restored before each trial and at the end) and keeps the fastest.
+    // ms receives the chosen and the starting choice's ms per step.
+    Tuning autotune(int steps, double* ms);
 
     void upload(int s, int first, int count, const float* xyvv, const float* mats);
     void upload_accel(int s, int first, int count, const float* accel);
@@ -234,6 +250,13 @@ private:
     void adjoint_free();
     void adjoint_work();
     void adjoint_save_slot(int j);
+
+    // launch shapes
+    Tuning tuning_;
+    BigSegment* bigList_ = nullptr;
+    uint32_t* bigCounters_ = nullptr;
+    BigSegments big() const { return BigSegments{bigList_, bigCounters_, tuning_.bigLimit, 0}; }
+    double time_steps(int steps);
 
     // profiling
     bool profiling_ = false;

@@ -237,7 +237,14 @@ __device__ __forceinline__ int mesh_row(const SimDesc& s, int a) {
 // ---------------------------------------------------------------------------
 // DENSITY: also store the fixed-point deposit of the item's rows to density
 // (the readback of the density mesh, S/build_grid.comp's buffer).
-template <bool DENSITY>
+// RUNS: a lane sums the fixed-point corner weights of its consecutive bodies
+// in registers while they share a 3x3 window of mesh nodes (the nodes a
+// physical cell's bodies reach) and adds each non-zero sum to LDS when a body
+// leaves the window: in a crowded cell hundreds of bodies then cost a few LDS
+// atomics instead of four each, on a handful of contended addresses. The
+// corner weights round per body as before and integer sums do not depend on
+// their order, so the deposit is the same either way.
+template <bool DENSITY, bool RUNS>
 __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __restrict__ sims,
                                                              const LineWork* __restrict__ work,
                                                              const uint32_t* __restrict__ cellPrefix,
@@ -301,6 +308,36 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     // in its mesh row; an item of several lines routes each corner to the
     // line of its mesh row, so each body of the rows is visited once.
     const int line0 = mesh_row(s, first);
+    // RUNS: the open window's sums (row-major, rows ky.., columns kx..).
+    uint32_t run[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) run[k] = 0u;
+    int kx = -0x40000000, ky = -0x40000000;
+    auto flush = [&]() {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int my = wrap_axis(ky + j, s.mh);
+            int l1, l2;
+            if (count == 1) {
+                l1 = my == line0 ? 0 : -1;
+                l2 = -1;
+            } else {
+                l1 = !s.bounded || my <= s.ph ? my - first : -1;
+                l2 = s.bounded && my == s.mh - 1 ? s.ph + 1 - first : -1;
+            }
+            const bool in1 = (unsigned)l1 < (unsigned)count;
+            const bool in2 = (unsigned)l2 < (unsigned)count;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const uint32_t v = run[j * 3 + i];
+                run[j * 3 + i] = 0u;
+                if (v == 0u || (!in1 && !in2)) continue;
+                const int mx = wrap_axis(kx + i, s.mw);
+                if (in1) atomicAdd(&mass[l1 * L + mx], v);
+                if (in2) atomicAdd(&mass[l2 * L + mx], v);
+            }
+        }
+    };
     // The rows of a range are contiguous in the cell segments.
     for (int q = 0; q < nr; q++) {
         {
@@ -324,6 +361,33 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
                     float fx = gx - flx, fy = gy - fly;
                     float wgt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy),
                                     (1.0f - fx) * fy, fx * fy};
+                    if (RUNS) {
+                        // The window of the body's physical cell: its corners
+                        // are columns bx, bx + 1 of columns cx - 1 .. cx + 1.
+                        const int wx = bx - (fx < 0.5f ? 1 : 0);
+                        const int wy = by - (fy < 0.5f ? 1 : 0);
+                        if (wx != kx || wy != ky) {
+                            flush();
+                            kx = wx;
+                            ky = wy;
+                        }
+                        const int ox = bx - kx, oy = by - ky;
+                        uint32_t qv[4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            float wk = wgt[k] * bd.z;
+                            qv[k] = (uint32_t)rintf(wk * scale);
+                        }
+#pragma unroll
+                        for (int j = 0; j < 3; j++) {
+                            const uint32_t qa = j == oy ? qv[0] : (j == oy + 1 ? qv[2] : 0u);
+                            const uint32_t qb = j == oy ? qv[1] : (j == oy + 1 ? qv[3] : 0u);
+#pragma unroll
+                            for (int i = 0; i < 3; i++)
+                                run[j * 3 + i] += i == ox ? qa : (i == ox + 1 ? qb : 0u);
+                        }
+                        continue;
+                    }
                     if (count == 1) {
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
@@ -356,6 +420,7 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
             }
         }
     }
+    if (RUNS) flush();
     tws.store(twl, twLen);
     __syncthreads();
     if (DENSITY) {
@@ -562,14 +627,13 @@ void launch_fft_rows_forward_values(hipStream_t st, const SimDesc* sims, const L
 
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
-                             const float2* twiddles, int twiddleLength, uint32_t* density) {
+                             const float2* twiddles, int twiddleLength, bool runs,
+                             uint32_t* density) {
     if (items == 0) return;
-    if (density)
-        launch(k_fft_rows_forward<true>, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
-               cellBody, spec, twiddles, twiddleLength, density);
-    else
-        launch(k_fft_rows_forward<false>, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
-               cellBody, spec, twiddles, twiddleLength, density);
+    auto k = density ? (runs ? k_fft_rows_forward<true, true> : k_fft_rows_forward<true, false>)
+                     : (runs ? k_fft_rows_forward<false, true> : k_fft_rows_forward<false, false>);
+    launch(k, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix, cellBody, spec, twiddles,
+           twiddleLength, density);
 }
 
 void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,

@@ -51,8 +51,11 @@ struct LineWork { int sim; int first; int count; int pad; };
 // Exclusive scan of len values (out[len] = total). rec, if given, receives
 // {prefix, count, 0, 0} as each element's 16-byte record, and near (two
 // uint4 per element) {count, 0, 0, 0} as the first half of its near record.
+// A scan of more than directBlocks blocks of 1024 values scans the block
+// totals in a launch of their own; a shorter one has each block sum the
+// totals before it.
 void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals,
-                 uint4* rec = nullptr, uint4* near = nullptr);
+                 int directBlocks, uint4* rec = nullptr, uint4* near = nullptr);
 int scan_totals_size(int len);
 
 // ---- spatial sort ----
@@ -66,7 +69,8 @@ void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
 void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* slotKey,
                        const uint32_t* tilePrefix, const uint32_t* slotSrc,
                        const float4* bodyIn, const float2* accIn, const float4* matIn,
-                       float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest);
+                       float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest,
+                       const BigSegments& big);
 
 // ---- collisions ----
 void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b);
@@ -100,11 +104,14 @@ void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
 constexpr int NEAR_INLINE = 2;
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* slotCell,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
-                       const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear);
-// density (nullable) receives the fixed-point deposit of every live mesh row.
+                       const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear,
+                       const BigSegments& big);
+// density (nullable) receives the fixed-point deposit of every live mesh row;
+// runs: sum a lane's bodies of one cell in registers before the LDS atomics.
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
-                             const float2* twiddles, int twiddleLength, uint32_t* density = nullptr);
+                             const float2* twiddles, int twiddleLength, bool runs,
+                             uint32_t* density = nullptr);
 // forwardOnly: leave the 2D density spectrum in spec (every row) instead of
 // multiplying and transforming back.
 void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,

@@ -69,6 +69,8 @@ __global__ void __launch_bounds__(WG) k_cell_scatter(const SimDesc* __restrict__
 // cell stores its sum, any other accumulates it with one atomic into the
 // cell record and the near record (whose masses the cell scan zeroed). The
 // bodies of a cell of at most NEAR_INLINE go into its near record as well.
+// A cell of more than big.limit bodies is listed for k_cell_canon_big, which
+// orders it.
 __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __restrict__ slotCell,
                                                    const uint32_t* __restrict__ cellPrefix,
                                                    const uint32_t* __restrict__ slotBody,
@@ -76,7 +78,7 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
                                                    const float4* __restrict__ mats,
                                                    float4* __restrict__ cellBody,
                                                    uint4* __restrict__ cellRec,
-                                                   uint4* __restrict__ cellNear) {
+                                                   uint4* __restrict__ cellNear, BigSegments big) {
     int d = (int)(blockIdx.x * WG + threadIdx.x);
     const bool active = d < slots;
     uint32_t c = 0xffffffffu, units = 0u;
@@ -89,17 +91,21 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
         const float m = mats[g].x;
         s0 = (int)cellPrefix[c];
         s1 = (int)cellPrefix[c + 1];
-        const int rank = segment_rank<true>(slotBody, s0, s1, g);
-        cellBody[s0 + rank] = make_float4(p.x, p.y, m, __uint_as_float(g));
         units = (uint32_t)rintf(m * 256.0f);
-        if (s1 - s0 <= NEAR_INLINE) {
-            float* rec = reinterpret_cast<float*>(cellNear + 2 * (size_t)c);
-            if (rank == 0) {
-                *reinterpret_cast<float2*>(rec + 2) = make_float2(p.x, p.y);
-                rec[7] = m;
-            } else {
-                *reinterpret_cast<float2*>(rec + 4) = make_float2(p.x, p.y);
-                rec[6] = m;
+        if (is_big_segment(big, s1 - s0)) {
+            if (d == s0) list_big_segment(big, s0, s1, 0);
+        } else {
+            const int rank = segment_rank<true>(slotBody, s0, s1, g);
+            cellBody[s0 + rank] = make_float4(p.x, p.y, m, __uint_as_float(g));
+            if (s1 - s0 <= NEAR_INLINE) {
+                float* rec = reinterpret_cast<float*>(cellNear + 2 * (size_t)c);
+                if (rank == 0) {
+                    *reinterpret_cast<float2*>(rec + 2) = make_float2(p.x, p.y);
+                    rec[7] = m;
+                } else {
+                    *reinterpret_cast<float2*>(rec + 4) = make_float2(p.x, p.y);
+                    rec[6] = m;
+                }
             }
         }
     }
@@ -121,6 +127,19 @@ __global__ void __launch_bounds__(WG) k_cell_canon(int slots, const uint32_t* __
             atomicAdd(nearMass, units);
         }
     }
+}
+
+// The cells k_cell_canon listed as big, each ordered by descending body
+// index by one workgroup.
+__global__ void __launch_bounds__(BIG_NT) k_cell_canon_big(const uint32_t* __restrict__ slotBody,
+                                                           const float4* __restrict__ body,
+                                                           const float4* __restrict__ mats,
+                                                           float4* __restrict__ cellBody,
+                                                           BigSegments big) {
+    order_big_segments<true>(big, slotBody, [&](const BigSegment& seg, int rank, uint32_t g) {
+        const float4 p = body[g];
+        cellBody[seg.s0 + (uint32_t)rank] = make_float4(p.x, p.y, mats[g].x, __uint_as_float(g));
+    });
 }
 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
@@ -388,10 +407,14 @@ void launch_cell_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
 
 void launch_cell_canon(hipStream_t st, int slots, const uint32_t* slotCell,
                        const uint32_t* cellPrefix, const uint32_t* slotBody, const float4* body,
-                       const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear) {
+                       const float4* mats, float4* cellBody, uint4* cellRec, uint4* cellNear,
+                       const BigSegments& big) {
     if (slots <= 0) return;
     launch(k_cell_canon, dim3((slots + WG - 1) / WG), dim3(WG), 0, st, slots, slotCell,
-                       cellPrefix, slotBody, body, mats, cellBody, cellRec, cellNear);
+                       cellPrefix, slotBody, body, mats, cellBody, cellRec, cellNear, big);
+    if (big.limit != 0)
+        launch(k_cell_canon_big, dim3(BIG_GRID), dim3(BIG_NT), 0, st, slotBody, body, mats,
+               cellBody, big);
 }
 
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,

@@ -72,8 +72,9 @@ __global__ void __launch_bounds__(WG) k_scan_local(const uint32_t* __restrict__ 
 // {prefix, count, 0, 0} and the first half of its 32-byte near record
 // {count, 0, 0, 0} (the cell tables of the kick; the cell canon accumulates
 // the mass into the third word of the one and the second of the other).
+// With scanned, totals already holds each block's offset (k_scan_totals).
 __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int len,
-                                                 const uint32_t* __restrict__ totals,
+                                                 const uint32_t* __restrict__ totals, int scanned,
                                                  const uint32_t* __restrict__ in,
                                                  uint4* __restrict__ rec,
                                                  uint4* __restrict__ near) {
@@ -89,16 +90,20 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int
         o[q] = out[min(base + q * WG, len)];
         c[q] = in[min(base + q * WG, max(len - 1, 0))];
     }
-    uint32_t part = 0u;
-    for (int i = (int)threadIdx.x; i < nb; i += 4 * WG) {
-        uint32_t t[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) t[q] = totals[min(i + q * WG, nb - 1)];
-#pragma unroll
-        for (int q = 0; q < 4; q++) part += i + q * WG < nb ? t[q] : 0u;
-    }
     uint32_t add;
-    block_exclusive(part, &add);
+    if (scanned) {
+        add = totals[nb];
+    } else {
+        uint32_t part = 0u;
+        for (int i = (int)threadIdx.x; i < nb; i += 4 * WG) {
+            uint32_t t[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) t[q] = totals[min(i + q * WG, nb - 1)];
+#pragma unroll
+            for (int q = 0; q < 4; q++) part += i + q * WG < nb ? t[q] : 0u;
+        }
+        block_exclusive(part, &add);
+    }
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int idx = base + q * WG;
@@ -113,13 +118,40 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, int
     }
 }
 
+// Exclusive scan of the nb block totals in place, by one workgroup,
+// TOTALS_PER consecutive totals a lane per round.
+constexpr int TOTALS_PER = 16;
+__global__ void __launch_bounds__(WG) k_scan_totals(uint32_t* __restrict__ totals, int nb) {
+    uint32_t carry = 0u;
+    for (int base = 0; base < nb; base += WG * TOTALS_PER) {
+        const int first = base + (int)threadIdx.x * TOTALS_PER;
+        uint32_t v[TOTALS_PER];
+        uint32_t sum = 0u;
+#pragma unroll
+        for (int q = 0; q < TOTALS_PER; q++) {
+            v[q] = first + q < nb ? totals[first + q] : 0u;
+            sum += v[q];
+        }
+        uint32_t total;
+        uint32_t before = block_exclusive(sum, &total) + carry;
+#pragma unroll
+        for (int q = 0; q < TOTALS_PER; q++) {
+            if (first + q < nb) totals[first + q] = before;
+            before += v[q];
+        }
+        carry += total;
+    }
+}
+
 int scan_totals_size(int len) { return (len + 1 + SCAN_BLOCK - 1) / SCAN_BLOCK; }
 
 void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int len, uint32_t* totals,
-                 uint4* rec, uint4* near) {
+                 int directBlocks, uint4* rec, uint4* near) {
     int blocks = scan_totals_size(len);
+    const int scanned = blocks > directBlocks ? 1 : 0;
     launch(k_scan_local, dim3(blocks), dim3(WG), 0, st, in, out, len, totals);
-    launch(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, in, rec, near);
+    if (scanned) launch(k_scan_totals, dim3(1), dim3(WG), 0, st, totals, blocks);
+    launch(k_scan_add, dim3(blocks), dim3(WG), 0, st, out, len, totals, scanned, in, rec, near);
 }
 
 // ---------------------------------------------------------------------------
@@ -189,14 +221,15 @@ __global__ void __launch_bounds__(WG) k_sort_scatter(const SimDesc* __restrict__
     slotKey[slot] = t;
 }
 
-// Orders every tile by original index and moves the bodies.
+// Orders every tile by original index and moves the bodies; a tile of more
+// than big.limit bodies is listed for k_sort_canon_big, which orders it.
 __global__ void __launch_bounds__(WG) k_sort_canon(
     const SimDesc* __restrict__ sims, const BodyBlock* __restrict__ blocks,
     const uint32_t* __restrict__ slotKey, const uint32_t* __restrict__ tilePrefix,
     const uint32_t* __restrict__ slotSrc, const float4* __restrict__ bodyIn,
     const float2* __restrict__ accIn, const float4* __restrict__ matIn,
     float4* __restrict__ bodyOut, float2* __restrict__ accOut, float4* __restrict__ matOut,
-    uint32_t* __restrict__ dest) {
+    uint32_t* __restrict__ dest, BigSegments big) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int d = b.first + (int)threadIdx.x;
@@ -209,11 +242,31 @@ __global__ void __launch_bounds__(WG) k_sort_canon(
     const float4 ma = matIn[src];
     int s0 = s.base + (int)(tilePrefix[t] - (uint32_t)s.rank0);
     int s1 = s.base + (int)(tilePrefix[t + 1] - (uint32_t)s.rank0);
+    if (is_big_segment(big, s1 - s0)) {
+        if (d == s0) list_big_segment(big, s0, s1, s.base);
+        return;
+    }
     int fin = s0 + segment_rank<false>(slotSrc, s0, s1, src);
     bodyOut[fin] = bo;
     accOut[fin] = ac;
     matOut[fin] = ma;
     dest[src] = (uint32_t)(fin - s.base);
+}
+
+// The tiles k_sort_canon listed as big, each ordered by ascending original
+// index by one workgroup, and their bodies moved.
+__global__ void __launch_bounds__(BIG_NT) k_sort_canon_big(
+    const uint32_t* __restrict__ slotSrc, const float4* __restrict__ bodyIn,
+    const float2* __restrict__ accIn, const float4* __restrict__ matIn,
+    float4* __restrict__ bodyOut, float2* __restrict__ accOut, float4* __restrict__ matOut,
+    uint32_t* __restrict__ dest, BigSegments big) {
+    order_big_segments<false>(big, slotSrc, [&](const BigSegment& seg, int rank, uint32_t src) {
+        const uint32_t fin = seg.s0 + (uint32_t)rank;
+        bodyOut[fin] = bodyIn[src];
+        accOut[fin] = accIn[src];
+        matOut[fin] = matIn[src];
+        dest[src] = fin - seg.base;
+    });
 }
 
 void launch_sort_count(hipStream_t st, const LaunchShape& ls, const float4* body,
@@ -234,11 +287,15 @@ void launch_sort_scatter(hipStream_t st, const LaunchShape& ls, const uint32_t* 
 void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* slotKey,
                        const uint32_t* tilePrefix, const uint32_t* slotSrc,
                        const float4* bodyIn, const float2* accIn, const float4* matIn,
-                       float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest) {
+                       float4* bodyOut, float2* accOut, float4* matOut, uint32_t* dest,
+                       const BigSegments& big) {
     if (ls.bodyBlocks == 0) return;
     launch(k_sort_canon, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        slotKey, tilePrefix, slotSrc, bodyIn, accIn, matIn, bodyOut, accOut, matOut,
-                       dest);
+                       dest, big);
+    if (big.limit != 0)
+        launch(k_sort_canon_big, dim3(BIG_GRID), dim3(BIG_NT), 0, st, slotSrc, bodyIn, accIn,
+               matIn, bodyOut, accOut, matOut, dest, big);
 }
 
 }  // namespace gs

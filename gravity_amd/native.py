@@ -70,6 +70,12 @@ class GsClusterStatistics(ctypes.Structure):
                 ("average_speed_pixels_per_second", ctypes.c_double), ("non_finite_bodies", ctypes.c_int)]
 
 
+class GsTuning(ctypes.Structure):
+    """Mirror of gs_tuning: the step's launch shapes (no reference counterpart)."""
+    _fields_ = [("big_segment_limit", ctypes.c_int), ("forward_item_bodies", ctypes.c_int),
+                ("deposit_runs", ctypes.c_int), ("scan_direct_blocks", ctypes.c_int)]
+
+
 class GsKernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 40), ("launches", ctypes.c_int), ("total_ms", ctypes.c_double)]
 
@@ -110,6 +116,9 @@ SIGNATURES = {
     "gs_query": (_I, [_P, ctypes.POINTER(_I)]),
     "gs_profile_steps": (_I, [_P, _I, ctypes.POINTER(GsKernelStat), _I, ctypes.POINTER(_I)]),
     "gs_set_graphs": (_I, [_P, _I]),
+    "gs_get_tuning": (_I, [_P, ctypes.POINTER(GsTuning)]),
+    "gs_set_tuning": (_I, [_P, ctypes.POINTER(GsTuning)]),
+    "gs_autotune": (_I, [_P, _I, ctypes.POINTER(GsTuning), ctypes.POINTER(ctypes.c_double)]),
     "gs_batch_create": (_I, [ctypes.POINTER(GsConfig), _I, ctypes.POINTER(_P), ctypes.POINTER(_I),
                              ctypes.POINTER(_P)]),
     "gs_batch_size": (_I, [_P]),

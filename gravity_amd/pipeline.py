@@ -12,7 +12,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import native
+from . import native, tuning
 from .config import GravityConfig
 
 MAXIMUM_UNIT = np.float32(0.99999994)
@@ -81,6 +81,24 @@ class GravityPipeline:
         self._field = (0.0, 0.0)
         self._substeps = config.collisions.maximumSubsteps
         self.boundedSpaceSupported = GravityPipeline.supportsWalls(config.gravity)
+        self._cfg = cfg
+
+    # -- launch shapes (no reference counterpart) ----------------------------
+    @property
+    def tuning(self):
+        """The launch shapes of the step (gs_tuning); no choice changes a result."""
+        return tuning.get(self._lib, self._h)
+
+    @tuning.setter
+    def tuning(self, value) -> None:
+        tuning.set(self._lib, self._h, value)
+
+    def autotune(self, activeBodies: int, steps: int = 3, cache=None):
+        """Chooses the launch shapes for `activeBodies` active bodies (see
+        gravity_amd.tuning.autotune); the simulation's state is unchanged."""
+        native.check(self._lib.gs_batch_set_active(self._h, 0, int(activeBodies)))
+        return tuning.autotune(self._lib, self._h, tuning.workload_key([self._cfg], [activeBodies]),
+                               steps, cache)
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:

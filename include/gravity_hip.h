@@ -248,6 +248,37 @@ int gs_profile_steps(gs_sim *sim, int steps, gs_kernel_stat *out, int capacity, 
  * every kernel eagerly. */
 int gs_set_graphs(gs_sim *sim, int enabled);
 
+/* Launch shapes of the step. No reference counterpart: the reference's
+ * workgroup sizes are fixed in its shaders. No choice changes a result.
+ *   big_segment_limit    a P3M cell or sort tile of more bodies is ordered by
+ *                        one workgroup's LDS sort instead of per-lane ranks
+ *                        (0: never; else at least 32; default 128)
+ *   forward_item_bodies  bodies per forward-FFT row item, on average (the
+ *                        deposit's load balance; 64 .. 2^24, default 4096)
+ *   deposit_runs         1: sum a lane's bodies of one cell in registers
+ *                        before the LDS atomics of the deposit (default 0)
+ *   scan_direct_blocks   scans of more 1024-value blocks scan their block
+ *                        totals in a launch of their own (default 1024) */
+typedef struct gs_tuning {
+    int big_segment_limit;
+    int forward_item_bodies;
+    int deposit_runs;
+    int scan_direct_blocks;
+} gs_tuning;
+int gs_get_tuning(gs_sim *sim, gs_tuning *out);
+/* GS_EINVAL for a value out of range. Takes effect from the next step. */
+int gs_set_tuning(gs_sim *sim, const gs_tuning *tuning);
+/* Autotuner: times `steps` graph-replayed steps of the handle's own
+ * workload for each candidate (coordinate descent over the fields above,
+ * starting from the current choice; a candidate must be 1 % faster) and
+ * keeps the fastest. The state is copied on the device first and put back
+ * before every trial and at the end, so the simulations continue exactly
+ * as if untuned (the copies need about 80 bytes per body slot of device
+ * memory, more with sleeping tiles, while it runs). chosen (nullable)
+ * receives the choice; ms (nullable, 2 doubles) its ms per step and the
+ * starting choice's. Blocks. */
+int gs_autotune(gs_sim *sim, int steps, gs_tuning *chosen, double *ms);
+
 /* Batched ensembles. No reference counterpart: the reference runs one
  * simulation per GravityPipeline. A batch handle holds `count` independent
  * simulations, each with its own configuration, viewport, body count and
