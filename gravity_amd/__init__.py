@@ -4,5 +4,7 @@ spatial sort, behind the reference's GravityPipeline API over the C ABI of
 libgravity_hip.so; GravityBatch steps many simulations as one."""
 from .config import (CollisionConfig, GravityConfig, GravityFieldConfig, LiquidConfig,  # noqa: F401
                      ParticleConfig, StartConfig)
-from .pipeline import ClusterStatistics, GravityPipeline, projectile_offsets  # noqa: F401
+from .pipeline import (AdaptiveCollisionStatistics, ClusterStatistics, GravityPipeline,  # noqa: F401
+                       projectile_offsets)
+from .governor import SubstepGovernor  # noqa: F401
 from .batch import GravityBatch  # noqa: F401

@@ -68,6 +68,14 @@ class GravityBatch:
         native.check(self._lib.gs_batch_set_active(self._h, int(index), int(activeBodies)))
         self._active[int(index)] = int(activeBodies)
 
+    def measureAdaptiveCollisions(self, index: int):
+        """The adaptive collision state of member `index` after the last step
+        (GravityPipeline.measureAdaptiveCollisions)."""
+        from .pipeline import AdaptiveCollisionStatistics
+        out = native.GsAdaptiveStatistics()
+        native.check(self._lib.gs_measure_adaptive_collisions(self._h, int(index), ctypes.byref(out)))
+        return AdaptiveCollisionStatistics.from_native(out)
+
     @property
     def tuning(self) -> tuning.Tuning:
         """The launch shapes of the batch's step (gs_tuning)."""

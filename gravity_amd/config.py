@@ -106,6 +106,12 @@ class CollisionConfig:
     neighborSkinPixels: float = 0.5
     sleep: bool = True
     sleepSpeedPixels: float = 6.0
+    gpuBudgetMillis: float = 8.5
+    allowedDisplacementFraction: float = 0.25
+    allowedPenetrationFraction: float = 0.125
+    denseContactThreshold: int = 4
+    denseBodyFraction: float = 0.01
+    minimumDenseBodies: int = 256
 
     def __post_init__(self):
         _require(self.timeStep > 0 and math.isfinite(self.timeStep), "dt")
@@ -118,6 +124,12 @@ class CollisionConfig:
         _require(self.coincidentDistanceSquaredEpsilon > 0, "coincident-distance-epsilon")
         _require(self.neighborSkinPixels > 0, "collision-skin")
         _require(self.sleepSpeedPixels >= 0, "sleep-speed")
+        _require(self.gpuBudgetMillis >= 0 and math.isfinite(self.gpuBudgetMillis), "gpu-budget-ms")
+        _require(0 <= self.allowedDisplacementFraction <= 1, "adaptive-displacement-fraction")
+        _require(0 <= self.allowedPenetrationFraction <= 1, "adaptive-penetration-fraction")
+        _require(self.denseContactThreshold >= 1, "adaptive-dense-contact-count")
+        _require(0 <= self.denseBodyFraction <= 1, "adaptive-dense-body-fraction")
+        _require(self.minimumDenseBodies >= 1, "adaptive-dense-min-bodies")
 
 
 @dataclass(frozen=True)
@@ -257,6 +269,14 @@ class GravityConfig:
             neighborSkinPixels=number("collision-skin", d.collisions.neighborSkinPixels * scale),
             sleep=boolean("sleep", d.collisions.sleep),
             sleepSpeedPixels=number("sleep-speed", d.collisions.sleepSpeedPixels * scale),
+            gpuBudgetMillis=number("gpu-budget-ms", d.collisions.gpuBudgetMillis),
+            allowedDisplacementFraction=number("adaptive-displacement-fraction",
+                                               d.collisions.allowedDisplacementFraction),
+            allowedPenetrationFraction=number("adaptive-penetration-fraction",
+                                              d.collisions.allowedPenetrationFraction),
+            denseContactThreshold=integer("adaptive-dense-contact-count", d.collisions.denseContactThreshold),
+            denseBodyFraction=number("adaptive-dense-body-fraction", d.collisions.denseBodyFraction),
+            minimumDenseBodies=integer("adaptive-dense-min-bodies", d.collisions.minimumDenseBodies),
         )
         liquid = LiquidConfig(
             kernelRadiusDiameters=number("liquid-kernel", d.liquid.kernelRadiusDiameters),
@@ -307,6 +327,12 @@ class GravityConfig:
         c.liquid_iterations = lq.iterations
         c.liquid_viscosity = lq.viscosity
         c.collisions_enabled = int(self.collisionsEnabled)
+        c.adaptive_collisions = int(k.adaptive)
+        c.allowed_displacement_fraction = k.allowedDisplacementFraction
+        c.allowed_penetration_fraction = k.allowedPenetrationFraction
+        c.dense_contact_threshold = k.denseContactThreshold
+        c.dense_body_fraction = k.denseBodyFraction
+        c.minimum_dense_bodies = k.minimumDenseBodies
         return c
 
 

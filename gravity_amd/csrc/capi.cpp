@@ -73,6 +73,12 @@ void gs_config_defaults(gs_config* c) {
     c->field_acceleration_x = 0.0f;
     c->field_acceleration_y = 0.0f;
     c->collisions_enabled = 1;
+    c->adaptive_collisions = 0;
+    c->allowed_displacement_fraction = 0.25f;
+    c->allowed_penetration_fraction = 0.125f;
+    c->dense_contact_threshold = 4;
+    c->dense_body_fraction = 0.01f;
+    c->minimum_dense_bodies = 256;
 }
 
 int gs_create(const gs_config* cfg, const float* state, int n, gs_sim** out) {
@@ -434,6 +440,13 @@ int gs_set_graphs(gs_sim* sim, int enabled) {
     SIM_OR_FAIL
     e.set_graphs(enabled != 0);
     return GS_OK;
+}
+
+int gs_measure_adaptive_collisions(gs_sim* sim, int index, gs_adaptive_statistics* out) {
+    SIM_OR_FAIL
+    if (!out) return fail(GS_EINVAL, "null argument");
+    if (index < 0 || index >= e.sims()) return fail(GS_EINVAL, "simulation index out of range");
+    return guarded([&] { *out = e.measure_adaptive(index); });
 }
 
 static void tuning_out(const gs::Tuning& t, gs_tuning* out) {

@@ -221,17 +221,22 @@ __device__ __forceinline__ void walk_stencil_serial(const SimDesc& s, int cx, in
 // With the sleeping tiles on, the step's kick pass tells every body's state
 // for the step: a body outside the awake tiles (and not a fluid body) rests,
 // its velocity dropped, and no pass of the step moves it.
+// With adaptive collisions a drift takes the step's substep, dt over the
+// substeps chosen (S/collision_cache_drift.comp:79-80).
 __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__ sims,
                                                      const BodyBlock* __restrict__ blocks,
                                                      float4* __restrict__ body,
                                                      const float2* __restrict__ acc,
                                                      const float4* __restrict__ mats,
                                                      float kickDt, float driftDt, int insert,
-                                                     GridBuf grid, SleepBufs sleep) {
+                                                     GridBuf grid, SleepBufs sleep,
+                                                     const uint32_t* __restrict__ adapt) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
+    if (adapt != nullptr && s.adaptive && driftDt != 0.0f)
+        driftDt = s.dt / (float)max(adapt[ADAPT_WORDS * b.sim + 2], 1u);
     float4 v = body[g];
     bool awake = true;
     if (s.sleepOn) {
@@ -384,6 +389,9 @@ __device__ __forceinline__ uint32_t count_contacts(const SimDesc& s, int g, floa
     return n;
 }
 
+// ADAPTIVE: some simulation of the batch has adaptive collisions; a substep
+// pass at or past its chosen count does nothing (S/collision_cache_count.comp:131).
+template <bool ADAPTIVE>
 __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
                                               const OrderedBlock* __restrict__ order,
                                               uint32_t* __restrict__ waveWork, GridBuf grid,
@@ -391,14 +399,18 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
                                               const float4* __restrict__ mats,
                                               uint32_t* __restrict__ counts,
                                               uint32_t* __restrict__ contacts, int planeStride,
-                                              int guardLimit, ContactPool pool, SleepBufs sleep) {
+                                              int guardLimit, ContactPool pool, SleepBufs sleep,
+                                              int pass, const uint32_t* __restrict__ adapt) {
     const OrderedBlock ob = order[blockIdx.x];
     const int bi = ob.index;
     const BodyBlock b{ob.sim, ob.first};
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
     uint32_t n = 0;
-    if (g - s.base < s.n) {
+    bool active = true;
+    if (ADAPTIVE && s.adaptive && pass >= 0)
+        active = (uint32_t)pass < adapt[ADAPT_WORDS * b.sim + 2];
+    if (g - s.base < s.n && active) {
         // The body's cell in the previous pass's grid and its own node here
         // load together (cellOf is always allocated, so the first load is
         // unconditional); the clearing store waits only for the first.
@@ -484,6 +496,7 @@ void identity_block_order(const BodyBlock* blocks, int count, OrderedBlock* orde
 
 struct Accum {
     float pcx, pcy, vcx, vcy, pressure;
+    float deepest;  // the deepest pair penetration of a pressure pass (adaptive state)
 };
 
 // 1/k for small contact counts: each workgroup of the solve divides once
@@ -621,6 +634,7 @@ __device__ __forceinline__ void accumulate_contact(const SolveK& s, const float*
         acc.vcy += fy;
     }
     if (writePressure) {
+        acc.deepest = fmaxf(acc.deepest, penetration);
         float approach = -rvn > 0.0f ? -rvn : 0.0f;
         acc.pressure += 0.5f * pairWeight * (penetration / s.subDt + approach);
     }
@@ -682,6 +696,13 @@ __device__ __forceinline__ void pool_next(const uint32_t* __restrict__ items, ui
 // unrolled; three contacts a batch measured faster than two or four.
 constexpr int SOLVE_BATCH = 3;
 
+// ADAPTIVE: some simulation of the batch has adaptive collisions. For such a
+// simulation a substep pass at or past the step's chosen count does nothing,
+// the others take dt over that count as their substep, the last of them
+// drifts no further and inserts for the final pass, and the final pass
+// (writePressure) stores each body's deepest pair penetration in `deepest`
+// for k_adapt_stats (S/collision_cache_solve.comp:239-243, 277-311).
+template <bool ADAPTIVE>
 __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               const OrderedBlock* __restrict__ order,
                                               const float4* __restrict__ bodyIn,
@@ -693,7 +714,9 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
                                               const float2* __restrict__ fallback,
                                               float* __restrict__ pressures, int writePressure,
                                               float nextDriftDt, int insertNext, GridBuf next,
-                                              ContactPool pool, SleepBufs sleep) {
+                                              ContactPool pool, SleepBufs sleep, int pass,
+                                              int lastInsert, const uint32_t* __restrict__ adapt,
+                                              float* __restrict__ deepest) {
     // The pool's entries were all written by the count; its counter starts
     // the next count from zero (no kernel of this launch reads it).
     if (blockIdx.x == 0 && threadIdx.x == 0) *pool.top = 0u;
@@ -701,7 +724,7 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     const OrderedBlock ob = order[blockIdx.x];
     const BodyBlock b{ob.sim, ob.first};
     const SimDesc& s = sims[b.sim];
-    const SolveK k = solve_k(s);
+    SolveK k = solve_k(s);
     // Each wave fills its own copy of the table (one entry a lane), so no
     // workgroup barrier holds the waves: a wave's LDS writes are complete
     // for its own later reads once the wave has waited for them.
@@ -715,6 +738,16 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
+    if (ADAPTIVE && s.adaptive) {
+        const uint32_t chosen = adapt[ADAPT_WORDS * b.sim + 2];
+        if (pass >= 0 && (uint32_t)pass >= chosen) return;
+        k.subDt = s.dt / (float)max(chosen, 1u);
+        if (pass >= 0) {
+            const bool lastActive = (uint32_t)pass + 1u >= chosen;
+            nextDriftDt = lastActive ? 0.0f : k.subDt;
+            insertNext = lastActive ? lastInsert : 1;
+        }
+    }
     float4 me = bodyIn[g];
     bool sleeping = false;
     uint32_t stamp = 0u;
@@ -732,7 +765,7 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     float4 ownMat = mats[g];
     float ppx = me.x * k.vw, ppy = me.y * k.vh;
     float vpx = me.z * k.vw, vpy = me.w * k.vh;
-    Accum acc{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    Accum acc{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t own = counts[g];
     // Up to CONTACT_SORTED contacts are listed in ascending order; a body
     // with more lists its CONTACT_SLOTS smallest and takes the rest below.
@@ -874,6 +907,110 @@ __global__ void __launch_bounds__(WG) k_solve(const SimDesc* __restrict__ sims,
     }
     bodyOut[g] = o;
     if (insertNext) grid_insert(s, g, o.x, o.y, next);
+    if (ADAPTIVE && writePressure && s.adaptive) deepest[g] = acc.deepest;
+}
+
+// Adaptive collisions, after the final pass (S/collision_cache_solve.comp:
+// 239-243, 309-311): the deepest pair penetration, the most contacts of a
+// body and the bodies at or above the dense contact count, over the bodies
+// the pass solved (active, not sleeping); a wave's figures are reduced
+// first, one atomic each a wave.
+__global__ void __launch_bounds__(WG) k_adapt_stats(const SimDesc* __restrict__ sims,
+                                                    const BodyBlock* __restrict__ blocks,
+                                                    const uint32_t* __restrict__ counts,
+                                                    const float* __restrict__ deepest,
+                                                    SleepBufs sleep, uint32_t* __restrict__ adapt) {
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[b.sim];
+    if (!s.adaptive) return;
+    int g = b.first + (int)threadIdx.x;
+    uint32_t most = 0u, pen = 0u, dense = 0u;
+    if (g - s.base < s.n) {
+        bool solved = true;
+        if (s.sleepOn) {
+            const SleepState st = sleep.state[b.sim];
+            solved = !sleep_on(s, st) || sleep.bodyAwake[g] == st.stamp;
+        }
+        if (solved) {
+            most = counts[g];
+            pen = __float_as_uint(deepest[g]);
+            dense = most >= (uint32_t)s.adaptDenseContacts ? 1u : 0u;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        most = max(most, (uint32_t)__shfl_xor((int)most, o));
+        pen = max(pen, (uint32_t)__shfl_xor((int)pen, o));
+        dense += (uint32_t)__shfl_xor((int)dense, o);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        uint32_t* st = adapt + ADAPT_WORDS * b.sim;
+        if (most != 0u) atomicMax(&st[3], most);
+        if (pen != 0u) atomicMax(&st[1], pen);
+        if (dense != 0u) atomicAdd(&st[4], dense);
+    }
+}
+
+// Adaptive collisions, once per step before the first pass
+// (S/collision_adaptive_reduce.comp): the largest speed in px/s of each
+// adaptive simulation's active bodies, as float bits (non-negative, so their
+// unsigned order is the float order), a wave's maximum first.
+__global__ void __launch_bounds__(WG) k_adapt_reduce(const SimDesc* __restrict__ sims,
+                                                     const BodyBlock* __restrict__ blocks,
+                                                     const float4* __restrict__ body,
+                                                     uint32_t* __restrict__ adapt) {
+    BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[b.sim];
+    if (!s.adaptive) return;
+    int g = b.first + (int)threadIdx.x;
+    uint32_t bits = 0u;
+    if (g - s.base < s.n) {
+        const float4 v = body[g];
+        const float sx = v.z * s.vw, sy = v.w * s.vh;
+        bits = __float_as_uint(sqrtf(sx * sx + sy * sy));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, o));
+    if ((threadIdx.x & 63u) == 0u && bits != 0u) atomicMax(&adapt[ADAPT_WORDS * b.sim + 5], bits);
+}
+
+// GLSL uint(x) of a non-negative ceil: exact below 2^32; larger values and
+// NaN (whose conversion GLSL leaves undefined) give the largest uint, which
+// the substep clamp then takes.
+__device__ __forceinline__ uint32_t ceil_to_uint(float x) {
+    const float c = ceilf(x);
+    if (!(c < 4294967296.0f)) return 0xffffffffu;
+    return c > 0.0f ? (uint32_t)c : 0u;
+}
+
+// S/collision_adaptive_finalize.comp:16-56, one lane per simulation: the
+// substeps of the step from its starting speed and the last final pass's
+// penetration and dense bodies (all of them on the first step, and while the
+// contact graph is dense), rounded up to the parity of the maximum so that
+// the ping-pong buffers keep their roles across the skipped passes; then the
+// last final pass's figures are cleared for this step's.
+__global__ void __launch_bounds__(64) k_adapt_finalize(const SimDesc* __restrict__ sims,
+                                                       int count, uint32_t* __restrict__ adapt) {
+    const int i = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (i >= count) return;
+    const SimDesc& s = sims[i];
+    if (!s.adaptive) return;
+    uint32_t* st = adapt + ADAPT_WORDS * i;
+    const bool first = st[2] == 0u;
+    const float speed = __uint_as_float(st[5]);
+    const float deepest = __uint_as_float(st[1]);
+    const uint32_t most = (uint32_t)s.adaptMax;
+    const uint32_t displacementSteps = ceil_to_uint(speed * s.dt / s.adaptDisp);
+    const uint32_t penetrationSteps = ceil_to_uint(deepest / s.adaptPen);
+    uint32_t required = first ? most : min(max(max(displacementSteps, penetrationSteps), 1u), most);
+    if (st[4] >= (uint32_t)s.adaptDenseBodies) required = most;
+    if ((required & 1u) != (most & 1u)) required = min(required + 1u, most);
+    st[0] = st[5];
+    st[1] = 0u;
+    st[2] = required;
+    st[3] = 0u;
+    st[4] = 0u;
+    st[5] = 0u;
 }
 
 // S/stop_bodies.comp:9-15: zero the velocity of every active body in both
@@ -904,29 +1041,45 @@ void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, co
                          const CollideGrid& grid) {
     if (ls.bodyBlocks == 0) return;
     launch(k_drift_insert, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
-                       body, acc, mats, kickDt, driftDt, insert, grid_of(grid), ls.sleep);
+                       body, acc, mats, kickDt, driftDt, insert, grid_of(grid), ls.sleep,
+                       (const uint32_t*)ls.adapt);
+}
+
+void launch_adapt(hipStream_t st, const LaunchShape& ls, int simCount, const float4* body) {
+    if (ls.bodyBlocks == 0 || ls.adapt == nullptr) return;
+    launch(k_adapt_reduce, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body,
+           ls.adapt);
+    launch(k_adapt_finalize, dim3((simCount + 63) / 64), dim3(64), 0, st, ls.sims, simCount,
+           ls.adapt);
 }
 
 void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
                   const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
-                  uint32_t* contacts) {
+                  uint32_t* contacts, int pass) {
     if (ls.bodyBlocks == 0) return;
-    launch(k_count, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.order,
-                       ls.waveWork,
-                       grid_of(grid), grid_of(prev), clearPrev, mats, counts, contacts,
-                       ls.planeStride, ls.guardLimit, ls.pool, ls.sleep);
+    launch(ls.adapt ? k_count<true> : k_count<false>, dim3(ls.bodyBlocks), dim3(WG), 0, st,
+           ls.sims, ls.order, ls.waveWork, grid_of(grid), grid_of(prev), clearPrev, mats, counts,
+           contacts, ls.planeStride, ls.guardLimit, ls.pool, ls.sleep, pass,
+           (const uint32_t*)ls.adapt);
 }
 
 void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, float4* bodyOut,
                   const float4* mats, const uint32_t* counts, const uint32_t* contacts,
                   const CollideGrid& grid, const float2* fallback, float* pressures,
-                  int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next) {
+                  int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next,
+                  int pass, int lastInsert) {
     if (ls.bodyBlocks == 0) return;
-    launch(k_solve, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.order,
-                       bodyIn,
-                       bodyOut, mats, counts, contacts, ls.planeStride, grid_of(grid),
-                       ls.guardLimit, fallback, pressures, writePressure, nextDriftDt, insertNext,
-                       grid_of(next), ls.pool, ls.sleep);
+    launch(ls.adapt ? k_solve<true> : k_solve<false>, dim3(ls.bodyBlocks), dim3(WG), 0, st,
+           ls.sims, ls.order, bodyIn, bodyOut, mats, counts, contacts, ls.planeStride,
+           grid_of(grid), ls.guardLimit, fallback, pressures, writePressure, nextDriftDt,
+           insertNext, grid_of(next), ls.pool, ls.sleep, pass, lastInsert,
+           (const uint32_t*)ls.adapt, ls.adaptDeepest);
+}
+
+void launch_adapt_stats(hipStream_t st, const LaunchShape& ls, const uint32_t* counts) {
+    if (ls.bodyBlocks == 0 || ls.adapt == nullptr) return;
+    launch(k_adapt_stats, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, counts,
+           (const float*)ls.adaptDeepest, ls.sleep, ls.adapt);
 }
 
 }  // namespace gs

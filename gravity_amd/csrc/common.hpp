@@ -104,7 +104,23 @@ struct SimDesc {
     float settling2;        // (0.005 d)^2: a pushed body is not at rest
     float tileMark2;        // (16 px)^2: a faster body also wakes where it heads
     int spad0, spad1;
+    // Adaptive collisions (S/collision_adaptive_finalize.comp, K/GravityPipeline.kt:1155-1211)
+    int adaptive;           // the device picks each step's substeps (contact passes on)
+    int adaptMax;           // maximumSubsteps
+    float adaptDisp;        // diameter * allowedDisplacementFraction, px
+    float adaptPen;         // diameter * allowedPenetrationFraction, px
+    int adaptDenseContacts; // denseContactThreshold
+    int adaptDenseBodies;   // max(minimumDenseBodies, ceil(n * denseBodyFraction))
+    int apad0, apad1;
 };
+
+// Adaptive state of a simulation, ADAPT_WORDS words (the reference's five,
+// then the running speed maximum of the step being measured): [0] the largest
+// speed the step started with (float bits), [1] the deepest pair penetration
+// of the last final pass, [2] the substeps chosen (0 before the first step),
+// [3] the most contacts of a body and [4] the bodies at or above the dense
+// contact count in the last final pass, [5] the running speed maximum.
+constexpr int ADAPT_WORDS = 8;
 
 // Per-simulation sleep state, advanced on the device at the end of a step
 // (K/SleepTiles.kt endStep): the step stamp (never 0) and the steps left in

@@ -58,6 +58,14 @@ static void validate(const gs_config& c) {
     req(c.friction >= 0.0f && std::isfinite(c.friction), "friction");
     req(c.coincident_distance_squared_epsilon > 0.0f, "coincident-distance-epsilon");
     req(c.neighbor_skin_pixels > 0.0f, "collision-skin");
+    // K/GravityConfig.kt:725-729
+    req(c.allowed_displacement_fraction >= 0.0f && c.allowed_displacement_fraction <= 1.0f,
+        "adaptive-displacement-fraction");
+    req(c.allowed_penetration_fraction >= 0.0f && c.allowed_penetration_fraction <= 1.0f,
+        "adaptive-penetration-fraction");
+    req(c.dense_contact_threshold >= 1, "adaptive-dense-contact-count");
+    req(c.dense_body_fraction >= 0.0f && c.dense_body_fraction <= 1.0f, "adaptive-dense-body-fraction");
+    req(c.minimum_dense_bodies >= 1, "adaptive-dense-min-bodies");
     req(c.kernel_radius_diameters >= 1.5f && c.kernel_radius_diameters <= 4.0f,
         "liquid kernel must be 1.5..4 diameters");
     req(c.liquid_iterations >= 1 && c.liquid_iterations <= 16, "liquid iterations must be in 1..16");
@@ -122,6 +130,11 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
         // whole batch (no kernel reads a member's own choice).
         if ((c.collisions_enabled != 0) != (cfgs[0].collisions_enabled != 0))
             throw ArgumentError("a batch shares collisionsEnabled");
+        // Adaptive members keep every substep pass in the schedule (their
+        // kernels skip the ones past the count they chose); a fixed member's
+        // substep count is the schedule's.
+        if ((c.adaptive_collisions != 0) != (cfgs[0].adaptive_collisions != 0))
+            throw ArgumentError("a batch shares adaptive collisions");
     }
     for (const auto& c : cfgs)
         for (const auto& o : cfgs)
@@ -260,6 +273,10 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     dBlocks_ = dalloc<BodyBlock>((size_t)maxBlocks);
     order_ = dalloc<OrderedBlock>((size_t)maxBlocks);
     waveWork_ = dalloc<uint32_t>((size_t)maxBlocks * (WG / 64));
+    adapt_ = dalloc<uint32_t>(sims_.size() * ADAPT_WORDS);
+    bool adaptiveBatch = false;
+    for (const auto& h : sims_) adaptiveBatch |= h.cfg.adaptive_collisions && h.cfg.collisions_enabled;
+    if (adaptiveBatch) adaptDeepest_ = dalloc<float>(S);
     int maxLines = 0;
     for (auto& h : sims_) maxLines += 2 * std::max(h.cfg.grid_width, h.cfg.grid_height) * 2 + 2;
     dRowsF_ = dalloc<LineWork>((size_t)maxLines);
@@ -312,7 +329,7 @@ Engine::~Engine() {
                     liqCount_, liqPrefix_, liqCellOf_, liqUnordered_, liqItems_, lambdas_,
                     sleepState_, bodyAwake_, tileAwake_, tileActive_, dTileBlocks_,
                     dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_, poolItems_, poolBase_, poolTop_,
-                    bigList_, bigCounters_};
+                    bigList_, bigCounters_, adapt_, adaptDeepest_};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (st_) hipStreamDestroy(st_);
@@ -393,6 +410,14 @@ void Engine::fill_desc(HostSim& h) {
     d.lradius = 0.5f * c.contact_diameter_pixels;
     // K/SleepTiles.kt:51, 71-72 and S/collision_cache_solve.comp:452-466
     d.sleepOn = c.sleep && c.collisions_enabled ? 1 : 0;
+    // K/GravityPipeline.kt:1155-1211: the finalize shader's uniforms
+    d.adaptive = c.adaptive_collisions && c.collisions_enabled ? 1 : 0;
+    d.adaptMax = c.maximum_substeps;
+    d.adaptDisp = c.contact_diameter_pixels * c.allowed_displacement_fraction;
+    d.adaptPen = c.contact_diameter_pixels * c.allowed_penetration_fraction;
+    d.adaptDenseContacts = c.dense_contact_threshold;
+    d.adaptDenseBodies = std::max(c.minimum_dense_bodies,
+                                  (int)std::ceil((float)h.n * c.dense_body_fraction));
     float disp = c.sleep_speed_pixels * c.time_step;
     d.sleepDisp2 = disp * disp;
     float radius = 0.5f * c.contact_diameter_pixels, slack = 0.05f * c.contact_diameter_pixels;
@@ -434,7 +459,7 @@ void Engine::ensure_green(int s) {
 void Engine::rebuild_tables() {
     std::vector<BodyBlock> blocks;
     std::vector<LineWork> rowsF, rowsI, cols;
-    anyMutual_ = anyLiquid_ = anyCollide_ = anySleep_ = false;
+    anyMutual_ = anyLiquid_ = anyCollide_ = anySleep_ = anyAdaptive_ = false;
     gravitySlots_ = 0;
     liquidSlots_ = 0;
     anyViscous_ = false;
@@ -448,6 +473,7 @@ void Engine::rebuild_tables() {
             anyViscous_ |= h.cfg.liquid_viscosity > 0.0f;
         }
         anyCollide_ |= h.cfg.collisions_enabled != 0;
+        anyAdaptive_ |= h.desc.adaptive != 0;
         if (!h.mutual) continue;
         anyMutual_ = true;
         gravitySlots_ += h.n;
@@ -492,6 +518,8 @@ void Engine::rebuild_tables() {
     shape_.sleep = SleepBufs{sleepState_, bodyAwake_, tileAwake_, tileActive_};
     shape_.order = order_;
     shape_.waveWork = waveWork_;
+    shape_.adapt = anyAdaptive_ ? adapt_ : nullptr;
+    shape_.adaptDeepest = adaptDeepest_;
     shape_.pool = ContactPool{poolItems_, poolBase_, poolTop_, (uint32_t)totalSlots_ * CONTACT_SLOTS};
     invalidate_graphs();
 }
@@ -560,7 +588,9 @@ void Engine::set_field(int s, float ax, float ay) {
 
 void Engine::set_substeps(int substeps) {
     for (auto& h : sims_) {
-        int v = std::min(std::max(substeps, 1), h.cfg.maximum_substeps);
+        // K/GravityPipeline.kt:147-151: adaptive collisions choose their own
+        int v = h.cfg.adaptive_collisions ? h.cfg.maximum_substeps
+                                          : std::min(std::max(substeps, 1), h.cfg.maximum_substeps);
         if (v == h.substeps) continue;
         h.substeps = v;
         fill_desc(h);
@@ -724,12 +754,12 @@ void Engine::enqueue_pass(float kickDt, float driftDt, bool writePressure) {
                             driftDt, 1, grid_[g]);
     });
     timed("count", [&] {
-        launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+        launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_, -1);
     });
     timed("solve", [&] {
         launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
                      mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, writePressure ? 1 : 0,
-                     0.0f, 0, grid_[g ^ 1]);
+                     0.0f, 0, grid_[g ^ 1], -1, 0);
     });
     bodyPar_ ^= 1;
     gridPar_ ^= 1;
@@ -742,6 +772,8 @@ void Engine::enqueue_step(bool sortDue) {
     const float dt = h0.cfg.time_step;
     const float sub = h0.desc.subDt;
     const int S = h0.substeps;
+    // K/GravityPipeline.kt:458-461: adaptive simulations choose the step's substeps
+    if (anyAdaptive_) timed("adaptive", [&] { launch_adapt(st_, shape_, (int)sims_.size(), body_[bodyPar_]); });
     if (anyCollide_) {
         timed("drift_insert", [&] {
             launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_], dt, sub, 1,
@@ -750,15 +782,16 @@ void Engine::enqueue_step(bool sortDue) {
         for (int k = 0; k < S; k++) {
             int g = gridPar_;
             timed("count", [&] {
-                launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+                launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_, k);
             });
             bool last = k == S - 1;
             float nextDrift = last ? 0.0f : sub;
-            int insertNext = last ? (anyLiquid_ ? 0 : 1) : 1;
+            const int lastInsert = anyLiquid_ ? 0 : 1;
+            int insertNext = last ? lastInsert : 1;
             timed("solve", [&] {
                 launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
                              mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, 0, nextDrift,
-                             insertNext, grid_[g ^ 1]);
+                             insertNext, grid_[g ^ 1], k, lastInsert);
             });
             bodyPar_ ^= 1;
             gridPar_ ^= 1;
@@ -774,13 +807,14 @@ void Engine::enqueue_step(bool sortDue) {
         // The final zero-drift pass writes pressures (K/GravityPipeline.kt:480-486).
         int g = gridPar_;
         timed("count", [&] {
-            launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_);
+            launch_count(st_, shape_, grid_[g], grid_[g ^ 1], 1, mat_[auxPar_], counts_, contacts_, -1);
         });
         timed("solve", [&] {
             launch_solve(st_, shape_, body_[bodyPar_], body_[bodyPar_ ^ 1],
                          mat_[auxPar_], counts_, contacts_, grid_[g], fallback_, press_, 1, 0.0f, 0,
-                         grid_[g ^ 1]);
+                         grid_[g ^ 1], -1, 0);
         });
+        if (anyAdaptive_) timed("adaptive", [&] { launch_adapt_stats(st_, shape_, counts_); });
         bodyPar_ ^= 1;
         gridPar_ ^= 1;
         timed("block_order", [&] { launch_block_order(st_, shape_, order_); });

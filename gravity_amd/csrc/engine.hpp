@@ -99,6 +99,8 @@ public:
                          float velocityY, float diameter);
     int projectile_body_count(int s, float diameter) const;
     gs_cluster_statistics measure_cluster(int s);
+    // measureAdaptiveCollisions (K/GravityPipeline.kt:425-445)
+    gs_adaptive_statistics measure_adaptive(int s);
     void render_view(int s, void** bodies, void** pressures);
     void render_copy(int s, int count, void* bodies, void* pressures);
 
@@ -208,7 +210,9 @@ private:
     LineWork *dRowsF_ = nullptr, *dCols_ = nullptr, *dRowsI_ = nullptr;
     int bodyBlocks_ = 0, rowsF_ = 0, cols_ = 0, rowsI_ = 0, gravitySlots_ = 0, liquidSlots_ = 0;
     bool anyMutual_ = false, anyLiquid_ = false, anyCollide_ = false, anyViscous_ = false;
-    bool anySleep_ = false;
+    bool anySleep_ = false, anyAdaptive_ = false;
+    uint32_t* adapt_ = nullptr;      // ADAPT_WORDS per simulation (adaptive collisions)
+    float* adaptDeepest_ = nullptr;  // per body, with adaptive collisions
     LaunchShape shape_{};
 
     // graphs keyed by (bodyPar, auxPar, gridPar, sortDue), with the parities

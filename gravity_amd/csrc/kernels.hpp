@@ -42,6 +42,10 @@ struct LaunchShape {
     // once per step.
     const OrderedBlock* order;
     uint32_t* waveWork;
+    // ADAPT_WORDS per simulation when any simulation of the batch has
+    // adaptive collisions, else null (the passes then take the plain kernels).
+    uint32_t* adapt;
+    float* adaptDeepest;   // per body: its deepest pair penetration in the last final pass
 };
 
 // Mesh work item: {simulation, first line, lines}.
@@ -82,13 +86,24 @@ void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, co
 void launch_block_order(hipStream_t st, const LaunchShape& ls, OrderedBlock* order);
 // The order xcd_block alone gives, on the host.
 void identity_block_order(const BodyBlock* blocks, int count, OrderedBlock* order);
+// pass: the substep index of the pass (-1 for a pass outside the substeps,
+// always active); with adaptive collisions a substep at or past the step's
+// chosen count does nothing. lastInsert: whether the last active substep's
+// solve inserts into the next grid (it does unless liquid passes follow).
 void launch_count(hipStream_t st, const LaunchShape& ls, const CollideGrid& grid,
                   const CollideGrid& prev, int clearPrev, const float4* mats, uint32_t* counts,
-                  uint32_t* contacts);
+                  uint32_t* contacts, int pass);
 void launch_solve(hipStream_t st, const LaunchShape& ls, const float4* bodyIn, float4* bodyOut,
                   const float4* mats, const uint32_t* counts, const uint32_t* contacts,
                   const CollideGrid& grid, const float2* fallback, float* pressures,
-                  int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next);
+                  int writePressure, float nextDriftDt, int insertNext, const CollideGrid& next,
+                  int pass, int lastInsert);
+// Adaptive collisions: the largest speed of every adaptive simulation's
+// active bodies, then each one's substeps for the step
+// (S/collision_adaptive_{reset,reduce,finalize}.comp).
+void launch_adapt(hipStream_t st, const LaunchShape& ls, int simCount, const float4* body);
+// ... and after the final pass, its penetration and contact figures.
+void launch_adapt_stats(hipStream_t st, const LaunchShape& ls, const uint32_t* counts);
 
 // ---- P3M gravity ----
 void launch_bin(hipStream_t st, const LaunchShape& ls, const float4* body, uint32_t* cellCount,

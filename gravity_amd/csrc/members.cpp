@@ -8,6 +8,7 @@
 // library is built without contraction, as the JVM evaluates them).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "engine.hpp"
@@ -187,6 +188,21 @@ gs_cluster_statistics Engine::measure_cluster(int s) {
     out.max_speed_pixels_per_second = maxSpeed;
     out.average_speed_pixels_per_second = totalSpeed / n;
     out.non_finite_bodies = nonFinite;
+    return out;
+}
+
+gs_adaptive_statistics Engine::measure_adaptive(int s) {
+    sims_.at(s);
+    uint32_t st[ADAPT_WORDS];
+    check(hipStreamSynchronize(st_), "adaptive state");
+    check(hipMemcpy(st, adapt_ + (size_t)ADAPT_WORDS * s, sizeof(st), hipMemcpyDeviceToHost),
+          "adaptive state");
+    gs_adaptive_statistics out{};
+    std::memcpy(&out.maximum_speed_pixels_per_second, &st[0], 4);
+    std::memcpy(&out.maximum_penetration_pixels, &st[1], 4);
+    out.substeps = (int)st[2];
+    out.maximum_contact_count = (int)st[3];
+    out.dense_contact_bodies = (int)st[4];
     return out;
 }
 

@@ -58,6 +58,12 @@ class GsConfig(ctypes.Structure):
         ("field_acceleration_x", ctypes.c_float),
         ("field_acceleration_y", ctypes.c_float),
         ("collisions_enabled", ctypes.c_int),
+        ("adaptive_collisions", ctypes.c_int),
+        ("allowed_displacement_fraction", ctypes.c_float),
+        ("allowed_penetration_fraction", ctypes.c_float),
+        ("dense_contact_threshold", ctypes.c_int),
+        ("dense_body_fraction", ctypes.c_float),
+        ("minimum_dense_bodies", ctypes.c_int),
     ]
 
 
@@ -68,6 +74,13 @@ class GsClusterStatistics(ctypes.Structure):
                 ("core90_area_pixels", ctypes.c_double), ("core99_area_pixels", ctypes.c_double),
                 ("max_speed_pixels_per_second", ctypes.c_double),
                 ("average_speed_pixels_per_second", ctypes.c_double), ("non_finite_bodies", ctypes.c_int)]
+
+
+class GsAdaptiveStatistics(ctypes.Structure):
+    """Mirror of gs_adaptive_statistics (AdaptiveCollisionStatistics, K/GravityPipeline.kt:65-71)."""
+    _fields_ = [("maximum_speed_pixels_per_second", ctypes.c_float),
+                ("maximum_penetration_pixels", ctypes.c_float), ("substeps", ctypes.c_int),
+                ("maximum_contact_count", ctypes.c_int), ("dense_contact_bodies", ctypes.c_int)]
 
 
 class GsTuning(ctypes.Structure):
@@ -110,6 +123,7 @@ SIGNATURES = {
     "gs_projectile_diameter_for_body_count": (_I, [_F, _I, ctypes.POINTER(_F)]),
     "gs_measure_cluster": (_I, [_P, _I, ctypes.POINTER(GsClusterStatistics)]),
     "gs_supports_walls": (_I, [_I, _I]),
+    "gs_measure_adaptive_collisions": (_I, [_P, _I, ctypes.POINTER(GsAdaptiveStatistics)]),
     "gs_render_view": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "gs_render_copy": (_I, [_P, _I, _P, _P]),
     "gs_synchronize": (_I, [_P]),

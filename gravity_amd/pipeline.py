@@ -54,6 +54,21 @@ def projectile_diameter_for_body_count(contact_diameter: float, bodies: int) -> 
     return float(d.value)
 
 
+@dataclass(frozen=True)
+class AdaptiveCollisionStatistics:
+    """K/GravityPipeline.kt:65-71"""
+    maximumSpeedPixelsPerSecond: float
+    maximumPenetrationPixels: float
+    substeps: int
+    maximumContactCount: int
+    denseContactBodies: int
+
+    @staticmethod
+    def from_native(o) -> "AdaptiveCollisionStatistics":
+        return AdaptiveCollisionStatistics(o.maximum_speed_pixels_per_second, o.maximum_penetration_pixels,
+                                           o.substeps, o.maximum_contact_count, o.dense_contact_bodies)
+
+
 class GravityPipeline:
     """GravityPipeline(config, viewportWidth, viewportHeight, profiler=null,
     scenarioState=null, bounded=false) (K/GravityPipeline.kt:73-81)."""
@@ -82,6 +97,13 @@ class GravityPipeline:
         self._substeps = config.collisions.maximumSubsteps
         self.boundedSpaceSupported = GravityPipeline.supportsWalls(config.gravity)
         self._cfg = cfg
+
+    def measureAdaptiveCollisions(self) -> AdaptiveCollisionStatistics:
+        """K/GravityPipeline.kt:425-445: blocking end-of-run diagnostic of the
+        adaptive collision state after the last step."""
+        out = native.GsAdaptiveStatistics()
+        native.check(self._lib.gs_measure_adaptive_collisions(self._h, 0, ctypes.byref(out)))
+        return AdaptiveCollisionStatistics.from_native(out)
 
     # -- launch shapes (no reference counterpart) ----------------------------
     @property
@@ -170,8 +192,10 @@ class GravityPipeline:
 
     @collisionSubsteps.setter
     def collisionSubsteps(self, value: int) -> None:
+        """K/GravityPipeline.kt:147-151: adaptive collisions choose their own."""
         native.check(self._lib.gs_set_substeps(self._h, int(value)))
-        self._substeps = min(max(int(value), 1), self.config.collisions.maximumSubsteps)
+        k = self.config.collisions
+        self._substeps = k.maximumSubsteps if k.adaptive else min(max(int(value), 1), k.maximumSubsteps)
 
     # -- the step (K/GravityPipeline.kt:366-494) ----------------------------
     def initializeAcceleration(self, activeBodies: int) -> None:

@@ -78,6 +78,15 @@ typedef struct gs_config {
     /* Extension without a reference counterpart: 0 replaces the contact
      * passes by plain drifts (the gravity-only benchmark configurations). */
     int collisions_enabled;
+    /* adaptive collisions (CollisionConfig.adaptive and its fractions,
+     * K/GravityConfig.kt:686-705): the device picks each step's substep
+     * count (K/GravityPipeline.kt:1155-1211) */
+    int adaptive_collisions;
+    float allowed_displacement_fraction;
+    float allowed_penetration_fraction;
+    int dense_contact_threshold;
+    float dense_body_fraction;
+    int minimum_dense_bodies;
 } gs_config;
 
 typedef struct gs_sim gs_sim;
@@ -175,6 +184,21 @@ typedef struct gs_cluster_statistics {
     int non_finite_bodies;
 } gs_cluster_statistics;
 int gs_measure_cluster(gs_sim *sim, int active, gs_cluster_statistics *out);
+
+/* measureAdaptiveCollisions (K/GravityPipeline.kt:425-445,
+ * AdaptiveCollisionStatistics :65-71): the adaptive state of simulation
+ * `index` (0 for a plain handle) after the last step: the largest speed the
+ * step started with, the deepest pair penetration, the substeps it chose, the
+ * most contacts of one body and the bodies at or above the dense contact
+ * count, the last three from the step's final pass. Blocks. */
+typedef struct gs_adaptive_statistics {
+    float maximum_speed_pixels_per_second;
+    float maximum_penetration_pixels;
+    int substeps;
+    int maximum_contact_count;
+    int dense_contact_bodies;
+} gs_adaptive_statistics;
+int gs_measure_adaptive_collisions(gs_sim *sim, int index, gs_adaptive_statistics *out);
 
 /* GravityPipeline.supportsWalls(gravity) (K/GravityPipeline.kt:1628): 1 when
  * the doubled mesh fits the 2048-value FFT line, else 0. Host only. */

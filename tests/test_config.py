@@ -140,3 +140,50 @@ def test_batch_rejects_mixed_collision_schedules_before_touching_the_gpu():
     b = GravityConfig(particles=ParticleConfig(capacity=100, initialBodies=1), collisionsEnabled=False)
     with pytest.raises(ValueError, match="shares collisionsEnabled"):
         GravityBatch([a, b], [(256, 256), (256, 256)])
+
+
+def test_adaptive_collision_options():
+    # K/GravityConfig.kt:72-83, 336-372, 686-705, 725-729
+    cfg = GravityConfig.parse(["--adaptive-collisions=true", "--adaptive-displacement-fraction=0.5",
+                               "--adaptive-penetration-fraction=0.25", "--adaptive-dense-contact-count=6",
+                               "--adaptive-dense-body-fraction=0.02", "--adaptive-dense-min-bodies=64",
+                               "--gpu-budget-ms=12"], {})
+    k = cfg.collisions
+    assert k.adaptive and k.allowedDisplacementFraction == 0.5 and k.allowedPenetrationFraction == 0.25
+    assert (k.denseContactThreshold, k.denseBodyFraction, k.minimumDenseBodies) == (6, 0.02, 64)
+    assert k.gpuBudgetMillis == 12.0
+    c = cfg.native(640, 320, False)
+    assert c.adaptive_collisions == 1 and c.dense_contact_threshold == 6 and c.minimum_dense_bodies == 64
+    assert abs(c.allowed_displacement_fraction - 0.5) < 1e-7 and abs(c.dense_body_fraction - 0.02) < 1e-7
+    d = CollisionConfig()
+    assert not d.adaptive and (d.allowedDisplacementFraction, d.allowedPenetrationFraction) == (0.25, 0.125)
+    assert (d.denseContactThreshold, d.denseBodyFraction, d.minimumDenseBodies) == (4, 0.01, 256)
+    for bad in (["--adaptive-displacement-fraction=1.5"], ["--adaptive-penetration-fraction=-0.1"],
+                ["--adaptive-dense-contact-count=0"], ["--adaptive-dense-body-fraction=2"],
+                ["--adaptive-dense-min-bodies=0"], ["--gpu-budget-ms=-1"]):
+        with pytest.raises(ValueError):
+            GravityConfig.parse(bad, {})
+
+
+def test_library_validates_the_adaptive_fields_before_touching_the_gpu():
+    from gravity_amd import native
+    lib = native.load()
+    h = ctypes.c_void_p()
+    for field, value, message in [("allowed_displacement_fraction", 1.5, b"adaptive-displacement-fraction"),
+                                  ("allowed_penetration_fraction", -1.0, b"adaptive-penetration-fraction"),
+                                  ("dense_contact_threshold", 0, b"adaptive-dense-contact-count"),
+                                  ("dense_body_fraction", 2.0, b"adaptive-dense-body-fraction"),
+                                  ("minimum_dense_bodies", 0, b"adaptive-dense-min-bodies")]:
+        c = native.defaults()
+        setattr(c, field, value)
+        assert lib.gs_create(ctypes.byref(c), None, 0, ctypes.byref(h)) == native.GS_EINVAL, field
+        assert message in lib.gs_last_error(), field
+
+
+def test_batch_rejects_mixed_adaptive_collisions_before_touching_the_gpu():
+    from gravity_amd import GravityBatch, GravityConfig, ParticleConfig
+    a = GravityConfig(particles=ParticleConfig(capacity=100, initialBodies=1))
+    b = GravityConfig(particles=ParticleConfig(capacity=100, initialBodies=1),
+                      collisions=CollisionConfig(adaptive=True))
+    with pytest.raises(ValueError, match="shares adaptive collisions"):
+        GravityBatch([a, b], [(256, 256), (256, 256)])
