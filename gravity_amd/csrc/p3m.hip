@@ -214,45 +214,46 @@ __device__ __forceinline__ void body_term(const SimDesc& s, float4 o, float x, f
 // Near field of rx = ry = R cells for exactNeighborBodyLimit <= NEAR_INLINE:
 // the column offsets and Gaussian factors of a body are computed once; the
 // home cell's bodies (its record hrec is loaded with the mesh force) load
-// together; then each stencil row loads the 32-byte near records of its
-// cells together, which hold the count, the mass and, for a cell taken body
-// by body, its bodies, so a row costs one memory round trip.
-template <int R>
-__device__ __forceinline__ void near_field_fixed(const SimDesc& s, uint4 hrec,
-                                                 const uint4* __restrict__ cellNear,
-                                                 const float4* __restrict__ cellBody, int g,
-                                                 float x, float y, float& ax, float& ay) {
+// together; then each stencil row's 32-byte near records, which hold the
+// count, the mass and, for a cell taken body by body, its bodies, from the
+// workgroup's LDS stage of the records around its bodies or, for a block
+// whose box does not fit the stage, from global memory (near_field_rows);
+// both add the same terms in the same order.
+
+// The home cell: its first homeLimit list entries, the body itself skipped
+// (under walls, none for a body on or beyond the far edge).
+__device__ __forceinline__ void near_home(const SimDesc& s, uint4 hrec,
+                                          const float4* __restrict__ cellBody, int g, int hx,
+                                          int hy, float x, float y, float k2, float& ax,
+                                          float& ay) {
 #pragma clang fp contract(fast)
-    const float k2 = s.invTwoSigma2;
-    const float cellMassUnit = 1.0f / 256.0f;
-    const uint32_t exactLimit = (uint32_t)s.exactLimit;
-    int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
-    constexpr int D = 2 * R + 1;
     constexpr int HOME_BATCH = 8;
-    // The home cell: its first homeLimit list entries, the body itself
-    // skipped (under walls, none for a body on or beyond the far edge).
-    {
-        const bool homeOk = !s.bounded || (hx >= 0 && hx < s.pw && hy >= 0 && hy < s.ph);
-        const int take = homeOk ? min((int)hrec.y, s.homeLimit) : 0;
-        // The home cell holds the body itself, so its list is never empty and
-        // the loads, clamped to the list, are unconditional (no load waits on
-        // a branch).
-        const int last = max((int)hrec.y, 1) - 1;
-        float4 hb[HOME_BATCH];
+    const bool homeOk = !s.bounded || (hx >= 0 && hx < s.pw && hy >= 0 && hy < s.ph);
+    const int take = homeOk ? min((int)hrec.y, s.homeLimit) : 0;
+    // The home cell holds the body itself, so its list is never empty and
+    // the loads, clamped to the list, are unconditional (no load waits on
+    // a branch).
+    const int last = max((int)hrec.y, 1) - 1;
+    float4 hb[HOME_BATCH];
 #pragma unroll
-        for (int t = 0; t < HOME_BATCH; t++) hb[t] = cellBody[hrec.x + (uint32_t)min(t, last)];
+    for (int t = 0; t < HOME_BATCH; t++) hb[t] = cellBody[hrec.x + (uint32_t)min(t, last)];
 #pragma unroll
-        for (int t = 0; t < HOME_BATCH; t++)
-            if (t < take && __float_as_uint(hb[t].w) != (uint32_t)g) body_term(s, hb[t], x, y, k2, ax, ay);
-        for (int t = HOME_BATCH; t < take; t++) {
-            float4 o = cellBody[hrec.x + (uint32_t)t];
-            if (__float_as_uint(o.w) != (uint32_t)g) body_term(s, o, x, y, k2, ax, ay);
-        }
+    for (int t = 0; t < HOME_BATCH; t++)
+        if (t < take && __float_as_uint(hb[t].w) != (uint32_t)g) body_term(s, hb[t], x, y, k2, ax, ay);
+    for (int t = HOME_BATCH; t < take; t++) {
+        float4 o = cellBody[hrec.x + (uint32_t)t];
+        if (__float_as_uint(o.w) != (uint32_t)g) body_term(s, o, x, y, k2, ax, ay);
     }
-    float cpx[D], cex[D];
+}
+
+// The column offsets, Gaussian column factors and the columns that take part.
+template <int R>
+__device__ __forceinline__ uint32_t near_columns(const SimDesc& s, int hx, float x, float k2,
+                                                 float (&cpx)[2 * R + 1], float (&cex)[2 * R + 1]) {
+#pragma clang fp contract(fast)
     uint32_t colOk = 0u;
 #pragma unroll
-    for (int k = 0; k < D; k++) {
+    for (int k = 0; k < 2 * R + 1; k++) {
         int cx = hx + k - R;
         if (!s.bounded || (cx >= 0 && cx < s.pw)) colOk |= 1u << k;
         int wx = wrap_axis(cx, s.pw);
@@ -260,42 +261,84 @@ __device__ __forceinline__ void near_field_fixed(const SimDesc& s, uint4 hrec,
         cpx[k] = px;
         cex[k] = __expf(-(px * px) * k2);
     }
+    return colOk;
+}
+
+// One stencil cell of a row from its near record (lo, hi): a monopole at the
+// cell centre, or its one or two bodies.
+__device__ __forceinline__ void near_cell(const SimDesc& s, uint32_t count, uint4 lo, uint4 hi,
+                                          float px, float ex, float py, float py2, float ey,
+                                          float x, float y, float k2, float& ax, float& ay) {
+#pragma clang fp contract(fast)
+    const float cellMassUnit = 1.0f / 256.0f;
+    if (count > (uint32_t)s.exactLimit) {
+        near_term(s, px, py, px * px + py2, ex * ey, (float)lo.y * cellMassUnit, ax, ay);
+    } else {
+        if (count > 0u)
+            body_term(s, make_float4(__uint_as_float(lo.z), __uint_as_float(lo.w),
+                                     __uint_as_float(hi.w), 0.0f),
+                      x, y, k2, ax, ay);
+        if (count > 1u)
+            body_term(s, make_float4(__uint_as_float(hi.x), __uint_as_float(hi.y),
+                                     __uint_as_float(hi.z), 0.0f),
+                      x, y, k2, ax, ay);
+    }
+}
+
+// The row's y offset, its Gaussian factor, and whether it takes part.
+__device__ __forceinline__ bool near_row(const SimDesc& s, int cy, float y, float k2, int& wy,
+                                         float& py, float& py2, float& ey) {
+#pragma clang fp contract(fast)
+    if (s.bounded && (cy < 0 || cy >= s.ph)) return false;
+    wy = wrap_axis(cy, s.ph);
+    py = min_image(s, ((float)wy + 0.5f) * s.invPh - y);
+    py2 = py * py;
+    ey = __expf(-py2 * k2);
+    return true;
+}
+
+// The same rows from `Rows` (a record source: rec(cy, cx) gives the near
+// record of the cell at unwrapped coordinates (cx, cy)), each cell's record
+// read as the row reaches it: the workgroup's LDS stage (StagedRows) or
+// global memory (GlobalRows).
+struct StagedRows {
+    const uint4* stage;
+    int x0, y0, bw;
+    __device__ __forceinline__ const uint4* rec(const SimDesc&, int cy, int cx) const {
+        return stage + 2 * ((cy - y0) * bw + (cx - x0));
+    }
+};
+struct GlobalRows {
+    const uint4* __restrict__ cellNear;
+    __device__ __forceinline__ const uint4* rec(const SimDesc& s, int cy, int cx) const {
+        return cellNear + 2 * (size_t)(s.physBase + wrap_axis(cy, s.ph) * s.pw + wrap_axis(cx, s.pw));
+    }
+};
+
+template <int R, typename Rows>
+__device__ __forceinline__ void near_field_rows(const SimDesc& s, uint4 hrec, const Rows& rows,
+                                                const float4* __restrict__ cellBody, int g,
+                                                float x, float y, float& ax, float& ay) {
+    const float k2 = s.invTwoSigma2;
+    int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
+    constexpr int D = 2 * R + 1;
+    near_home(s, hrec, cellBody, g, hx, hy, x, y, k2, ax, ay);
+    float cpx[D], cex[D];
+    const uint32_t colOk = near_columns<R>(s, hx, x, k2, cpx, cex);
 #pragma unroll 1
     for (int j = 0; j < D; j++) {
-        int cy = hy + j - R;
-        if (s.bounded && (cy < 0 || cy >= s.ph)) continue;
-        int wy = wrap_axis(cy, s.ph);
-        float py = min_image(s, ((float)wy + 0.5f) * s.invPh - y);
-        float py2 = py * py;
-        float ey = __expf(-py2 * k2);
-        const uint4* row = cellNear + 2 * (size_t)(s.physBase + wy * s.pw);
+        int wy;
+        float py, py2, ey;
+        const int cy = hy + j - R;
+        if (!near_row(s, cy, y, k2, wy, py, py2, ey)) continue;
         const uint32_t rowOk = j == R ? colOk & ~(1u << R) : colOk;
-        // Every wrapped column is a valid record: the loads are unconditional
-        // and a column that does not take part reads as an empty cell.
-        uint4 lo[D], hi[D];
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            const uint4* rec = row + 2 * wrap_axis(hx + k - R, s.pw);
-            lo[k] = rec[0];
-            hi[k] = rec[1];
-        }
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            const uint32_t count = (rowOk >> k) & 1u ? lo[k].x : 0u;
-            if (count > exactLimit) {
-                float px = cpx[k];
-                near_term(s, px, py, px * px + py2, cex[k] * ey, (float)lo[k].y * cellMassUnit,
-                          ax, ay);
-            } else {
-                if (count > 0u)
-                    body_term(s, make_float4(__uint_as_float(lo[k].z), __uint_as_float(lo[k].w),
-                                             __uint_as_float(hi[k].w), 0.0f),
-                              x, y, k2, ax, ay);
-                if (count > 1u)
-                    body_term(s, make_float4(__uint_as_float(hi[k].x), __uint_as_float(hi[k].y),
-                                             __uint_as_float(hi[k].z), 0.0f),
-                              x, y, k2, ax, ay);
-            }
+            const uint4* rec = rows.rec(s, cy, hx + k - R);
+            const uint4 lo = rec[0];
+            const uint32_t count = (rowOk >> k) & 1u ? lo.x : 0u;
+            const uint4 hi = count - 1u < (uint32_t)NEAR_INLINE ? rec[1] : make_uint4(0u, 0u, 0u, 0u);
+            near_cell(s, count, lo, hi, cpx[k], cex[k], py, py2, ey, x, y, k2, ax, ay);
         }
     }
 }
@@ -337,9 +380,19 @@ __device__ __forceinline__ void near_field_any(const SimDesc& s, const uint4* __
 }
 
 // S/kick.comp:105-156. A simulation with the default 3-cell near field (and
-// exact cells of at most NEAR_INLINE bodies) takes near_field_fixed<3>, any
+// exact cells of at most NEAR_INLINE bodies) takes near_field_rows<3>, any
 // other near_field_any; the choice is per simulation, so a simulation kicks
 // the same alone or in a batch.
+//
+// For the fixed near field the workgroup first stages in LDS the near records
+// of the box of cells its bodies' stencils cover (the home cells' bounding box
+// grown by the radius), one coalesced copy of up to KICK_STAGE records, and
+// its lanes read their rows from there: the 256 bodies of a spatially sorted
+// block share most of their 49 stencil cells, which each would otherwise
+// fetch from L2 on its own. A block whose box is larger (a sparse region, a
+// block across the torus seam, a non-finite position) reads global memory.
+constexpr int KICK_STAGE = 640;   // near records a workgroup stages (20 KB of LDS)
+
 __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
                                              const BodyBlock* __restrict__ blocks,
                                              float4* __restrict__ body,
@@ -349,18 +402,23 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
                                              const uint4* __restrict__ cellNear,
                                              const float4* __restrict__ cellBody, int initial,
                                              SleepBufs sleep) {
+    __shared__ uint4 stage[2 * KICK_STAGE];
+    __shared__ int waveBox[WG / 64][4];
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
     int g = b.first + (int)threadIdx.x;
-    if (g - s.base >= s.n) return;
+    // Every lane stays to the staging barriers; a lane without a body to kick
+    // leaves after them.
+    bool live = g - s.base < s.n;
     // S/kick.comp:109-110: a sleeping body is not kicked.
-    if (!initial && s.sleepOn) {
+    if (live && !initial && s.sleepOn) {
         const SleepState st = sleep.state[b.sim];
-        if (sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp) return;
+        if (sleep_on(s, st) && sleep.bodyAwake[g] != st.stamp) live = false;
     }
     float h = 0.5f * s.dt;
-    float4 p = body[g];
+    float4 p = live ? body[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (!s.mutual) {
+        if (!live) return;
         if (!initial) {
             p.z += s.fieldX * h;
             p.w += s.fieldY * h;
@@ -369,13 +427,67 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
         acc[g] = make_float2(s.fieldX, s.fieldY);
         return;
     }
+    const bool fixed = s.rx == 3 && s.ry == 3 && s.exactLimit <= NEAR_INLINE;
+    bool staged = false;
+    int x0 = 0, y0 = 0, bw = 0;
+    if (fixed) {
+        constexpr int R = 3;
+        // the home cells' bounding box over the workgroup's bodies
+        const int hx = live ? (int)floorf(p.x * (float)s.pw) : 0x3fffffff;
+        const int hy = live ? (int)floorf(p.y * (float)s.ph) : 0x3fffffff;
+        int lox = hx, hix = live ? hx : -0x3fffffff, loy = hy, hiy = live ? hy : -0x3fffffff;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lox = min(lox, __shfl_xor(lox, o));
+            hix = max(hix, __shfl_xor(hix, o));
+            loy = min(loy, __shfl_xor(loy, o));
+            hiy = max(hiy, __shfl_xor(hiy, o));
+        }
+        const int wave = (int)(threadIdx.x >> 6);
+        if ((threadIdx.x & 63u) == 0u) {
+            waveBox[wave][0] = lox;
+            waveBox[wave][1] = hix;
+            waveBox[wave][2] = loy;
+            waveBox[wave][3] = hiy;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < WG / 64; w++) {
+            lox = min(lox, waveBox[w][0]);
+            hix = max(hix, waveBox[w][1]);
+            loy = min(loy, waveBox[w][2]);
+            hiy = max(hiy, waveBox[w][3]);
+        }
+        // Only boxes of cells inside the grid are staged (which also keeps
+        // the sizes below from overflowing).
+        if (lox >= 0 && hix < s.pw && loy >= 0 && hiy < s.ph && lox <= hix && loy <= hiy) {
+            x0 = lox - R;
+            y0 = loy - R;
+            bw = hix - lox + 2 * R + 1;
+            const int bh = hiy - loy + 2 * R + 1;
+            staged = bw * bh <= KICK_STAGE;
+            if (staged) {
+                for (int i = (int)threadIdx.x; i < bw * bh; i += WG) {
+                    const int cy = y0 + i / bw, cx = x0 + i % bw;
+                    const size_t c = (size_t)(s.physBase + wrap_axis(cy, s.ph) * s.pw + wrap_axis(cx, s.pw));
+                    stage[2 * i] = cellNear[2 * c];
+                    stage[2 * i + 1] = cellNear[2 * c + 1];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
     float ax, ay;
-    if (s.rx == 3 && s.ry == 3 && s.exactLimit <= NEAR_INLINE) {
+    if (fixed) {
         const uint4 hrec = cellRec[home_cell(s, p.x, p.y)];
         float2 mf = mesh_force(s, force, p.x, p.y);
         ax = mf.x + s.fieldX;
         ay = mf.y + s.fieldY;
-        near_field_fixed<3>(s, hrec, cellNear, cellBody, g, p.x, p.y, ax, ay);
+        if (staged)
+            near_field_rows<3>(s, hrec, StagedRows{stage, x0, y0, bw}, cellBody, g, p.x, p.y, ax, ay);
+        else
+            near_field_rows<3>(s, hrec, GlobalRows{cellNear}, cellBody, g, p.x, p.y, ax, ay);
     } else {
         float2 mf = mesh_force(s, force, p.x, p.y);
         ax = mf.x + s.fieldX;
