@@ -57,7 +57,7 @@ bool supports_walls(int gridWidth, int gridHeight);
 // Launch shapes of the step that the autotuner chooses among; no choice
 // changes a result (each is checked bit for bit against the others).
 struct Tuning {
-    int bigLimit = 128;            // cells and tiles longer than this: workgroup sort (0: never)
+    int bigLimit = 0;              // cells and tiles longer than this: workgroup sort (0: never)
     int forwardItemBodies = 4096;  // bodies per forward row item, on average
     int depositRuns = 0;           // register runs in the deposit (0: an atomic per corner)
     int scanDirectBlocks = 1024;   // scans of more blocks scan their totals apart

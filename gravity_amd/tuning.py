@@ -21,7 +21,7 @@ from . import native
 
 @dataclass(frozen=True)
 class Tuning:
-    bigSegmentLimit: int = 128
+    bigSegmentLimit: int = 0
     forwardItemBodies: int = 4096
     depositRuns: int = 0
     scanDirectBlocks: int = 1024

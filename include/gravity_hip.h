@@ -276,7 +276,7 @@ int gs_set_graphs(gs_sim *sim, int enabled);
  * workgroup sizes are fixed in its shaders. No choice changes a result.
  *   big_segment_limit    a P3M cell or sort tile of more bodies is ordered by
  *                        one workgroup's LDS sort instead of per-lane ranks
- *                        (0: never; else at least 32; default 128)
+ *                        (0: never, the default; else at least 32)
  *   forward_item_bodies  bodies per forward-FFT row item, on average (the
  *                        deposit's load balance; 64 .. 2^24, default 4096)
  *   deposit_runs         1: sum a lane's bodies of one cell in registers
