@@ -145,6 +145,27 @@ class LiquidConfig:
 
 
 @dataclass(frozen=True)
+class TrajectoryConfig:
+    """TrajectoryConfig (K/GravityConfig.kt:739-758): the preview path."""
+    steps: int = 1024
+    maximumTimeStep: float = 1.0
+    targetSegmentPixels: float = 6.0
+    maximumPathViewportDiagonals: float = 1.0
+    updatesPerSecond: int = 30
+    minimumDashPixels: float = 3.0
+
+    def __post_init__(self):
+        _require(1 <= self.steps <= 65536, "trajectory-steps")
+        _require(self.maximumTimeStep > 0 and math.isfinite(self.maximumTimeStep), "trajectory-max-dt")
+        _require(self.targetSegmentPixels > 0 and math.isfinite(self.targetSegmentPixels),
+                 "trajectory-segment-px")
+        _require(self.maximumPathViewportDiagonals > 0 and math.isfinite(self.maximumPathViewportDiagonals),
+                 "trajectory-path-diagonals")
+        _require(1 <= self.updatesPerSecond <= 1000, "trajectory-updates-per-second")
+        _require(self.minimumDashPixels > 0 and math.isfinite(self.minimumDashPixels), "trajectory-min-dash-px")
+
+
+@dataclass(frozen=True)
 class StartConfig:
     """StartConfig.UNIFORM, the start benchmarks were measured with."""
     walls: bool = False
@@ -159,6 +180,7 @@ class GravityConfig:
     collisions: CollisionConfig = field(default_factory=CollisionConfig)
     liquid: LiquidConfig = field(default_factory=LiquidConfig)
     start: StartConfig = field(default_factory=StartConfig)
+    trajectory: TrajectoryConfig = field(default_factory=TrajectoryConfig)
     # Extension: False runs plain drifts instead of contact passes.
     collisionsEnabled: bool = True
 
@@ -288,8 +310,17 @@ class GravityConfig:
             mutualGravity=boolean("mutual-gravity", True),
             liquid=boolean("liquid", False),
         )
+        trajectory = TrajectoryConfig(
+            steps=integer("trajectory-steps", d.trajectory.steps),
+            maximumTimeStep=number("trajectory-max-dt", d.trajectory.maximumTimeStep),
+            targetSegmentPixels=number("trajectory-segment-px", d.trajectory.targetSegmentPixels),
+            maximumPathViewportDiagonals=number("trajectory-path-diagonals",
+                                                d.trajectory.maximumPathViewportDiagonals),
+            updatesPerSecond=integer("trajectory-updates-per-second", d.trajectory.updatesPerSecond),
+            minimumDashPixels=number("trajectory-min-dash-px", d.trajectory.minimumDashPixels),
+        )
         return GravityConfig(particles=particles, gravity=gravity, collisions=collisions, liquid=liquid,
-                             start=start)
+                             start=start, trajectory=trajectory)
 
     def native(self, width: int, height: int, bounded: bool) -> native.GsConfig:
         """The flat C struct gs_create takes."""
@@ -333,6 +364,12 @@ class GravityConfig:
         c.dense_contact_threshold = k.denseContactThreshold
         c.dense_body_fraction = k.denseBodyFraction
         c.minimum_dense_bodies = k.minimumDenseBodies
+        t = self.trajectory
+        c.trajectory_steps = t.steps
+        c.trajectory_maximum_time_step = t.maximumTimeStep
+        c.trajectory_target_segment_pixels = t.targetSegmentPixels
+        c.trajectory_maximum_path_diagonals = t.maximumPathViewportDiagonals
+        c.trajectory_minimum_dash_pixels = t.minimumDashPixels
         return c
 
 

@@ -79,6 +79,11 @@ void gs_config_defaults(gs_config* c) {
     c->dense_contact_threshold = 4;
     c->dense_body_fraction = 0.01f;
     c->minimum_dense_bodies = 256;
+    c->trajectory_steps = 1024;
+    c->trajectory_maximum_time_step = 1.0f;
+    c->trajectory_target_segment_pixels = 6.0f;
+    c->trajectory_maximum_path_diagonals = 1.0f;
+    c->trajectory_minimum_dash_pixels = 3.0f;
 }
 
 int gs_create(const gs_config* cfg, const float* state, int n, gs_sim** out) {
@@ -440,6 +445,26 @@ int gs_set_graphs(gs_sim* sim, int enabled) {
     SIM_OR_FAIL
     e.set_graphs(enabled != 0);
     return GS_OK;
+}
+
+int gs_update_trajectory(gs_sim* sim, int active, float origin_x, float origin_y, float velocity_x,
+                         float velocity_y) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        e.set_active(0, active);
+        e.update_trajectory(origin_x, origin_y, velocity_x, velocity_y);
+    });
+}
+
+int gs_trajectory_view(gs_sim* sim, void** vertices, int* vertex_count) {
+    SIM_OR_FAIL
+    return guarded([&] { e.trajectory_view(vertices, vertex_count); });
+}
+
+int gs_download_trajectory(gs_sim* sim, int first, int count, float* dst) {
+    SIM_OR_FAIL
+    if (count > 0 && !dst) return fail(GS_EINVAL, "null argument");
+    return guarded([&] { e.download_trajectory(first, count, dst); });
 }
 
 int gs_measure_adaptive_collisions(gs_sim* sim, int index, gs_adaptive_statistics* out) {

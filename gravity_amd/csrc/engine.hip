@@ -66,6 +66,17 @@ static void validate(const gs_config& c) {
     req(c.dense_contact_threshold >= 1, "adaptive-dense-contact-count");
     req(c.dense_body_fraction >= 0.0f && c.dense_body_fraction <= 1.0f, "adaptive-dense-body-fraction");
     req(c.minimum_dense_bodies >= 1, "adaptive-dense-min-bodies");
+    // K/GravityConfig.kt:747-756
+    req(c.trajectory_steps >= 1 && c.trajectory_steps <= 65536, "trajectory-steps");
+    req(c.trajectory_maximum_time_step > 0.0f && std::isfinite(c.trajectory_maximum_time_step),
+        "trajectory-max-dt");
+    req(c.trajectory_target_segment_pixels > 0.0f && std::isfinite(c.trajectory_target_segment_pixels),
+        "trajectory-segment-px");
+    req(c.trajectory_maximum_path_diagonals > 0.0f &&
+            std::isfinite(c.trajectory_maximum_path_diagonals),
+        "trajectory-path-diagonals");
+    req(c.trajectory_minimum_dash_pixels > 0.0f && std::isfinite(c.trajectory_minimum_dash_pixels),
+        "trajectory-min-dash-px");
     req(c.kernel_radius_diameters >= 1.5f && c.kernel_radius_diameters <= 4.0f,
         "liquid kernel must be 1.5..4 diameters");
     req(c.liquid_iterations >= 1 && c.liquid_iterations <= 16, "liquid iterations must be in 1..16");
@@ -274,6 +285,8 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     order_ = dalloc<OrderedBlock>((size_t)maxBlocks);
     waveWork_ = dalloc<uint32_t>((size_t)maxBlocks * (WG / 64));
     adapt_ = dalloc<uint32_t>(sims_.size() * ADAPT_WORDS);
+    trajectoryVertices_ = sims_[0].cfg.trajectory_steps * 6;
+    trajectory_ = dalloc<float4>((size_t)trajectoryVertices_);
     bool adaptiveBatch = false;
     for (const auto& h : sims_) adaptiveBatch |= h.cfg.adaptive_collisions && h.cfg.collisions_enabled;
     if (adaptiveBatch) adaptDeepest_ = dalloc<float>(S);
@@ -329,7 +342,7 @@ Engine::~Engine() {
                     liqCount_, liqPrefix_, liqCellOf_, liqUnordered_, liqItems_, lambdas_,
                     sleepState_, bodyAwake_, tileAwake_, tileActive_, dTileBlocks_,
                     dSims_, dBlocks_, dRowsF_, dRowsI_, dCols_, order_, waveWork_, poolItems_, poolBase_, poolTop_,
-                    bigList_, bigCounters_, adapt_, adaptDeepest_};
+                    bigList_, bigCounters_, adapt_, adaptDeepest_, trajectory_};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (st_) hipStreamDestroy(st_);

@@ -101,6 +101,11 @@ public:
     gs_cluster_statistics measure_cluster(int s);
     // measureAdaptiveCollisions (K/GravityPipeline.kt:425-445)
     gs_adaptive_statistics measure_adaptive(int s);
+    // updateTrajectory and its vertex buffer (K/GravityPipeline.kt:514-602),
+    // simulation 0
+    void update_trajectory(float originX, float originY, float velocityX, float velocityY);
+    void trajectory_view(void** vertices, int* count);
+    void download_trajectory(int first, int count, float* dst);
     void render_view(int s, void** bodies, void** pressures);
     void render_copy(int s, int count, void* bodies, void* pressures);
 
@@ -213,6 +218,8 @@ private:
     bool anySleep_ = false, anyAdaptive_ = false;
     uint32_t* adapt_ = nullptr;      // ADAPT_WORDS per simulation (adaptive collisions)
     float* adaptDeepest_ = nullptr;  // per body, with adaptive collisions
+    float4* trajectory_ = nullptr;   // simulation 0's preview vertices
+    int trajectoryVertices_ = 0;
     LaunchShape shape_{};
 
     // graphs keyed by (bodyPar, auxPar, gridPar, sortDue), with the parities

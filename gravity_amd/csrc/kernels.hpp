@@ -139,6 +139,27 @@ void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* ac
                  const float2* force, const uint4* cellRec, const uint4* cellNear,
                  const float4* cellBody, int initial);
 
+// ---- the trajectory preview (simulation 0 of the batch) ----
+// Near-field terms one stencil chunk of 64 cells may hold (the home cell's
+// homeLimit and 63 cells' exactLimit at most).
+constexpr int TRAJ_TERMS = 2048;
+struct TrajectoryInputs {
+    const float2* force;      // the mesh force of the last force build
+    const uint4* cellRec;     // its cell table
+    const float4* cellBody;   // its cell lists (body ids in .w)
+    const float4* body;       // the current bodies
+    const float4* mats;
+    int steps;                // TrajectoryConfig.steps
+    float maximumDt;          // maximumTimeStep
+    float segmentPx;          // targetSegmentPixels
+    float maxPathPx;          // maximumPathViewportDiagonals x the viewport diagonal
+    float minimumDashPx;      // minimumDashPixels
+    float originX, originY;   // placed (wrapped or clamped), px
+    float velocityX, velocityY;  // px/s
+};
+// out: steps x 6 vertices vec4(x, y, dash coordinate px, path fraction)
+void launch_trajectory(hipStream_t st, const SimDesc* sims, const TrajectoryInputs& in, float4* out);
+
 // ---- sleeping tiles: wake the next step's tiles, then advance the stamps ----
 void launch_sleep_advance(hipStream_t st, const SimDesc* sims, int simCount,
                           const BodyBlock* tileBlocks, int tileBlockCount, const SleepBufs& sleep);

@@ -206,6 +206,53 @@ gs_adaptive_statistics Engine::measure_adaptive(int s) {
     return out;
 }
 
+// K/GravityPipeline.kt:514-598: the uniforms of S/trajectory.comp, the
+// origin placed as placeX / placeY place it (:773-779).
+void Engine::update_trajectory(float originX, float originY, float velocityX, float velocityY) {
+    normalize();
+    const HostSim& h = sims_[0];
+    const SimDesc& d = h.desc;
+    if (d.mutual) {
+        const int cells = (2 * d.rx + 1) * (2 * d.ry + 1);
+        const long worst = (long)d.homeLimit + (long)(std::min(cells, 64) - 1) * d.exactLimit;
+        if (worst > TRAJ_TERMS)
+            throw ArgumentError("the trajectory preview holds 2048 near-field terms per 64 stencil cells");
+    }
+    const float W = (float)h.cfg.viewport_width, H = (float)h.cfg.viewport_height;
+    auto coerce = [](float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    TrajectoryInputs in{};
+    in.force = spec_;
+    in.cellRec = cellRec_;
+    in.cellBody = cellBody_;
+    in.body = body_[bodyPar_];
+    in.mats = mat_[auxPar_];
+    in.steps = h.cfg.trajectory_steps;
+    in.maximumDt = h.cfg.trajectory_maximum_time_step;
+    in.segmentPx = h.cfg.trajectory_target_segment_pixels;
+    in.maxPathPx = h.cfg.trajectory_maximum_path_diagonals * std::sqrt(W * W + H * H);
+    in.minimumDashPx = h.cfg.trajectory_minimum_dash_pixels;
+    in.originX = h.bounded ? coerce(originX, 0.0f, W * MAXIMUM_UNIT_F) : wrap_pixels(originX, h.cfg.viewport_width);
+    in.originY = h.bounded ? coerce(originY, 0.0f, H * MAXIMUM_UNIT_F) : wrap_pixels(originY, h.cfg.viewport_height);
+    in.velocityX = velocityX;
+    in.velocityY = velocityY;
+    launch_trajectory(st_, dSims_, in, trajectory_);
+    check(hipGetLastError(), "trajectory");
+}
+
+void Engine::trajectory_view(void** vertices, int* count) {
+    if (vertices) *vertices = trajectory_;
+    if (count) *count = trajectoryVertices_;
+}
+
+void Engine::download_trajectory(int first, int count, float* dst) {
+    if (first < 0 || count < 0 || first > trajectoryVertices_ || count > trajectoryVertices_ - first)
+        throw ArgumentError("trajectory range exceeds its vertices");
+    if (count == 0) return;
+    check(hipStreamSynchronize(st_), "trajectory");
+    check(hipMemcpy(dst, trajectory_ + first, (size_t)count * sizeof(float4), hipMemcpyDeviceToHost),
+          "trajectory");
+}
+
 bool supports_walls(int gridWidth, int gridHeight) {
     return 2 * std::max(gridWidth, gridHeight) <= MAXIMUM_FFT_AXIS;
 }

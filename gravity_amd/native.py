@@ -64,6 +64,11 @@ class GsConfig(ctypes.Structure):
         ("dense_contact_threshold", ctypes.c_int),
         ("dense_body_fraction", ctypes.c_float),
         ("minimum_dense_bodies", ctypes.c_int),
+        ("trajectory_steps", ctypes.c_int),
+        ("trajectory_maximum_time_step", ctypes.c_float),
+        ("trajectory_target_segment_pixels", ctypes.c_float),
+        ("trajectory_maximum_path_diagonals", ctypes.c_float),
+        ("trajectory_minimum_dash_pixels", ctypes.c_float),
     ]
 
 
@@ -124,6 +129,9 @@ SIGNATURES = {
     "gs_measure_cluster": (_I, [_P, _I, ctypes.POINTER(GsClusterStatistics)]),
     "gs_supports_walls": (_I, [_I, _I]),
     "gs_measure_adaptive_collisions": (_I, [_P, _I, ctypes.POINTER(GsAdaptiveStatistics)]),
+    "gs_update_trajectory": (_I, [_P, _I, _F, _F, _F, _F]),
+    "gs_trajectory_view": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_I)]),
+    "gs_download_trajectory": (_I, [_P, _I, _I, _P]),
     "gs_render_view": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "gs_render_copy": (_I, [_P, _I, _P, _P]),
     "gs_synchronize": (_I, [_P]),

@@ -98,6 +98,31 @@ class GravityPipeline:
         self.boundedSpaceSupported = GravityPipeline.supportsWalls(config.gravity)
         self._cfg = cfg
 
+    # -- the trajectory preview (K/GravityPipeline.kt:514-602) ---------------
+    @property
+    def trajectoryVertexCount(self) -> int:
+        return self.config.trajectory.steps * 6
+
+    def updateTrajectory(self, activeBodies: int, originX: float, originY: float, velocityX: float,
+                         velocityY: float) -> None:
+        native.check(self._lib.gs_update_trajectory(self._h, int(activeBodies), float(originX),
+                                                    float(originY), float(velocityX), float(velocityY)))
+
+    def trajectory(self) -> np.ndarray:
+        """The preview's vertices, (steps * 6, 4): x, y, dash coordinate px,
+        path fraction (-1 for an unused vertex)."""
+        n = self.trajectoryVertexCount
+        out = np.empty((n, 4), np.float32)
+        native.check(self._lib.gs_download_trajectory(self._h, 0, n, native.ptr(out)))
+        return out
+
+    def trajectoryView(self) -> tuple[int, int]:
+        """bindTrajectoryForRender: the vertex buffer's device address and vertex count."""
+        v = ctypes.c_void_p()
+        n = ctypes.c_int()
+        native.check(self._lib.gs_trajectory_view(self._h, ctypes.byref(v), ctypes.byref(n)))
+        return v.value or 0, n.value
+
     def measureAdaptiveCollisions(self) -> AdaptiveCollisionStatistics:
         """K/GravityPipeline.kt:425-445: blocking end-of-run diagnostic of the
         adaptive collision state after the last step."""

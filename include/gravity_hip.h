@@ -87,6 +87,12 @@ typedef struct gs_config {
     int dense_contact_threshold;
     float dense_body_fraction;
     int minimum_dense_bodies;
+    /* the trajectory preview (TrajectoryConfig, K/GravityConfig.kt:739-758) */
+    int trajectory_steps;
+    float trajectory_maximum_time_step;
+    float trajectory_target_segment_pixels;
+    float trajectory_maximum_path_diagonals;
+    float trajectory_minimum_dash_pixels;
 } gs_config;
 
 typedef struct gs_sim gs_sim;
@@ -184,6 +190,24 @@ typedef struct gs_cluster_statistics {
     int non_finite_bodies;
 } gs_cluster_statistics;
 int gs_measure_cluster(gs_sim *sim, int active, gs_cluster_statistics *out);
+
+/* updateTrajectory(activeBodies, originX, originY, velocityX, velocityY)
+ * (K/GravityPipeline.kt:514-598, S/trajectory.comp): the preview path of a
+ * test particle from (origin_x, origin_y) px at (velocity_x, velocity_y) px/s
+ * through the last force build and the current bodies of simulation 0, into
+ * the handle's vertex buffer: trajectory_steps x 6 vertices
+ * vec4(x, y, dash coordinate px, path fraction), unused ones (0, 0, 0, -1).
+ * Asynchronous on the handle's stream. GS_EINVAL when the near-field limits
+ * would put more than 2048 terms in one 64-cell chunk of the stencil. */
+int gs_update_trajectory(gs_sim *sim, int active, float origin_x, float origin_y, float velocity_x,
+                         float velocity_y);
+/* bindTrajectoryForRender (:600-602) and trajectoryVertexCount (:242-243):
+ * the device address of the vertex buffer and its vertex count (both
+ * nullable), for a renderer that reads HIP memory or maps it through
+ * HIP-GL interop. */
+int gs_trajectory_view(gs_sim *sim, void **vertices, int *vertex_count);
+/* Copies vertices [first, first + count) to host memory (4 floats each). Blocks. */
+int gs_download_trajectory(gs_sim *sim, int first, int count, float *dst);
 
 /* measureAdaptiveCollisions (K/GravityPipeline.kt:425-445,
  * AdaptiveCollisionStatistics :65-71): the adaptive state of simulation

@@ -952,6 +952,198 @@ void orc_sim_kick(orc_sim *s, int initial) {
     }
 }
 
+/* ---- the trajectory preview (S/trajectory.comp) -------------------------- */
+
+/* S/trajectory.comp:77-98: localAcceleration, the test particle being no body */
+static void trajectory_local(const orc_sim *s, float sx, float sy, float tx, float ty, float mass,
+                             float *ax, float *ay) {
+    float aspect = (float)s->p.width / (float)s->p.height;
+    float dx = sx - tx, dy = sy - ty;
+    if (!s->p.bounded) { dx -= floorf(dx + 0.5f); dy -= floorf(dy + 0.5f); }
+    float px = dx * aspect, py = dy * 1.0f;
+    float r2 = px * px + py * py;
+    float sigma = s->p.softening_cells / (float)s->ph;
+    float lw = expf(-r2 / (2.0f * sigma * sigma));
+    float cw = aspect / (float)s->pw, ch = 1.0f / (float)s->ph;
+    float md = s->p.min_dist_frac * (cw < ch ? cw : ch);
+    float scale = s->p.gravity * mass * lw / (s->total_mass * (r2 > md * md ? r2 : md * md));
+    *ax += scale * px / aspect;
+    *ay += scale * py / 1.0f;
+}
+
+/* S/trajectory.comp:124-168: accelerationAt */
+static void trajectory_acceleration(const orc_sim *s, float x, float y, float *ax, float *ay) {
+    float fieldx = s->p.field_x / (float)s->p.width, fieldy = s->p.field_y / (float)s->p.height;
+    if (!s->p.mutual_gravity) { *ax = fieldx; *ay = fieldy; return; }
+    interpolate_force(s, x, y, ax, ay);
+    *ax += fieldx;
+    *ay += fieldy;
+    int rx, ry;
+    orc_neighbor_radius(&s->p, &rx, &ry);
+    int hx = (int)floorf(x * (float)s->pw), hy = (int)floorf(y * (float)s->ph);
+    for (int dy = -ry; dy <= ry; dy++) {
+        for (int dx = -rx; dx <= rx; dx++) {
+            int cx = hx + dx, cy = hy + dy;
+            if (s->p.bounded && (cx < 0 || cy < 0 || cx >= s->pw || cy >= s->ph)) continue;
+            int wx = wrap_axis(cx, s->pw), wy = wrap_axis(cy, s->ph);
+            int index = wx + wy * s->pw;
+            int count = s->cell_count[index];
+            int home = dx == 0 && dy == 0;
+            if (!home && count > s->p.exact_limit) {
+                float cxn = ((float)wx + 0.5f) / (float)s->pw;
+                float cyn = ((float)wy + 0.5f) / (float)s->ph;
+                trajectory_local(s, cxn, cyn, x, y, (float)s->cell_mass[index] / 256.0f, ax, ay);
+            } else {
+                int limit = home ? s->p.home_limit : s->p.exact_limit;
+                int visited = 0;
+                for (int t = s->cell_start[index]; t < s->cell_start[index + 1] && visited < limit; t++) {
+                    int other = s->cell_list[t];
+                    trajectory_local(s, s->bodies[4 * other], s->bodies[4 * other + 1], x, y,
+                                     s->mats[4 * other], ax, ay);
+                    visited++;
+                }
+            }
+        }
+    }
+}
+
+/* S/trajectory.comp:170-186: integrationDt */
+static float trajectory_dt(const orc_sim *s, const orc_trajectory_params *t, float vx, float vy,
+                           float ax, float ay) {
+    float W = (float)s->p.width, H = (float)s->p.height;
+    float sx = vx * W, sy = vy * H, gx = ax * W, gy = ay * H;
+    float speed = sqrtf(sx * sx + sy * sy);
+    float accel = sqrtf(gx * gx + gy * gy);
+    float r;
+    if (accel > 1e-5f)
+        r = (sqrtf(speed * speed + 2.0f * accel * t->segment_px) - speed) / accel;
+    else
+        r = t->segment_px / (speed > 1e-5f ? speed : 1e-5f);
+    return clampf(r, s->p.dt, t->maximum_dt);
+}
+
+/* S/trajectory.comp:188-259: emitPeriodicStep */
+static void trajectory_emit(float *out, int step, float sx, float sy, float dx, float dy,
+                            float start_distance, float step_distance, float start_dash,
+                            float step_dash) {
+    float *v = out + (size_t)step * 6 * 4;
+    for (int k = 0; k < 6; k++) { v[4 * k] = 0.0f; v[4 * k + 1] = 0.0f; v[4 * k + 2] = 0.0f; v[4 * k + 3] = -1.0f; }
+    float ex = sx + dx, ey = sy + dy;
+    float cuts[4];
+    int cut_count = 2;
+    cuts[0] = 0.0f;
+    cuts[1] = 1.0f;
+    if (ex < 0.0f) cuts[cut_count++] = -sx / dx;
+    else if (ex >= 1.0f) cuts[cut_count++] = (1.0f - sx) / dx;
+    if (ey < 0.0f) cuts[cut_count++] = -sy / dy;
+    else if (ey >= 1.0f) cuts[cut_count++] = (1.0f - sy) / dy;
+    for (int index = 1; index < cut_count; index++) {
+        float value = clampf(cuts[index], 0.0f, 1.0f);
+        int insertion = index;
+        while (insertion > 0 && cuts[insertion - 1] > value) {
+            cuts[insertion] = cuts[insertion - 1];
+            insertion--;
+        }
+        cuts[insertion] = value;
+    }
+    float unique[4];
+    int unique_count = 0;
+    for (int index = 0; index < cut_count; index++)
+        if (unique_count == 0 || fabsf(cuts[index] - unique[unique_count - 1]) > 1e-6f)
+            unique[unique_count++] = cuts[index];
+    for (int piece = 0; piece < unique_count - 1; piece++) {
+        float f1 = unique[piece], f2 = unique[piece + 1];
+        float mid = 0.5f * (f1 + f2);
+        float tx = floorf(sx + dx * mid), ty = floorf(sy + dy * mid);
+        float *a = v + 4 * (piece * 2);
+        a[0] = clampf(sx + dx * f1 - tx, 0.0f, 1.0f);
+        a[1] = clampf(sy + dy * f1 - ty, 0.0f, 1.0f);
+        a[2] = start_dash + step_dash * f1;
+        a[3] = start_distance + step_distance * f1;
+        a[4] = clampf(sx + dx * f2 - tx, 0.0f, 1.0f);
+        a[5] = clampf(sy + dy * f2 - ty, 0.0f, 1.0f);
+        a[6] = start_dash + step_dash * f2;
+        a[7] = start_distance + step_distance * f2;
+    }
+}
+
+/* S/trajectory.comp:261-329 with the uniforms of K/GravityPipeline.kt:514-598 */
+void orc_sim_trajectory(const orc_sim *s, float origin_x, float origin_y, float vx_px, float vy_px,
+                        const orc_trajectory_params *t, float *out) {
+    float W = (float)s->p.width, H = (float)s->p.height;
+    /* placeX / placeY (K/GravityPipeline.kt:773-779) */
+    float ox, oy;
+    if (s->p.bounded) {
+        ox = clampf(origin_x, 0.0f, W * MAXIMUM_UNIT);
+        oy = clampf(origin_y, 0.0f, H * MAXIMUM_UNIT);
+    } else {
+        ox = origin_x - floorf(origin_x / W) * W;
+        oy = origin_y - floorf(origin_y / H) * H;
+    }
+    float max_path = t->path_diagonals * sqrtf(W * W + H * H);
+    float x = ox / W, y = oy / H;
+    float vx = vx_px / W, vy = vy_px / H;
+    float ax, ay;
+    trajectory_acceleration(s, x, y, &ax, &ay);
+    float path = 0.0f, dash = 0.0f;
+    int finished = 0;
+    for (int step = 0; step < t->steps; step++) {
+        if (finished) {
+            float *v = out + (size_t)step * 6 * 4;
+            for (int k = 0; k < 6; k++) { v[4 * k] = 0.0f; v[4 * k + 1] = 0.0f; v[4 * k + 2] = 0.0f; v[4 * k + 3] = -1.0f; }
+            continue;
+        }
+        float dt = trajectory_dt(s, t, vx, vy, ax, ay);
+        float hvx = vx + ax * (0.5f * dt), hvy = vy + ay * (0.5f * dt);
+        float dx = hvx * dt, dy = hvy * dt;
+        if (s->p.bounded) {
+            /* walls act like body contacts: the normal motion stops, and
+               rebounds by the restitution coefficient */
+            float nx = x + dx, ny = y + dy;
+            float cx = clampf(nx, 0.0f, MAXIMUM_UNIT), cy = clampf(ny, 0.0f, MAXIMUM_UNIT);
+            /* mix(v, -e v, hit) as GLSL forms it */
+            float hx = nx != cx ? 1.0f : 0.0f, hy = ny != cy ? 1.0f : 0.0f;
+            hvx = hvx * (1.0f - hx) + (-s->p.restitution * hvx) * hx;
+            hvy = hvy * (1.0f - hy) + (-s->p.restitution * hvy) * hy;
+            dx = cx - x;
+            dy = cy - y;
+        }
+        float sdx = dx * W, sdy = dy * H;
+        float step_distance = sqrtf(sdx * sdx + sdy * sdy);
+        float remaining = max_path - path;
+        if (step_distance > remaining) {
+            float k = remaining / step_distance;
+            dx *= k;
+            dy *= k;
+            step_distance = remaining;
+            finished = 1;
+        }
+        float speed_multiplier = 1.0f + sqrtf(hvx * hvx + hvy * hvy);
+        float dash_length = t->minimum_dash_px * speed_multiplier;
+        float step_dash = step_distance * t->minimum_dash_px / dash_length;
+        trajectory_emit(out, step, x, y, dx, dy, path, step_distance, dash, step_dash);
+        path += step_distance;
+        dash += step_dash;
+        if (s->p.bounded) {
+            x = x + dx;
+            y = y + dy;
+        } else {
+            x = x + dx - floorf(x + dx);
+            y = y + dy - floorf(y + dy);
+        }
+        if (path >= max_path - 1e-4f) {
+            finished = 1;
+            continue;
+        }
+        trajectory_acceleration(s, x, y, &ax, &ay);
+        vx = hvx + ax * (0.5f * dt);
+        vy = hvy + ay * (0.5f * dt);
+    }
+    float inv = 1.0f / (path > 1e-6f ? path : 1e-6f);
+    for (size_t k = 0; k < (size_t)t->steps * 6; k++)
+        if (out[4 * k + 3] >= 0.0f) out[4 * k + 3] = clampf(out[4 * k + 3] * inv, 0.0f, 1.0f);
+}
+
 /* K/GravityPipeline.kt:366-370 */
 void orc_sim_initialize_acceleration(orc_sim *s) {
     sim_normalize(s);

@@ -120,6 +120,19 @@ void orc_sim_build_force(orc_sim *s);
 void orc_sim_forces(const orc_sim *s, float *fx, float *fy);
 void orc_sim_density_spectrum(const orc_sim *s, float *spectrum);
 void orc_sim_kick(orc_sim *s, int initial);
+/* The trajectory preview (S/trajectory.comp, K/GravityPipeline.kt:514-598)
+ * of a test particle from (origin_x, origin_y) px at (vx, vy) px/s through the
+ * last force build and the current bodies: steps * 6 vertices vec4(x, y, dash
+ * coordinate px, path fraction) into out, unused ones (0, 0, 0, -1). */
+typedef struct orc_trajectory_params {
+    int steps;
+    float maximum_dt;            /* TrajectoryConfig.maximumTimeStep */
+    float segment_px;            /* targetSegmentPixels */
+    float path_diagonals;        /* maximumPathViewportDiagonals */
+    float minimum_dash_px;       /* minimumDashPixels */
+} orc_trajectory_params;
+void orc_sim_trajectory(const orc_sim *s, float origin_x, float origin_y, float vx, float vy,
+                        const orc_trajectory_params *t, float *out);
 void orc_sim_initialize_acceleration(orc_sim *s);
 void orc_sim_step(orc_sim *s);
 /* GravityPipeline.wake(): every body awake for the next two steps. */

@@ -13,6 +13,11 @@ HERE = Path(__file__).resolve().parent
 LIBRARY = HERE / "liboracle.so"
 
 
+class OrcTrajectoryParams(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("maximum_dt", ctypes.c_float), ("segment_px", ctypes.c_float),
+                ("path_diagonals", ctypes.c_float), ("minimum_dash_px", ctypes.c_float)]
+
+
 class OrcParams(ctypes.Structure):
     _fields_ = [
         ("width", ctypes.c_int), ("height", ctypes.c_int), ("bounded", ctypes.c_int),
@@ -87,6 +92,7 @@ def load():
             "orc_sim_last_sort_dest": (None, [P, P]),
             "orc_liquid_constants": (None, [F, F, P, P, P, P, P]),
             "orc_sim_resolve_liquid": (None, [P]),
+            "orc_sim_trajectory": (None, [P, F, F, F, F, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -299,3 +305,13 @@ class Sim:
 
     def resolve_liquid(self):
         load().orc_sim_resolve_liquid(self._h)
+
+    def trajectory(self, origin, velocity, steps=1024, maximum_dt=1.0, segment_px=6.0,
+                   path_diagonals=1.0, minimum_dash_px=3.0):
+        """The preview path (S/trajectory.comp): (steps * 6, 4) vertices."""
+        t = OrcTrajectoryParams(int(steps), float(maximum_dt), float(segment_px),
+                                float(path_diagonals), float(minimum_dash_px))
+        out = np.empty((steps * 6, 4), np.float32)
+        load().orc_sim_trajectory(self._h, float(origin[0]), float(origin[1]), float(velocity[0]),
+                                  float(velocity[1]), ctypes.byref(t), _p(out))
+        return out
