@@ -277,7 +277,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     }
     spec_ = dalloc<float2>((size_t)totalMesh_);
     green_ = dalloc<float2>((size_t)totalMesh_);
-    tw_ = dalloc<float2>((size_t)twLen_);
+    tw_ = dalloc<float2>((size_t)fft_twiddle_entries());
     fallback_ = dalloc<float2>(65536);
     dSims_ = dalloc<SimDesc>(sims_.size());
     int maxBlocks = totalSlots_ / WG;
@@ -297,7 +297,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
     dCols_ = dalloc<LineWork>((size_t)maxLines);
 
     std::vector<float> host;
-    fft_twiddles(twLen_, host);
+    fft_twiddle_table(host);
     check(hipMemcpy(tw_, host.data(), host.size() * 4, hipMemcpyHostToDevice), "twiddles");
     fallback_normals(host);
     check(hipMemcpy(fallback_, host.data(), host.size() * 4, hipMemcpyHostToDevice), "normals");
@@ -744,11 +744,11 @@ void Engine::enqueue_force() {
                               mat_[auxPar_], cellBody_, cellRec_, cellNear_, big());
         });
         timed("fft_rows_forward", [&] {
-            launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_,
+            launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_,
                                     tuning_.depositRuns != 0);
         });
-        timed("fft_columns", [&] { launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_); });
-        timed("fft_rows_inverse", [&] { launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_); });
+        timed("fft_columns", [&] { launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_); });
+        timed("fft_rows_inverse", [&] { launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_); });
     }
 }
 
@@ -1032,9 +1032,9 @@ void Engine::debug_accel_vjp(const float* w, float* out) {
     launch_adj_direct(st_, shape_, body_[bodyPar_], mat_[auxPar_], spec_, cellRec_, cellBody_,
                       adjX_, adjV_, 0.0f, adjW_, adjG_, adjSpread_);
     launch_fft_rows_forward_values(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, adjSpread_,
-                                   spec_, tw_, twLen_);
-    launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_);
-    launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_);
+                                   spec_, tw_);
+    launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_);
+    launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_);
     launch_adj_finish(st_, shape_, body_[bodyPar_], mat_[auxPar_], spec_, adjG_, adjX_, adjV_, 0.0f,
                       1.0f);
     check(hipMemcpyAsync(out, adjX_, n * sizeof(float2), hipMemcpyDeviceToHost, st_), "vjp");
@@ -1113,9 +1113,9 @@ void Engine::adjoint_vjp(int count, const float* dLdx, float* dLdx0, float* dLdv
                               sl.force - h.desc.meshBase, sl.rec - h.desc.physBase,
                               sl.cellBody, adjX_, adjV_, dt, adjW_, adjG_, adjSpread_);
             launch_fft_rows_forward_values(st_, dSims_, dRowsF_, rowsF_, sl.prefix - h.desc.physBase,
-                                           sl.cellBody, adjSpread_, spec_, tw_, twLen_);
-            launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_);
-            launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_, twLen_);
+                                           sl.cellBody, adjSpread_, spec_, tw_);
+            launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_);
+            launch_fft_rows_inverse(st_, dSims_, dRowsI_, rowsI_, spec_, tw_);
             launch_adj_finish(st_, shape_, sl.body - h.base, sl.mats - h.base, spec_, adjG_, adjX_,
                               adjV_, dt, step == 0 ? 0.5f * dt : dt);
         }
@@ -1238,9 +1238,9 @@ void Engine::density_readback(uint32_t* staging, uint32_t* density, float* spect
     launch_cell_scatter(st_, shape_, homeOf_, cellCount_, cellPrefix_, slotBody_, slotCell_);
     launch_cell_canon(st_, gravitySlots_, slotCell_, cellPrefix_, slotBody_, body_[bodyPar_],
                       mat_[auxPar_], cellBody_, cellRec_, cellNear_, big());
-    launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_, twLen_,
+    launch_fft_rows_forward(st_, dSims_, dRowsF_, rowsF_, cellPrefix_, cellBody_, spec_, tw_,
                             tuning_.depositRuns != 0, staging);
-    launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, twLen_, true);
+    launch_fft_columns(st_, dSims_, dCols_, cols_, spec_, green_, tw_, true);
     check(hipGetLastError(), "density");
     check(hipStreamSynchronize(st_), "density");
     if (density)

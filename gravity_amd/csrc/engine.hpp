@@ -170,7 +170,6 @@ private:
     int totalSlots_ = 0;      // body slots (sum of 256-rounded capacities)
     int totalCells_ = 0, totalTiles_ = 0, totalPhys_ = 0, totalMesh_ = 0, totalLiq_ = 0;
     int totalSleepTiles_ = 0;
-    int twLen_ = 2048;
     bool useGraphs_ = true;
 
     // bodies (ping-pong): float4 (x, y, vx, vy) in the reference's units

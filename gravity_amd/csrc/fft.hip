@@ -13,15 +13,18 @@
 //   rows inverse : per needed mesh row, inverse transform along x. Because
 //                  the x and y force kernels are real, one complex inverse
 //                  yields Fx in the real part and Fy in the imaginary part.
-// Transforms are Stockham autosort in LDS: one lane owns one radix-8
-// butterfly of a stage (radix 4 or 2 for the last stage, a few butterflies a
-// lane), so a lane holds 8 complex values and many waves share a CU.
-// Twiddles come from an FP64-rounded table of 2048 entries, staged in LDS by
-// every workgroup. Under walls the mesh doubles;
-// rows that hold only zero padding are neither loaded nor transformed in
-// the forward row pass, and only rows the force gather reads are produced by
-// the inverse passes.
+// Transforms are Stockham autosort in LDS, specialised per line length (a
+// power of two, 16..2048): a lane owns one radix-8 butterfly of a row stage
+// (radix-16 for the columns; radix 4 or 2 for the last stage, a few
+// butterflies a lane), its twiddles sit in registers from host-built
+// per-stage tables, and the stages ping-pong between two LDS buffers. Under
+// walls the mesh doubles; rows that hold only zero padding are neither loaded
+// nor transformed in the forward row pass, and only rows the force gather
+// reads are produced by the inverse passes.
+#include <stdexcept>
+
 #include "common.hpp"
+#include "green.hpp"
 #include "kernels.hpp"
 
 namespace gs {
@@ -34,6 +37,7 @@ constexpr int ROW_LDS = 2048;   // complex values per row workgroup
 constexpr int ROW_NT = 256;     // lanes per row workgroup
 constexpr int COL_LDS = 4096;   // complex values per column workgroup
 constexpr int COL_NT = 512;     // lanes per column workgroup
+constexpr int COL_MAXR = 8;     // largest radix of the column transforms
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -61,24 +65,6 @@ __device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2&
     v2 = csub(a0, a2);
     v1 = cadd(a1, a3);
     v3 = csub(a1, a3);
-}
-
-template <int DIR>
-__device__ __forceinline__ void dft8(float2* v) {
-    const float c = 0.70710678118654752440f;
-    float2 a[4], b[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        a[k] = cadd(v[k], v[k + 4]);
-        b[k] = csub(v[k], v[k + 4]);
-    }
-    b[1] = cmul(b[1], make_float2(c, DIR * c));
-    b[2] = mul_w4<DIR>(b[2]);
-    b[3] = cmul(b[3], make_float2(-c, DIR * c));
-    dft4<DIR>(a[0], a[1], a[2], a[3]);
-    dft4<DIR>(b[0], b[1], b[2], b[3]);
-    v[0] = a[0]; v[2] = a[1]; v[4] = a[2]; v[6] = a[3];
-    v[1] = b[0]; v[3] = b[1]; v[5] = b[2]; v[7] = b[3];
 }
 
 // Radix 16 as 4 x 4: four 4-point DFTs over the inputs 4 apart, the
@@ -119,112 +105,177 @@ __device__ __forceinline__ void dft16(float2* v) {
     }
 }
 
-// One Stockham stage of radix R on nl lines of length L held in LDS;
-// element k of line l lives at buf[l * ls + k * es]. With NT lanes and at
-// most E values in LDS a lane takes Q = E / (R NT) butterflies. jFast:
-// consecutive lanes take consecutive butterflies of one line (rows);
-// otherwise the same butterfly of consecutive lines (columns).
-template <int DIR, int R, int Q, int NT>
-__device__ __forceinline__ void stockham_stage(float2* buf, int L, int nl, int ls, int es, int Ns,
-                                               bool jFast, const float2* __restrict__ tw,
-                                               int twLen) {
-    const int perLine = L / R;
-    const int total = perLine * nl;
-    float2 v[Q][R];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        int bi = (int)threadIdx.x + q * NT;
-        if (bi < total) {
-            int line = jFast ? bi / perLine : bi % nl, j = jFast ? bi % perLine : bi / nl;
-#pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = buf[line * ls + (j + r * perLine) * es];
-        }
-    }
-    __syncthreads();
-    const int twStep = twLen / (Ns * R);
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        int bi = (int)threadIdx.x + q * NT;
-        if (bi < total) {
-            int line = jFast ? bi / perLine : bi % nl, j = jFast ? bi % perLine : bi / nl;
-            int k = j & (Ns - 1);
-            if (Ns > 1) {
-#pragma unroll
-                for (int r = 1; r < R; r++) {
-                    float2 w = tw[r * k * twStep];
-                    if (DIR > 0) w.y = -w.y;
-                    v[q][r] = cmul(v[q][r], w);
-                }
-            }
-            if (R == 16) dft16<DIR>(v[q]);
-            else if (R == 8) dft8<DIR>(v[q]);
-            else if (R == 4) dft4<DIR>(v[q][0], v[q][1], v[q][2], v[q][3]);
-            else dft2<DIR>(v[q]);
-            int base = (j - k) * R + k;
-#pragma unroll
-            for (int r = 0; r < R; r++) buf[line * ls + (base + r * Ns) * es] = v[q][r];
-        }
-    }
-    __syncthreads();
-}
-
-// Transforms nl lines (nl * L <= E). DIR = -1 forward, +1 inverse
-// (unscaled, like S/fft_shared.comp). Stages of radix MAXR (8 or 16) first:
-// the column transforms, whose workgroups hold four lines, take radix 16
-// (one stage fewer per direction); the rows keep radix 8, which gives each of
-// their lanes a butterfly per stage.
-template <int DIR, int NT, int E, int MAXR = 8>
-__device__ __forceinline__ void lds_fft(float2* buf, int L, int nl, int ls, int es, const float2* __restrict__ tw,
-                        int twLen) {
-    const bool jFast = es == 1;
-    int Ns = 1;
-    while (Ns < L) {
-        int rem = L / Ns;
-        if (MAXR >= 16 && rem >= 16) {
-            stockham_stage<DIR, 16, (E / (16 * NT) > 0 ? E / (16 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
-            Ns *= 16;
-        } else if (rem >= 8) {
-            stockham_stage<DIR, 8, (E / (8 * NT) > 0 ? E / (8 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
-            Ns *= 8;
-        } else if (rem == 4) {
-            stockham_stage<DIR, 4, (E / (4 * NT) > 0 ? E / (4 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
-            Ns *= 4;
-        } else {
-            stockham_stage<DIR, 2, (E / (2 * NT) > 0 ? E / (2 * NT) : 1), NT>(buf, L, nl, ls, es, Ns, jFast, tw, twLen);
-            Ns *= 2;
-        }
-    }
-}
-
-// The twiddle table (at most TW_LDS entries) staged in LDS with the first
-// loads of a kernel: the stages then read it without a memory round trip.
-// A lane's TW_LDS / NT loads are issued together (clamped to the table), then
-// stored: one round trip, where a load-store loop waits for every load.
-// A kernel loads its lane's part of the table first and stores it to LDS
-// after issuing its other global loads, so those are in flight together.
+// Length of the FP64-rounded twiddle table the fixed tables are drawn from.
 constexpr int TW_LDS = 2048;
-template <int NT>
-struct Twiddles {
-    static constexpr int PER = TW_LDS / NT;
-    float2 v[PER];
-    __device__ __forceinline__ void load(const float2* __restrict__ tw, int twLen) {
+
+// ---------------------------------------------------------------------------
+// Fixed-length transforms. Mesh lines are powers of two from 16 to 2048, so a
+// kernel switches once on its line length and runs a transform whose every
+// index is a compile-time shift or mask:
+//   - a lane's twiddles for every stage load once, at the kernel's start,
+//     into registers (W_L^m = tw[m * TW_LDS / L] of the FP64-rounded table);
+//     the inverse uses their conjugates, so one set serves both directions;
+//   - the stages ping-pong between two LDS buffers: one barrier a stage;
+//   - LDS positions are XOR-swizzled (bits 3..7 onto bits 0..4), which puts
+//     the strided writes of the first stages (lane strides of 8 and 64
+//     values) on distinct banks and keeps contiguous runs of 32 contiguous;
+//   - complex products use fused multiply-adds (the transforms' results are
+//     compared with a tolerance, never bit for bit).
+__device__ __forceinline__ int fswz(int i) { return i ^ ((i >> 3) & 31); }
+
+template <int DIR>
+__device__ __forceinline__ float2 fmul_tw(float2 a, float2 w) {
+    // a * w forward, a * conj(w) inverse
+    if (DIR < 0) return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.x, w.y, a.y * w.x));
+    return make_float2(fmaf(a.x, w.x, a.y * w.y), fmaf(a.y, w.x, -(a.x * w.y)));
+}
+
+// a * exp(DIR i pi / 4) = (1 + DIR i) a / sqrt(2); a * exp(3 DIR i pi / 4)
+template <int DIR>
+__device__ __forceinline__ float2 rot8_1(float2 a) {
+    const float c = 0.70710678118654752440f;
+    return DIR < 0 ? make_float2((a.x + a.y) * c, (a.y - a.x) * c)
+                   : make_float2((a.x - a.y) * c, (a.y + a.x) * c);
+}
+template <int DIR>
+__device__ __forceinline__ float2 rot8_3(float2 a) {
+    const float c = 0.70710678118654752440f;
+    return DIR < 0 ? make_float2((a.y - a.x) * c, -(a.x + a.y) * c)
+                   : make_float2(-(a.x + a.y) * c, (a.x - a.y) * c);
+}
+
+template <int DIR>
+__device__ __forceinline__ void fdft8(float2* v) {
+    float2 a[4], b[4];
 #pragma unroll
-        for (int q = 0; q < PER; q++) v[q] = tw[min((int)threadIdx.x + q * NT, twLen - 1)];
+    for (int k = 0; k < 4; k++) {
+        a[k] = cadd(v[k], v[k + 4]);
+        b[k] = csub(v[k], v[k + 4]);
     }
-    __device__ __forceinline__ void store(float2* twl, int twLen) const {
+    b[1] = rot8_1<DIR>(b[1]);
+    b[2] = mul_w4<DIR>(b[2]);
+    b[3] = rot8_3<DIR>(b[3]);
+    dft4<DIR>(a[0], a[1], a[2], a[3]);
+    dft4<DIR>(b[0], b[1], b[2], b[3]);
+    v[0] = a[0]; v[2] = a[1]; v[4] = a[2]; v[6] = a[3];
+    v[1] = b[0]; v[3] = b[1]; v[5] = b[2]; v[7] = b[3];
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void fdft(float2* v) {
+    if constexpr (R == 16) dft16<DIR>(v);
+    else if constexpr (R == 8) fdft8<DIR>(v);
+    else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+    else dft2<DIR>(v);
+}
+
+// Radix of the stage that starts at span ns of a line of L values.
+__host__ __device__ constexpr int fixed_radix_of(int L, int maxr, int ns) {
+    return L / ns >= maxr ? maxr : (L / ns >= 8 ? 8 : L / ns);
+}
+template <int L, int MAXR>
+__host__ __device__ constexpr int fixed_radix(int ns) { return fixed_radix_of(L, MAXR, ns); }
+
+// The twiddle tables of the fixed-length transforms, one per line length L
+// (16..2048) and largest radix (8: rows, 16: columns), each holding the
+// twiddles W_L^(r k L / (ns R)) of every stage after the first, laid out
+// [stage][r - 1][k] (k < ns): the lanes of a wave load one r's twiddles from
+// contiguous entries.
+__host__ __device__ constexpr int fixed_table_size(int L, int maxr) {
+    int n = 0;
+    for (int ns = 1; ns < L; ns *= fixed_radix_of(L, maxr, ns))
+        if (ns > 1) n += (fixed_radix_of(L, maxr, ns) - 1) * ns;
+    return n;
+}
+__host__ __device__ constexpr int fixed_table_offset(int L, int maxr) {
+    int off = 0;
+    for (int m = 8; m <= 16; m *= 2)
+        for (int l = 16; l <= 2048; l *= 2) {
+            if (l == L && m == maxr) return off;
+            off += fixed_table_size(l, m);
+        }
+    return off;
+}
+constexpr int FIXED_TABLE_ENTRIES = fixed_table_offset(0, 0);
+
+// A WG's NL lines of L values: element e of line l at l * L + e (JFAST, rows:
+// consecutive lanes take consecutive butterflies of a line) or at e * NL + l
+// (columns: consecutive lanes take the same butterfly of consecutive lines).
+template <int L, int NL, bool JFAST>
+__device__ __forceinline__ int fixed_pos(int line, int e) {
+    return fswz(JFAST ? line * L + e : e * NL + line);
+}
+
+template <int L, int NL, int NT, int MAXR, int NS = 1>
+__host__ __device__ constexpr int fixed_twiddle_count() {
+    if constexpr (NS >= L) {
+        return 0;
+    } else {
+        constexpr int R = fixed_radix<L, MAXR>(NS);
+        constexpr int Q = (NL * (L / R) + NT - 1) / NT;
+        return (NS > 1 ? Q * (R - 1) : 0) + fixed_twiddle_count<L, NL, NT, MAXR, NS * R>();
+    }
+}
+
+// A lane's twiddles of every stage, from the table of (L, MAXR); `table`
+// points at that table's first stage.
+template <int L, int NL, int NT, int MAXR, bool JFAST, int NS = 1>
+__device__ __forceinline__ void fixed_load_twiddles(float2* w, const float2* __restrict__ table) {
+    if constexpr (NS < L) {
+        constexpr int R = fixed_radix<L, MAXR>(NS);
+        constexpr int PER_LINE = L / R;
+        constexpr int Q = (NL * PER_LINE + NT - 1) / NT;
+        if constexpr (NS > 1) {
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int i = (int)threadIdx.x + q * NT;
-            if (i < twLen) twl[i] = v[q];
+            for (int q = 0; q < Q; q++) {
+                const int bi = (int)threadIdx.x + q * NT;
+                const int j = JFAST ? bi % PER_LINE : bi / NL;
+                const int k = j & (NS - 1);
+#pragma unroll
+                for (int r = 1; r < R; r++) w[q * (R - 1) + r - 1] = table[(r - 1) * NS + k];
+            }
+            fixed_load_twiddles<L, NL, NT, MAXR, JFAST, NS * R>(w + Q * (R - 1), table + (R - 1) * NS);
+        } else {
+            fixed_load_twiddles<L, NL, NT, MAXR, JFAST, NS * R>(w, table);
         }
     }
-};
-template <int NT>
-__device__ __forceinline__ void stage_twiddles(float2* twl, const float2* __restrict__ tw,
-                                               int twLen) {
-    Twiddles<NT> t;
-    t.load(tw, twLen);
-    t.store(twl, twLen);
+}
+
+// The stages from span NS on: read src, write dst, barrier, swap. Returns the
+// buffer that holds the transform (natural order). The caller has stored the
+// input into src at fixed_pos and passed a barrier.
+template <int DIR, int L, int NL, int NT, int MAXR, bool JFAST, int NS = 1>
+__device__ __forceinline__ float2* fixed_stages(float2* src, float2* dst, const float2* w) {
+    if constexpr (NS >= L) {
+        return src;
+    } else {
+        constexpr int R = fixed_radix<L, MAXR>(NS);
+        constexpr int PER_LINE = L / R;
+        constexpr int TOTAL = NL * PER_LINE;
+        constexpr int Q = (TOTAL + NT - 1) / NT;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int bi = (int)threadIdx.x + q * NT;
+            if (TOTAL % NT == 0 || bi < TOTAL) {
+                const int line = JFAST ? bi / PER_LINE : bi % NL;
+                const int j = JFAST ? bi % PER_LINE : bi / NL;
+                float2 v[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) v[r] = src[fixed_pos<L, NL, JFAST>(line, j + r * PER_LINE)];
+                if constexpr (NS > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; r++) v[r] = fmul_tw<DIR>(v[r], w[q * (R - 1) + r - 1]);
+                }
+                fdft<R, DIR>(v);
+                const int k = j & (NS - 1);
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) dst[fixed_pos<L, NL, JFAST>(line, base + r * NS)] = v[r];
+            }
+        }
+        __syncthreads();
+        return fixed_stages<DIR, L, NL, NT, MAXR, JFAST, NS * R>(dst, src, w + (NS > 1 ? Q * (R - 1) : 0));
+    }
 }
 
 // Mesh row of forward/inverse work row a: all rows on the torus; under walls
@@ -244,23 +295,53 @@ __device__ __forceinline__ int mesh_row(const SimDesc& s, int a) {
 // atomics instead of four each, on a handful of contended addresses. The
 // corner weights round per body as before and integer sums do not depend on
 // their order, so the deposit is the same either way.
+// The forward row transforms of a work item: FROM_MASS, its deposit `mass`
+// holds (fixed point, line-major) and converts (S/fft_init.comp) into the
+// first buffer; otherwise the first buffer holds the values (a barrier
+// passed). Then transform and store the item's lines.
+template <int L, bool FROM_MASS>
+__device__ __forceinline__ void rows_forward_fixed(const SimDesc& s, const LineWork& w,
+                                                   float2* __restrict__ spec,
+                                                   const float2* __restrict__ ftw,
+                                                   const uint32_t* mass, float2 (*buf)[ROW_LDS]) {
+    constexpr int NL = ROW_LDS / L;
+    constexpr int PER = ROW_LDS / ROW_NT;
+    float2 tws[fixed_twiddle_count<L, NL, ROW_NT, 8>() + 1];
+    fixed_load_twiddles<L, NL, ROW_NT, 8, true>(tws, ftw + fixed_table_offset(L, 8));
+    if (FROM_MASS) {
+        uint32_t m[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) m[q] = mass[(int)threadIdx.x + q * ROW_NT];
+#pragma unroll
+        for (int q = 0; q < PER; q++)
+            buf[0][fswz((int)threadIdx.x + q * ROW_NT)] = make_float2((float)m[q] * s.invMassScale, 0.0f);
+        __syncthreads();
+    }
+    const float2* out = fixed_stages<-1, L, NL, ROW_NT, 8, true>(buf[0], buf[1], tws);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int i = (int)threadIdx.x + q * ROW_NT;
+        const int l = i / L, x = i % L;
+        if (l < w.count) spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = out[fswz(i)];
+    }
+}
+
 template <bool DENSITY, bool RUNS>
 __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __restrict__ sims,
                                                              const LineWork* __restrict__ work,
                                                              const uint32_t* __restrict__ cellPrefix,
                                                              const float4* __restrict__ cellBody,
                                                              float2* __restrict__ spec,
-                                                             const float2* __restrict__ tw, int twLen,
+                                                             const float2* __restrict__ ftw,
                                                              uint32_t* __restrict__ density) {
-    __shared__ float2 data[ROW_LDS];
-    __shared__ uint32_t mass[ROW_LDS];
-    __shared__ float2 twl[TW_LDS];
-    LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
+    // The deposit accumulates in the second transform buffer, which the
+    // first stage overwrites only after every lane has converted its values.
+    __shared__ float2 buf[2][ROW_LDS];
+    uint32_t* mass = reinterpret_cast<uint32_t*>(buf[1]);
+    const LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
-    Twiddles<ROW_NT> tws;
-    tws.load(tw, twLen);  // stored after the deposit's loads
-    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) mass[i] = 0u;
+    for (int i = (int)threadIdx.x; i < ROW_LDS; i += ROW_NT) mass[i] = 0u;
     __syncthreads();
     const float scale = s.massScale;
     const int first = w.first, count = w.count;
@@ -421,7 +502,6 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
         }
     }
     if (RUNS) flush();
-    tws.store(twl, twLen);
     __syncthreads();
     if (DENSITY) {
         for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
@@ -429,130 +509,161 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
             density[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = mass[i];
         }
     }
-    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT)
-        data[i] = make_float2((float)mass[i] * s.invMassScale, 0.0f);
-    __syncthreads();
-    lds_fft<-1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, twl, twLen);
-    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
-        int l = i / L, x = i - l * L;
-        int y = mesh_row(s, w.first + l);
-        spec[s.meshBase + y * s.mw + x] = data[i];
+    switch (L) {
+        case 2048: rows_forward_fixed<2048, true>(s, w, spec, ftw, mass, buf); break;
+        case 1024: rows_forward_fixed<1024, true>(s, w, spec, ftw, mass, buf); break;
+        case 512: rows_forward_fixed<512, true>(s, w, spec, ftw, mass, buf); break;
+        case 256: rows_forward_fixed<256, true>(s, w, spec, ftw, mass, buf); break;
+        case 128: rows_forward_fixed<128, true>(s, w, spec, ftw, mass, buf); break;
+        case 64: rows_forward_fixed<64, true>(s, w, spec, ftw, mass, buf); break;
+        case 32: rows_forward_fixed<32, true>(s, w, spec, ftw, mass, buf); break;
+        default: rows_forward_fixed<16, true>(s, w, spec, ftw, mass, buf); break;
     }
 }
 
-// FORWARD_ONLY: stop after the forward transform and store every row (the
-// density spectrum of the reference's validateFft, K/GravityPipeline.kt:378-419).
-template <bool FORWARD_ONLY>
-__global__ void __launch_bounds__(COL_NT) k_fft_columns(const SimDesc* __restrict__ sims,
-                                                        const LineWork* __restrict__ work,
-                                                        float2* __restrict__ spec,
-                                                        const float2* __restrict__ green,
-                                                        const float2* __restrict__ tw, int twLen) {
-    __shared__ float2 data[COL_LDS];
-    __shared__ float2 twl[TW_LDS];
-    LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
-    const SimDesc& s = sims[w.sim];
-    const int H = s.mh, CB = w.count, c0 = w.first;
-    Twiddles<COL_NT> tws;
-    tws.load(tw, twLen);  // stored after the values' loads
-    const float2* src = spec + s.meshBase;
-    // A lane's COL_LDS / COL_NT values load together (a dead row or a lane
-    // past the item reads its column's row 0), then go to LDS.
+// The column transforms of a work item of CB <= NL = COL_LDS / H columns of
+// length H (interleaved: row y of column c at y * NL + c; a lane past the
+// item's columns repeats its last one). The spectrum values of the item's
+// nodes load during the forward transform. FORWARD_ONLY: stop after the
+// forward transform and store every row (the density spectrum of the
+// reference's validateFft, K/GravityPipeline.kt:378-419).
+template <bool FORWARD_ONLY, int H>
+__device__ __forceinline__ void columns_fixed(const SimDesc& s, const LineWork& w,
+                                              float2* __restrict__ spec,
+                                              const float2* __restrict__ green,
+                                              const float2* __restrict__ ftw,
+                                              float2 (*buf)[COL_LDS]) {
+    constexpr int NL = COL_LDS / H;
     constexpr int PER = COL_LDS / COL_NT;
-    {
-        float2 v[PER];
+    const int CB = w.count, c0 = w.first;
+    float2* const mesh = spec + s.meshBase;
+    // a dead row (zero padding under walls) reads its column's row 0 and
+    // stores zero
+    float2 v[PER];
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int i = min((int)threadIdx.x + q * COL_NT, CB * H - 1);
-            const int y = i / CB, c = i - y * CB;
-            const bool live = !s.bounded || y <= s.ph || y == H - 1;
-            v[q] = src[(live ? y * s.mw : 0) + c0 + c];
-        }
-        tws.store(twl, twLen);
+    for (int q = 0; q < PER; q++) {
+        const int i = (int)threadIdx.x + q * COL_NT;
+        const int y = i / NL, c = min(i % NL, CB - 1);
+        const bool live = !s.bounded || y <= s.ph || y == H - 1;
+        v[q] = mesh[(live ? y * s.mw : 0) + c0 + c];
+    }
+    float2 tws[fixed_twiddle_count<H, NL, COL_NT, COL_MAXR>() + 1];
+    fixed_load_twiddles<H, NL, COL_NT, COL_MAXR, false>(tws, ftw + fixed_table_offset(H, COL_MAXR));
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int i = (int)threadIdx.x + q * COL_NT;
+        const int y = i / NL;
+        const bool live = !s.bounded || y <= s.ph || y == H - 1;
+        buf[0][fswz(i)] = live ? v[q] : make_float2(0.0f, 0.0f);
+    }
+    float2 gv[PER];
+    if (!FORWARD_ONLY) {
+        const float2* G = green + s.meshBase;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int i = (int)threadIdx.x + q * COL_NT;
-            if (i < CB * H) {
-                const int y = i / CB;
-                const bool live = !s.bounded || y <= s.ph || y == H - 1;
-                data[i] = live ? v[q] : make_float2(0.0f, 0.0f);
-            }
-        }
-    }
-    // The spectrum values of this workgroup's nodes load now and arrive
-    // during the forward transform (COL_LDS / COL_NT of them per lane).
-    const float2* G = green + s.meshBase;
-    float2 gv[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        int i = (int)threadIdx.x + q * COL_NT;
-        if (i < CB * H) {
-            int y = i / CB, c = i - y * CB;
-            gv[q] = G[y * s.mw + c0 + c];
+            gv[q] = G[(i / NL) * s.mw + c0 + min(i % NL, CB - 1)];
         }
     }
     __syncthreads();
-    lds_fft<-1, COL_NT, COL_LDS, 16>(data, H, CB, 1, CB, twl, twLen);
+    float2* f = fixed_stages<-1, H, NL, COL_NT, COL_MAXR, false>(buf[0], buf[1], tws);
     if (FORWARD_ONLY) {
-        float2* out = spec + s.meshBase;
-        for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
-            int y = i / CB, c = i - y * CB;
-            out[y * s.mw + c0 + c] = data[i];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int i = (int)threadIdx.x + q * COL_NT;
+            const int y = i / NL, c = i % NL;
+            if (c < CB) mesh[y * s.mw + c0 + c] = f[fswz(i)];
         }
         return;
     }
+    // each lane multiplies the values it will not share before the barrier
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        int i = (int)threadIdx.x + q * COL_NT;
-        if (i < CB * H) {
-            float2 z = cmul(data[i], gv[q]);
-            data[i] = make_float2(z.x * s.invTotalMass, z.y * s.invTotalMass);
-        }
+        const int p = fswz((int)threadIdx.x + q * COL_NT);
+        const float2 z = cmul(f[p], gv[q]);
+        f[p] = make_float2(z.x * s.invTotalMass, z.y * s.invTotalMass);
     }
     __syncthreads();
-    lds_fft<1, COL_NT, COL_LDS, 16>(data, H, CB, 1, CB, twl, twLen);
-    float2* dst = spec + s.meshBase;
-    for (int i = (int)threadIdx.x; i < CB * H; i += COL_NT) {
-        int y = i / CB, c = i - y * CB;
-        bool live = !s.bounded || y <= s.ph || y == H - 1;
-        if (live) dst[y * s.mw + c0 + c] = data[i];
+    const float2* out = fixed_stages<1, H, NL, COL_NT, COL_MAXR, false>(f, f == buf[0] ? buf[1] : buf[0], tws);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int i = (int)threadIdx.x + q * COL_NT;
+        const int y = i / NL, c = i % NL;
+        const bool live = !s.bounded || y <= s.ph || y == H - 1;
+        if (live && c < CB) mesh[y * s.mw + c0 + c] = out[fswz(i)];
+    }
+}
+
+template <bool FORWARD_ONLY>
+// Two workgroups a CU: four waves a SIMD, so at most 128 VGPRs.
+__global__ void __launch_bounds__(COL_NT) __attribute__((amdgpu_waves_per_eu(4)))
+k_fft_columns(const SimDesc* __restrict__ sims,
+                                                        const LineWork* __restrict__ work,
+                                                        float2* __restrict__ spec,
+                                                        const float2* __restrict__ green,
+                                                        const float2* __restrict__ ftw) {
+    __shared__ float2 buf[2][COL_LDS];
+    const LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
+    const SimDesc& s = sims[w.sim];
+    switch (s.mh) {
+        case 2048: columns_fixed<FORWARD_ONLY, 2048>(s, w, spec, green, ftw, buf); break;
+        case 1024: columns_fixed<FORWARD_ONLY, 1024>(s, w, spec, green, ftw, buf); break;
+        case 512: columns_fixed<FORWARD_ONLY, 512>(s, w, spec, green, ftw, buf); break;
+        case 256: columns_fixed<FORWARD_ONLY, 256>(s, w, spec, green, ftw, buf); break;
+        case 128: columns_fixed<FORWARD_ONLY, 128>(s, w, spec, green, ftw, buf); break;
+        case 64: columns_fixed<FORWARD_ONLY, 64>(s, w, spec, green, ftw, buf); break;
+        case 32: columns_fixed<FORWARD_ONLY, 32>(s, w, spec, green, ftw, buf); break;
+        default: columns_fixed<FORWARD_ONLY, 16>(s, w, spec, green, ftw, buf); break;
+    }
+}
+
+// The inverse row transforms of a work item of lines of length L: a lane's
+// ROW_LDS / ROW_NT values and its twiddles load together, the values go to
+// LDS, the stages run, the lines the item holds are stored.
+template <int L>
+__device__ __forceinline__ void rows_inverse_fixed(const SimDesc& s, const LineWork& w,
+                                                   float2* __restrict__ spec,
+                                                   const float2* __restrict__ ftw,
+                                                   float2 (*buf)[ROW_LDS]) {
+    constexpr int NL = ROW_LDS / L;
+    constexpr int PER = ROW_LDS / ROW_NT;
+    float2 tws[fixed_twiddle_count<L, NL, ROW_NT, 8>() + 1];
+    float2 v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int i = (int)threadIdx.x + q * ROW_NT;
+        const int l = min(i / L, w.count - 1), x = i % L;
+        v[q] = spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x];
+    }
+    fixed_load_twiddles<L, NL, ROW_NT, 8, true>(tws, ftw + fixed_table_offset(L, 8));
+#pragma unroll
+    for (int q = 0; q < PER; q++) buf[0][fswz((int)threadIdx.x + q * ROW_NT)] = v[q];
+    __syncthreads();
+    const float2* out = fixed_stages<1, L, NL, ROW_NT, 8, true>(buf[0], buf[1], tws);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int i = (int)threadIdx.x + q * ROW_NT;
+        const int l = i / L, x = i % L;
+        if (l < w.count) spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = out[fswz(i)];
     }
 }
 
 __global__ void __launch_bounds__(ROW_NT) k_fft_rows_inverse(const SimDesc* __restrict__ sims,
                                                              const LineWork* __restrict__ work,
                                                              float2* __restrict__ spec,
-                                                             const float2* __restrict__ tw, int twLen) {
-    __shared__ float2 data[ROW_LDS];
-    __shared__ float2 twl[TW_LDS];
-    LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
+                                                             const float2* __restrict__ ftw) {
+    __shared__ float2 buf[2][ROW_LDS];
+    const LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
-    const int L = s.mw;
-    Twiddles<ROW_NT> tws;
-    tws.load(tw, twLen);  // stored after the values' loads
-    {
-        // A lane's ROW_LDS / ROW_NT values load together, then go to LDS.
-        constexpr int PER = ROW_LDS / ROW_NT;
-        const int n = w.count * L;
-        float2 v[PER];
-#pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int i = min((int)threadIdx.x + q * ROW_NT, n - 1);
-            const int l = i / L, x = i - l * L;
-            v[q] = spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x];
-        }
-        tws.store(twl, twLen);
-#pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int i = (int)threadIdx.x + q * ROW_NT;
-            if (i < n) data[i] = v[q];
-        }
-    }
-    __syncthreads();
-    lds_fft<1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, twl, twLen);
-    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
-        int l = i / L, x = i - l * L;
-        spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = data[i];
+    switch (s.mw) {
+        case 2048: rows_inverse_fixed<2048>(s, w, spec, ftw, buf); break;
+        case 1024: rows_inverse_fixed<1024>(s, w, spec, ftw, buf); break;
+        case 512: rows_inverse_fixed<512>(s, w, spec, ftw, buf); break;
+        case 256: rows_inverse_fixed<256>(s, w, spec, ftw, buf); break;
+        case 128: rows_inverse_fixed<128>(s, w, spec, ftw, buf); break;
+        case 64: rows_inverse_fixed<64>(s, w, spec, ftw, buf); break;
+        case 32: rows_inverse_fixed<32>(s, w, spec, ftw, buf); break;
+        default: rows_inverse_fixed<16>(s, w, spec, ftw, buf); break;
     }
 }
 
@@ -562,15 +673,12 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_inverse(const SimDesc* __re
 __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward_values(
     const SimDesc* __restrict__ sims, const LineWork* __restrict__ work,
     const uint32_t* __restrict__ cellPrefix, const float4* __restrict__ cellBody,
-    const float2* __restrict__ value, float2* __restrict__ spec, const float2* __restrict__ tw,
-    int twLen) {
-    __shared__ float2 data[ROW_LDS];
-    __shared__ float2 twl[TW_LDS];
-    LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
+    const float2* __restrict__ value, float2* __restrict__ spec, const float2* __restrict__ ftw) {
+    __shared__ float2 buf[2][ROW_LDS];
+    const LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
-    stage_twiddles<ROW_NT>(twl, tw, twLen);
-    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) data[i] = make_float2(0.0f, 0.0f);
+    for (int i = (int)threadIdx.x; i < ROW_LDS; i += ROW_NT) buf[0][i] = make_float2(0.0f, 0.0f);
     __syncthreads();
     for (int l = 0; l < w.count; l++) {
         int y = mesh_row(s, w.first + l);
@@ -601,58 +709,81 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward_values(
                     int my = wrap_axis(by + (k >> 1), s.mh);
                     if (my != y) continue;
                     int mx = wrap_axis(bx + (k & 1), s.mw);
-                    atomicAdd(&data[l * L + mx].x, wgt[k] * v.x);
-                    atomicAdd(&data[l * L + mx].y, wgt[k] * v.y);
+                    float2* node = &buf[0][fswz(l * L + mx)];
+                    atomicAdd(&node->x, wgt[k] * v.x);
+                    atomicAdd(&node->y, wgt[k] * v.y);
                 }
             }
         }
     }
     __syncthreads();
-    lds_fft<-1, ROW_NT, ROW_LDS>(data, L, w.count, L, 1, twl, twLen);
-    for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
-        int l = i / L, x = i - l * L;
-        int y = mesh_row(s, w.first + l);
-        spec[s.meshBase + y * s.mw + x] = data[i];
+    switch (L) {
+        case 2048: rows_forward_fixed<2048, false>(s, w, spec, ftw, nullptr, buf); break;
+        case 1024: rows_forward_fixed<1024, false>(s, w, spec, ftw, nullptr, buf); break;
+        case 512: rows_forward_fixed<512, false>(s, w, spec, ftw, nullptr, buf); break;
+        case 256: rows_forward_fixed<256, false>(s, w, spec, ftw, nullptr, buf); break;
+        case 128: rows_forward_fixed<128, false>(s, w, spec, ftw, nullptr, buf); break;
+        case 64: rows_forward_fixed<64, false>(s, w, spec, ftw, nullptr, buf); break;
+        case 32: rows_forward_fixed<32, false>(s, w, spec, ftw, nullptr, buf); break;
+        default: rows_forward_fixed<16, false>(s, w, spec, ftw, nullptr, buf); break;
     }
 }
 
 void launch_fft_rows_forward_values(hipStream_t st, const SimDesc* sims, const LineWork* work,
                                     int items, const uint32_t* cellPrefix, const float4* cellBody,
-                                    const float2* value, float2* spec, const float2* twiddles,
-                                    int twiddleLength) {
+                                    const float2* value, float2* spec, const float2* twiddles) {
     if (items == 0) return;
-    launch(k_fft_rows_forward_values, dim3(items), dim3(ROW_NT), 0, st, sims, work,
-                       cellPrefix, cellBody, value, spec, twiddles, twiddleLength);
+    launch(k_fft_rows_forward_values, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix,
+           cellBody, value, spec, twiddles);
 }
 
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
-                             const float2* twiddles, int twiddleLength, bool runs,
-                             uint32_t* density) {
+                             const float2* twiddles, bool runs, uint32_t* density) {
     if (items == 0) return;
     auto k = density ? (runs ? k_fft_rows_forward<true, true> : k_fft_rows_forward<true, false>)
                      : (runs ? k_fft_rows_forward<false, true> : k_fft_rows_forward<false, false>);
     launch(k, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix, cellBody, spec, twiddles,
-           twiddleLength, density);
+           density);
 }
 
 void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
-                        float2* spec, const float2* green, const float2* twiddles,
-                        int twiddleLength, bool forwardOnly) {
+                        float2* spec, const float2* green, const float2* twiddles, bool forwardOnly) {
     if (items == 0) return;
     if (forwardOnly)
-        launch(k_fft_columns<true>, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green,
-               twiddles, twiddleLength);
+        launch(k_fft_columns<true>, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green, twiddles);
     else
-        launch(k_fft_columns<false>, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green,
-               twiddles, twiddleLength);
+        launch(k_fft_columns<false>, dim3(items), dim3(COL_NT), 0, st, sims, work, spec, green, twiddles);
 }
 
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
-                             float2* spec, const float2* twiddles, int twiddleLength) {
+                             float2* spec, const float2* twiddles) {
     if (items == 0) return;
-    launch(k_fft_rows_inverse, dim3(items), dim3(ROW_NT), 0, st, sims, work, spec,
-                       twiddles, twiddleLength);
+    launch(k_fft_rows_inverse, dim3(items), dim3(ROW_NT), 0, st, sims, work, spec, twiddles);
+}
+
+int fft_twiddle_entries() { return FIXED_TABLE_ENTRIES; }
+
+void fft_twiddle_table(std::vector<float>& out) {
+    // every entry is an entry of the FP64-rounded table of 2048: W_L^m is
+    // exp(-2 pi i m (2048 / L) / 2048)
+    std::vector<float> tw;
+    fft_twiddles(TW_LDS, tw);
+    out.assign((size_t)2 * FIXED_TABLE_ENTRIES, 0.0f);
+    for (int maxr = 8; maxr <= 16; maxr *= 2)
+        for (int L = 16; L <= 2048; L *= 2) {
+            size_t o = (size_t)fixed_table_offset(L, maxr);
+            for (int ns = 1; ns < L; ns *= fixed_radix_of(L, maxr, ns)) {
+                const int R = fixed_radix_of(L, maxr, ns);
+                if (ns == 1) continue;
+                for (int r = 1; r < R; r++)
+                    for (int k = 0; k < ns; k++, o++) {
+                        const size_t m = (size_t)r * k * (L / (ns * R)) * (TW_LDS / L);
+                        out[2 * o] = tw[2 * m];
+                        out[2 * o + 1] = tw[2 * m + 1];
+                    }
+            }
+        }
 }
 
 int fft_row_lines_per_group(int L) { return L >= ROW_LDS ? 1 : ROW_LDS / L; }

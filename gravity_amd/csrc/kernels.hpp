@@ -125,15 +125,18 @@ void launch_cell_canon(hipStream_t st, int slots, const uint32_t* slotCell,
 // runs: sum a lane's bodies of one cell in registers before the LDS atomics.
 void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                              const uint32_t* cellPrefix, const float4* cellBody, float2* spec,
-                             const float2* twiddles, int twiddleLength, bool runs,
-                             uint32_t* density = nullptr);
+                             const float2* twiddles, bool runs, uint32_t* density = nullptr);
 // forwardOnly: leave the 2D density spectrum in spec (every row) instead of
 // multiplying and transforming back.
 void launch_fft_columns(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
                         float2* spec, const float2* green, const float2* twiddles,
-                        int twiddleLength, bool forwardOnly = false);
+                        bool forwardOnly = false);
 void launch_fft_rows_inverse(hipStream_t st, const SimDesc* sims, const LineWork* work, int items,
-                             float2* spec, const float2* twiddles, int twiddleLength);
+                             float2* spec, const float2* twiddles);
+// The transforms' twiddles (`twiddles` above): per line length and radix, the
+// stages' tables drawn from the FP64-rounded exp(-2 pi i m / 2048).
+int fft_twiddle_entries();
+void fft_twiddle_table(std::vector<float>& out);
 // cellRec: per physical cell {first slot, body count, mass in 1/256, 0}.
 void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* acc,
                  const float2* force, const uint4* cellRec, const uint4* cellNear,
@@ -168,8 +171,7 @@ void launch_sleep_wake(hipStream_t st, const SleepBufs& sleep, int sim, int step
 // ---- adjoint of the gravity step ----
 void launch_fft_rows_forward_values(hipStream_t st, const SimDesc* sims, const LineWork* work,
                                     int items, const uint32_t* cellPrefix, const float4* cellBody,
-                                    const float2* value, float2* spec, const float2* twiddles,
-                                    int twiddleLength);
+                                    const float2* value, float2* spec, const float2* twiddles);
 // wCell receives the adjoint weight of the body in every cell slot (scratch
 // of the near-field VJP, one float2 per body slot).
 void launch_adj_direct(hipStream_t st, const LaunchShape& ls, const float4* body,
