@@ -14,10 +14,10 @@
 //                  the x and y force kernels are real, one complex inverse
 //                  yields Fx in the real part and Fy in the imaginary part.
 // Transforms are Stockham autosort in LDS, specialised per line length (a
-// power of two, 16..2048): a lane owns one radix-8 butterfly of a row stage
-// (radix-16 for the columns; radix 4 or 2 for the last stage, a few
-// butterflies a lane), its twiddles sit in registers from host-built
-// per-stage tables, and the stages ping-pong between two LDS buffers. Under
+// power of two, 16..2048): a lane owns one radix-8 butterfly of a stage
+// (radix 4 or 2 for the last stage, a few butterflies a lane), its twiddles
+// sit in registers from host-built per-stage tables, and the stages
+// ping-pong between two LDS buffers. Under
 // walls the mesh doubles; rows that hold only zero padding are neither loaded
 // nor transformed in the forward row pass, and only rows the force gather
 // reads are produced by the inverse passes.
@@ -35,6 +35,7 @@ namespace gs {
 // global loads overlap another's LDS stages.
 constexpr int ROW_LDS = 2048;   // complex values per row workgroup
 constexpr int ROW_NT = 256;     // lanes per row workgroup
+constexpr int ROWF_NT = 512;    // lanes per forward row workgroup (deposit and transform)
 constexpr int COL_LDS = 4096;   // complex values per column workgroup
 constexpr int COL_NT = 512;     // lanes per column workgroup
 constexpr int COL_MAXR = 8;     // largest radix of the column transforms
@@ -299,35 +300,35 @@ __device__ __forceinline__ int mesh_row(const SimDesc& s, int a) {
 // holds (fixed point, line-major) and converts (S/fft_init.comp) into the
 // first buffer; otherwise the first buffer holds the values (a barrier
 // passed). Then transform and store the item's lines.
-template <int L, bool FROM_MASS>
+template <int L, bool FROM_MASS, int NT>
 __device__ __forceinline__ void rows_forward_fixed(const SimDesc& s, const LineWork& w,
                                                    float2* __restrict__ spec,
                                                    const float2* __restrict__ ftw,
                                                    const uint32_t* mass, float2 (*buf)[ROW_LDS]) {
     constexpr int NL = ROW_LDS / L;
-    constexpr int PER = ROW_LDS / ROW_NT;
-    float2 tws[fixed_twiddle_count<L, NL, ROW_NT, 8>() + 1];
-    fixed_load_twiddles<L, NL, ROW_NT, 8, true>(tws, ftw + fixed_table_offset(L, 8));
+    constexpr int PER = ROW_LDS / NT;
+    float2 tws[fixed_twiddle_count<L, NL, NT, 8>() + 1];
+    fixed_load_twiddles<L, NL, NT, 8, true>(tws, ftw + fixed_table_offset(L, 8));
     if (FROM_MASS) {
         uint32_t m[PER];
 #pragma unroll
-        for (int q = 0; q < PER; q++) m[q] = mass[(int)threadIdx.x + q * ROW_NT];
+        for (int q = 0; q < PER; q++) m[q] = mass[(int)threadIdx.x + q * NT];
 #pragma unroll
         for (int q = 0; q < PER; q++)
-            buf[0][fswz((int)threadIdx.x + q * ROW_NT)] = make_float2((float)m[q] * s.invMassScale, 0.0f);
+            buf[0][fswz((int)threadIdx.x + q * NT)] = make_float2((float)m[q] * s.invMassScale, 0.0f);
         __syncthreads();
     }
-    const float2* out = fixed_stages<-1, L, NL, ROW_NT, 8, true>(buf[0], buf[1], tws);
+    const float2* out = fixed_stages<-1, L, NL, NT, 8, true>(buf[0], buf[1], tws);
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int i = (int)threadIdx.x + q * ROW_NT;
+        const int i = (int)threadIdx.x + q * NT;
         const int l = i / L, x = i % L;
         if (l < w.count) spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = out[fswz(i)];
     }
 }
 
 template <bool DENSITY, bool RUNS>
-__global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __restrict__ sims,
+__global__ void __launch_bounds__(ROWF_NT) k_fft_rows_forward(const SimDesc* __restrict__ sims,
                                                              const LineWork* __restrict__ work,
                                                              const uint32_t* __restrict__ cellPrefix,
                                                              const float4* __restrict__ cellBody,
@@ -341,7 +342,7 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     const LineWork w = work[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[w.sim];
     const int L = s.mw;
-    for (int i = (int)threadIdx.x; i < ROW_LDS; i += ROW_NT) mass[i] = 0u;
+    for (int i = (int)threadIdx.x; i < ROW_LDS; i += ROWF_NT) mass[i] = 0u;
     __syncthreads();
     const float scale = s.massScale;
     const int first = w.first, count = w.count;
@@ -427,13 +428,13 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
             // Eight bodies a lane per round, their loads issued together and
             // unconditional (clamped to the range).
             constexpr int UNROLL = 8;
-            for (int t = t0 + (int)threadIdx.x; t < t1; t += UNROLL * ROW_NT) {
+            for (int t = t0 + (int)threadIdx.x; t < t1; t += UNROLL * ROWF_NT) {
                 float4 bds[UNROLL];
 #pragma unroll
-                for (int u = 0; u < UNROLL; u++) bds[u] = cellBody[min(t + u * ROW_NT, t1 - 1)];
+                for (int u = 0; u < UNROLL; u++) bds[u] = cellBody[min(t + u * ROWF_NT, t1 - 1)];
 #pragma unroll
                 for (int u = 0; u < UNROLL; u++) {
-                    if (t + u * ROW_NT >= t1) continue;
+                    if (t + u * ROWF_NT >= t1) continue;
                     float4 bd = bds[u];
                     // S/build_grid.comp:53-69, round() half to even.
                     float gx = bd.x * (float)s.pw - 0.5f, gy = bd.y * (float)s.ph - 0.5f;
@@ -504,20 +505,20 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward(const SimDesc* __re
     if (RUNS) flush();
     __syncthreads();
     if (DENSITY) {
-        for (int i = (int)threadIdx.x; i < w.count * L; i += ROW_NT) {
+        for (int i = (int)threadIdx.x; i < w.count * L; i += ROWF_NT) {
             int l = i / L, x = i - l * L;
             density[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = mass[i];
         }
     }
     switch (L) {
-        case 2048: rows_forward_fixed<2048, true>(s, w, spec, ftw, mass, buf); break;
-        case 1024: rows_forward_fixed<1024, true>(s, w, spec, ftw, mass, buf); break;
-        case 512: rows_forward_fixed<512, true>(s, w, spec, ftw, mass, buf); break;
-        case 256: rows_forward_fixed<256, true>(s, w, spec, ftw, mass, buf); break;
-        case 128: rows_forward_fixed<128, true>(s, w, spec, ftw, mass, buf); break;
-        case 64: rows_forward_fixed<64, true>(s, w, spec, ftw, mass, buf); break;
-        case 32: rows_forward_fixed<32, true>(s, w, spec, ftw, mass, buf); break;
-        default: rows_forward_fixed<16, true>(s, w, spec, ftw, mass, buf); break;
+        case 2048: rows_forward_fixed<2048, true, ROWF_NT>(s, w, spec, ftw, mass, buf); break;
+        case 1024: rows_forward_fixed<1024, true, ROWF_NT>(s, w, spec, ftw, mass, buf); break;
+        case 512: rows_forward_fixed<512, true, ROWF_NT>(s, w, spec, ftw, mass, buf); break;
+        case 256: rows_forward_fixed<256, true, ROWF_NT>(s, w, spec, ftw, mass, buf); break;
+        case 128: rows_forward_fixed<128, true, ROWF_NT>(s, w, spec, ftw, mass, buf); break;
+        case 64: rows_forward_fixed<64, true, ROWF_NT>(s, w, spec, ftw, mass, buf); break;
+        case 32: rows_forward_fixed<32, true, ROWF_NT>(s, w, spec, ftw, mass, buf); break;
+        default: rows_forward_fixed<16, true, ROWF_NT>(s, w, spec, ftw, mass, buf); break;
     }
 }
 
@@ -537,32 +538,32 @@ __device__ __forceinline__ void columns_fixed(const SimDesc& s, const LineWork& 
     constexpr int PER = COL_LDS / COL_NT;
     const int CB = w.count, c0 = w.first;
     float2* const mesh = spec + s.meshBase;
-    // a dead row (zero padding under walls) reads its column's row 0 and
-    // stores zero
+    // a dead row (zero padding under walls) loads nothing and stores zero
     float2 v[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = (int)threadIdx.x + q * COL_NT;
         const int y = i / NL, c = min(i % NL, CB - 1);
         const bool live = !s.bounded || y <= s.ph || y == H - 1;
-        v[q] = mesh[(live ? y * s.mw : 0) + c0 + c];
+        v[q] = live ? mesh[y * s.mw + c0 + c] : make_float2(0.0f, 0.0f);
     }
     float2 tws[fixed_twiddle_count<H, NL, COL_NT, COL_MAXR>() + 1];
     fixed_load_twiddles<H, NL, COL_NT, COL_MAXR, false>(tws, ftw + fixed_table_offset(H, COL_MAXR));
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int i = (int)threadIdx.x + q * COL_NT;
-        const int y = i / NL;
-        const bool live = !s.bounded || y <= s.ph || y == H - 1;
-        buf[0][fswz(i)] = live ? v[q] : make_float2(0.0f, 0.0f);
-    }
+    for (int q = 0; q < PER; q++) buf[0][fswz((int)threadIdx.x + q * COL_NT)] = v[q];
+    // G(kx, ky) from the table's quarter kx <= mw / 2, ky <= H / 2: a column
+    // past mw / 2 takes the conjugate (imaginary part negated), a row past
+    // H / 2 the negated conjugate (real part negated).
     float2 gv[PER];
     if (!FORWARD_ONLY) {
         const float2* G = green + s.meshBase;
+        const int hx = s.mw >> 1, qw = hx + 1;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int i = (int)threadIdx.x + q * COL_NT;
-            gv[q] = G[(i / NL) * s.mw + c0 + min(i % NL, CB - 1)];
+            const int ky = i / NL, kx = c0 + min(i % NL, CB - 1);
+            const int qx = kx > hx ? s.mw - kx : kx, qy = ky > H / 2 ? H - ky : ky;
+            gv[q] = G[qy * qw + qx];
         }
     }
     __syncthreads();
@@ -579,8 +580,13 @@ __device__ __forceinline__ void columns_fixed(const SimDesc& s, const LineWork& 
     // each lane multiplies the values it will not share before the barrier
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int p = fswz((int)threadIdx.x + q * COL_NT);
-        const float2 z = cmul(f[p], gv[q]);
+        const int i = (int)threadIdx.x + q * COL_NT;
+        const int ky = i / NL, kx = c0 + min(i % NL, CB - 1);
+        float2 g = gv[q];
+        if (kx > (s.mw >> 1)) g.y = -g.y;
+        if (ky > H / 2) g.x = -g.x;
+        const int p = fswz(i);
+        const float2 z = cmul(f[p], g);
         f[p] = make_float2(z.x * s.invTotalMass, z.y * s.invTotalMass);
     }
     __syncthreads();
@@ -718,14 +724,14 @@ __global__ void __launch_bounds__(ROW_NT) k_fft_rows_forward_values(
     }
     __syncthreads();
     switch (L) {
-        case 2048: rows_forward_fixed<2048, false>(s, w, spec, ftw, nullptr, buf); break;
-        case 1024: rows_forward_fixed<1024, false>(s, w, spec, ftw, nullptr, buf); break;
-        case 512: rows_forward_fixed<512, false>(s, w, spec, ftw, nullptr, buf); break;
-        case 256: rows_forward_fixed<256, false>(s, w, spec, ftw, nullptr, buf); break;
-        case 128: rows_forward_fixed<128, false>(s, w, spec, ftw, nullptr, buf); break;
-        case 64: rows_forward_fixed<64, false>(s, w, spec, ftw, nullptr, buf); break;
-        case 32: rows_forward_fixed<32, false>(s, w, spec, ftw, nullptr, buf); break;
-        default: rows_forward_fixed<16, false>(s, w, spec, ftw, nullptr, buf); break;
+        case 2048: rows_forward_fixed<2048, false, ROW_NT>(s, w, spec, ftw, nullptr, buf); break;
+        case 1024: rows_forward_fixed<1024, false, ROW_NT>(s, w, spec, ftw, nullptr, buf); break;
+        case 512: rows_forward_fixed<512, false, ROW_NT>(s, w, spec, ftw, nullptr, buf); break;
+        case 256: rows_forward_fixed<256, false, ROW_NT>(s, w, spec, ftw, nullptr, buf); break;
+        case 128: rows_forward_fixed<128, false, ROW_NT>(s, w, spec, ftw, nullptr, buf); break;
+        case 64: rows_forward_fixed<64, false, ROW_NT>(s, w, spec, ftw, nullptr, buf); break;
+        case 32: rows_forward_fixed<32, false, ROW_NT>(s, w, spec, ftw, nullptr, buf); break;
+        default: rows_forward_fixed<16, false, ROW_NT>(s, w, spec, ftw, nullptr, buf); break;
     }
 }
 
@@ -743,7 +749,7 @@ void launch_fft_rows_forward(hipStream_t st, const SimDesc* sims, const LineWork
     if (items == 0) return;
     auto k = density ? (runs ? k_fft_rows_forward<true, true> : k_fft_rows_forward<true, false>)
                      : (runs ? k_fft_rows_forward<false, true> : k_fft_rows_forward<false, false>);
-    launch(k, dim3(items), dim3(ROW_NT), 0, st, sims, work, cellPrefix, cellBody, spec, twiddles,
+    launch(k, dim3(items), dim3(ROWF_NT), 0, st, sims, work, cellPrefix, cellBody, spec, twiddles,
            density);
 }
 

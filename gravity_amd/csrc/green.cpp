@@ -76,21 +76,26 @@ void green_spectrum(const GreenKey& k, std::vector<float>& out) {
     }
     fft2d(kx, mw, mh);
     fft2d(ky, mw, mh);
-    out.resize((size_t)2 * mw * mh);
+    // Kx is odd in x and even in y, Ky the reverse (the sampling above keeps
+    // that exactly, the Nyquist column and row included), so Kx^ = i a and
+    // Ky^ = i b with a odd in kx and even in ky, b the reverse, and
+    // G = s (i a - b): G(-kx, ky) = conj G(kx, ky), G(kx, -ky) = -conj G(kx, ky).
+    // The quarter kx <= mw / 2, ky <= mh / 2 holds the whole table.
+    const int qw = mw / 2 + 1, qh = mh / 2 + 1;
+    out.resize((size_t)2 * qw * qh);
     const double inv = 1.0 / ((double)mw * (double)mh);
-    for (int ry = 0; ry < mh; ry++) {
-        int iky = ry <= mh / 2 ? ry : ry - mh;
-        double sy = sinc(M_PI * iky / mh);
-        for (int rx = 0; rx < mw; rx++) {
-            int ikx = rx <= mw / 2 ? rx : rx - mw;
-            double sx = sinc(M_PI * ikx / mw);
+    for (int ry = 0; ry < qh; ry++) {
+        double sy = sinc(M_PI * ry / mh);
+        for (int rx = 0; rx < qw; rx++) {
+            double sx = sinc(M_PI * rx / mw);
             double window = sx * sx * sy * sy;
             double deconv = 1.0 / std::fmax(window * window, (double)k.windowFloor);
             double s = deconv * inv;
             size_t id = (size_t)ry * mw + rx;
             cd g = s * (kx[id] + cd(0.0, 1.0) * ky[id]);
-            out[2 * id] = (float)g.real();
-            out[2 * id + 1] = (float)g.imag();
+            size_t q = (size_t)ry * qw + rx;
+            out[2 * q] = (float)g.real();
+            out[2 * q + 1] = (float)g.imag();
         }
     }
 }

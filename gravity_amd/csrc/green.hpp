@@ -16,7 +16,9 @@ struct GreenKey {
     }
 };
 
-// Interleaved complex float G(k) for a mw x mh mesh at unit total mass.
+// Interleaved complex float G(k) for a mw x mh mesh at unit total mass: the
+// quarter kx <= mw / 2, ky <= mh / 2, rows of mw / 2 + 1 (the rest follows by
+// symmetry: G(-kx, ky) = conj G(kx, ky), G(kx, -ky) = -conj G(kx, ky)).
 void green_spectrum(const GreenKey& key, std::vector<float>& out);
 // exp(-2 pi i m / length), m < length, interleaved complex float.
 void fft_twiddles(int length, std::vector<float>& out);
