@@ -19,8 +19,8 @@
 // sit in registers from host-built per-stage tables, and the stages
 // ping-pong between two LDS buffers. Under
 // walls the mesh doubles; rows that hold only zero padding are neither loaded
-// nor transformed in the forward row pass, and only rows the force gather
-// reads are produced by the inverse passes.
+// nor transformed in the forward row pass, and only the rows and columns the
+// force gather reads are produced by the inverse passes.
 #include <stdexcept>
 
 #include "common.hpp"
@@ -646,11 +646,14 @@ __device__ __forceinline__ void rows_inverse_fixed(const SimDesc& s, const LineW
     for (int q = 0; q < PER; q++) buf[0][fswz((int)threadIdx.x + q * ROW_NT)] = v[q];
     __syncthreads();
     const float2* out = fixed_stages<1, L, NL, ROW_NT, 8, true>(buf[0], buf[1], tws);
+    // under walls the gather reads columns 0..pw and the last (cells at -1
+    // wrap there); the padding columns are not stored
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = (int)threadIdx.x + q * ROW_NT;
         const int l = i / L, x = i % L;
-        if (l < w.count) spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = out[fswz(i)];
+        const bool read = !s.bounded || x <= s.pw || x == L - 1;
+        if (l < w.count && read) spec[s.meshBase + mesh_row(s, w.first + l) * s.mw + x] = out[fswz(i)];
     }
 }
 

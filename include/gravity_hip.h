@@ -144,7 +144,7 @@ int gs_upload_accel(gs_sim *sim, int first, int count, const float *accel);
 #define GS_MATERIALS 3       /* float[4 * count] */
 #define GS_CONTACT_COUNTS 4  /* uint32[count] */
 #define GS_SORT_DEST 5       /* uint32[count], new index of each body after the last sort */
-#define GS_FORCE_MESH 6      /* float[2 * mesh nodes]: (Fx, Fy) per node after a force build */
+#define GS_FORCE_MESH 6      /* float[2 * mesh nodes]: (Fx, Fy) per node the force gather reads, after a force build (under walls the padding rows and columns hold stale values) */
 #define GS_CONTACTS 7        /* uint32[16 * count]: the 16 smallest contact indices, ascending, body-major */
 int gs_download(gs_sim *sim, int what, int first, int count, void *dst);
 

@@ -48,8 +48,11 @@ def test_initial_acceleration_matches_oracle(oracle, bounded):
     assert err < FORCE_TOLERANCE, err
     mscale = max(np.abs(fx).max(), np.abs(fy).max())
     if bounded:
+        # the nodes the gather reads: rows 0..gh and the last, columns 0..gw
+        # and the last (the padding is not produced)
         rows = list(range(grid[1] + 1)) + [2 * grid[1] - 1]
-        mesh, fx, fy = mesh[rows], fx[rows], fy[rows]
+        cols = list(range(grid[0] + 1)) + [2 * grid[0] - 1]
+        mesh, fx, fy = mesh[rows][:, cols], fx[rows][:, cols], fy[rows][:, cols]
     merr = max(np.abs(mesh[..., 0] - fx).max(), np.abs(mesh[..., 1] - fy).max()) / mscale
     assert merr < FORCE_TOLERANCE, merr
 
