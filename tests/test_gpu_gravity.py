@@ -126,3 +126,35 @@ def test_switching_off_mutual_gravity_leaves_only_the_field():
     assert (free[:, 0] == 0).all()
     assert (free[:, 1] == np.float32(-300.0) / np.float32(H)).all()
     assert (np.abs(attracted[:, 0]) > 1e-3).any()
+
+
+# Every line length of the specialised transforms (16..2048), along rows and
+# along columns: (grid width, grid height) on the torus, where the mesh is the
+# grid.
+LINE_LENGTHS = [(16, 2048), (2048, 16), (32, 1024), (1024, 32), (64, 512), (512, 64), (128, 256),
+                (256, 128)]
+
+
+@pytest.mark.parametrize("grid", LINE_LENGTHS, ids=[f"{w}x{h}" for w, h in LINE_LENGTHS])
+def test_every_line_length_transforms_like_fp64_and_the_oracle(oracle, grid):
+    W, H, n = 1440, 720, 6000
+    bodies, mats = clustered(n, W, H, 5)
+    cfg = GravityConfig(particles=ParticleConfig(capacity=n, initialBodies=n, seed=1),
+                        gravity=GravityFieldConfig(gridWidth=grid[0], gridHeight=grid[1], minimumGrid=16))
+    print(f"[gravity] grid {grid[0]}x{grid[1]}", flush=True)
+    with GravityPipeline(cfg, W, H, scenarioState=bodies) as pipe:
+        pipe.upload(bodies, mats)
+        dens, spec, _ = pipe.densityMesh(n)
+        pipe.initializeAcceleration(n)
+        mesh = pipe.forceMesh()
+    # the forward transforms against numpy's FP64 transform of the same deposit
+    ref = np.fft.fft2(dens.astype(np.float64) / oracle.mass_scale(float(n)))
+    rel = np.abs(spec - ref).max() / np.abs(ref).max()
+    assert rel < 1.5e-7, rel
+    # the whole force build (forward, Green multiply, inverse) against the oracle's
+    sim = oracle.Sim(oracle.params(W, H, grid=grid), bodies, mats)
+    sim.initialize_acceleration()
+    fx, fy = sim.forces()
+    scale = max(np.abs(fx).max(), np.abs(fy).max())
+    merr = max(np.abs(mesh[..., 0] - fx).max(), np.abs(mesh[..., 1] - fy).max()) / scale
+    assert merr < FORCE_TOLERANCE, merr
