@@ -31,14 +31,13 @@ namespace gs {
 
 // A mesh line holds at most 2048 values. Row workgroups take one 2048-value
 // row (or several shorter ones) with a radix-8 butterfly per lane and stage;
-// column workgroups take 4096 values: 2 to 4 workgroups share a CU, so one's
+// column workgroups take 4096 values: 2 workgroups share a CU, so one's
 // global loads overlap another's LDS stages.
 constexpr int ROW_LDS = 2048;   // complex values per row workgroup
 constexpr int ROW_NT = 256;     // lanes per row workgroup
 constexpr int ROWF_NT = 512;    // lanes per forward row workgroup (deposit and transform)
 constexpr int COL_LDS = 4096;   // complex values per column workgroup
 constexpr int COL_NT = 512;     // lanes per column workgroup
-constexpr int COL_MAXR = 8;     // largest radix of the column transforms
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -66,44 +65,6 @@ __device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2&
     v2 = csub(a0, a2);
     v1 = cadd(a1, a3);
     v3 = csub(a1, a3);
-}
-
-// Radix 16 as 4 x 4: four 4-point DFTs over the inputs 4 apart, the
-// twiddles W16^(n2 k1), four 4-point DFTs across them; outputs in order.
-template <int DIR>
-__device__ __forceinline__ void dft16(float2* v) {
-    const float c1 = 0.92387953251128675613f;  // cos(pi / 8)
-    const float s1 = 0.38268343236508977173f;  // sin(pi / 8)
-    const float c2 = 0.70710678118654752440f;  // cos(pi / 4)
-    float2 y[16];
-#pragma unroll
-    for (int n2 = 0; n2 < 4; n2++) {
-        float2 a0 = v[n2], a1 = v[4 + n2], a2 = v[8 + n2], a3 = v[12 + n2];
-        dft4<DIR>(a0, a1, a2, a3);
-        y[4 * n2 + 0] = a0;
-        y[4 * n2 + 1] = a1;
-        y[4 * n2 + 2] = a2;
-        y[4 * n2 + 3] = a3;
-    }
-    // W16^m = (cos(2 pi m / 16), DIR sin(2 pi m / 16)) for m = n2 k1
-    y[5] = cmul(y[5], make_float2(c1, DIR * s1));     // m = 1
-    y[6] = cmul(y[6], make_float2(c2, DIR * c2));     // m = 2
-    y[7] = cmul(y[7], make_float2(s1, DIR * c1));     // m = 3
-    y[9] = cmul(y[9], make_float2(c2, DIR * c2));     // m = 2
-    y[10] = mul_w4<DIR>(y[10]);                       // m = 4
-    y[11] = cmul(y[11], make_float2(-c2, DIR * c2));  // m = 6
-    y[13] = cmul(y[13], make_float2(s1, DIR * c1));   // m = 3
-    y[14] = cmul(y[14], make_float2(-c2, DIR * c2));  // m = 6
-    y[15] = cmul(y[15], make_float2(-c1, -DIR * s1)); // m = 9
-#pragma unroll
-    for (int k1 = 0; k1 < 4; k1++) {
-        float2 b0 = y[k1], b1 = y[4 + k1], b2 = y[8 + k1], b3 = y[12 + k1];
-        dft4<DIR>(b0, b1, b2, b3);
-        v[k1] = b0;
-        v[k1 + 4] = b1;
-        v[k1 + 8] = b2;
-        v[k1 + 12] = b3;
-    }
 }
 
 // Length of the FP64-rounded twiddle table the fixed tables are drawn from.
@@ -164,8 +125,7 @@ __device__ __forceinline__ void fdft8(float2* v) {
 
 template <int R, int DIR>
 __device__ __forceinline__ void fdft(float2* v) {
-    if constexpr (R == 16) dft16<DIR>(v);
-    else if constexpr (R == 8) fdft8<DIR>(v);
+    if constexpr (R == 8) fdft8<DIR>(v);
     else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
     else dft2<DIR>(v);
 }
@@ -178,10 +138,9 @@ template <int L, int MAXR>
 __host__ __device__ constexpr int fixed_radix(int ns) { return fixed_radix_of(L, MAXR, ns); }
 
 // The twiddle tables of the fixed-length transforms, one per line length L
-// (16..2048) and largest radix (8: rows, 16: columns), each holding the
-// twiddles W_L^(r k L / (ns R)) of every stage after the first, laid out
-// [stage][r - 1][k] (k < ns): the lanes of a wave load one r's twiddles from
-// contiguous entries.
+// (16..2048, largest radix 8), each holding the twiddles W_L^(r k L / (ns R))
+// of every stage after the first, laid out [stage][r - 1][k] (k < ns): the
+// lanes of a wave load one r's twiddles from contiguous entries.
 __host__ __device__ constexpr int fixed_table_size(int L, int maxr) {
     int n = 0;
     for (int ns = 1; ns < L; ns *= fixed_radix_of(L, maxr, ns))
@@ -190,14 +149,13 @@ __host__ __device__ constexpr int fixed_table_size(int L, int maxr) {
 }
 __host__ __device__ constexpr int fixed_table_offset(int L, int maxr) {
     int off = 0;
-    for (int m = 8; m <= 16; m *= 2)
-        for (int l = 16; l <= 2048; l *= 2) {
-            if (l == L && m == maxr) return off;
-            off += fixed_table_size(l, m);
-        }
+    for (int l = 16; l <= 2048; l *= 2) {
+        if (l == L && maxr == 8) return off;
+        off += fixed_table_size(l, 8);
+    }
     return off;
 }
-constexpr int FIXED_TABLE_ENTRIES = fixed_table_offset(0, 0);
+constexpr int FIXED_TABLE_ENTRIES = fixed_table_offset(0, 8);
 
 // A WG's NL lines of L values: element e of line l at l * L + e (JFAST, rows:
 // consecutive lanes take consecutive butterflies of a line) or at e * NL + l
@@ -286,16 +244,6 @@ __device__ __forceinline__ int mesh_row(const SimDesc& s, int a) {
     return a <= s.ph ? a : s.mh - 1;
 }
 
-// ---------------------------------------------------------------------------
-// DENSITY: also store the fixed-point deposit of the item's rows to density
-// (the readback of the density mesh, S/build_grid.comp's buffer).
-// RUNS: a lane sums the fixed-point corner weights of its consecutive bodies
-// in registers while they share a 3x3 window of mesh nodes (the nodes a
-// physical cell's bodies reach) and adds each non-zero sum to LDS when a body
-// leaves the window: in a crowded cell hundreds of bodies then cost a few LDS
-// atomics instead of four each, on a handful of contended addresses. The
-// corner weights round per body as before and integer sums do not depend on
-// their order, so the deposit is the same either way.
 // The forward row transforms of a work item: FROM_MASS, its deposit `mass`
 // holds (fixed point, line-major) and converts (S/fft_init.comp) into the
 // first buffer; otherwise the first buffer holds the values (a barrier
@@ -327,6 +275,16 @@ __device__ __forceinline__ void rows_forward_fixed(const SimDesc& s, const LineW
     }
 }
 
+// ---------------------------------------------------------------------------
+// DENSITY: also store the fixed-point deposit of the item's rows to density
+// (the readback of the density mesh, S/build_grid.comp's buffer).
+// RUNS: a lane sums the fixed-point corner weights of its consecutive bodies
+// in registers while they share a 3x3 window of mesh nodes (the nodes a
+// physical cell's bodies reach) and adds each non-zero sum to LDS when a body
+// leaves the window: in a crowded cell hundreds of bodies then cost a few LDS
+// atomics instead of four each, on a handful of contended addresses. The
+// corner weights round per body as before and integer sums do not depend on
+// their order, so the deposit is the same either way.
 template <bool DENSITY, bool RUNS>
 __global__ void __launch_bounds__(ROWF_NT) k_fft_rows_forward(const SimDesc* __restrict__ sims,
                                                              const LineWork* __restrict__ work,
@@ -547,8 +505,8 @@ __device__ __forceinline__ void columns_fixed(const SimDesc& s, const LineWork& 
         const bool live = !s.bounded || y <= s.ph || y == H - 1;
         v[q] = live ? mesh[y * s.mw + c0 + c] : make_float2(0.0f, 0.0f);
     }
-    float2 tws[fixed_twiddle_count<H, NL, COL_NT, COL_MAXR>() + 1];
-    fixed_load_twiddles<H, NL, COL_NT, COL_MAXR, false>(tws, ftw + fixed_table_offset(H, COL_MAXR));
+    float2 tws[fixed_twiddle_count<H, NL, COL_NT, 8>() + 1];
+    fixed_load_twiddles<H, NL, COL_NT, 8, false>(tws, ftw + fixed_table_offset(H, 8));
 #pragma unroll
     for (int q = 0; q < PER; q++) buf[0][fswz((int)threadIdx.x + q * COL_NT)] = v[q];
     // G(kx, ky) from the table's quarter kx <= mw / 2, ky <= H / 2: a column
@@ -567,7 +525,7 @@ __device__ __forceinline__ void columns_fixed(const SimDesc& s, const LineWork& 
         }
     }
     __syncthreads();
-    float2* f = fixed_stages<-1, H, NL, COL_NT, COL_MAXR, false>(buf[0], buf[1], tws);
+    float2* f = fixed_stages<-1, H, NL, COL_NT, 8, false>(buf[0], buf[1], tws);
     if (FORWARD_ONLY) {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -590,7 +548,7 @@ __device__ __forceinline__ void columns_fixed(const SimDesc& s, const LineWork& 
         f[p] = make_float2(z.x * s.invTotalMass, z.y * s.invTotalMass);
     }
     __syncthreads();
-    const float2* out = fixed_stages<1, H, NL, COL_NT, COL_MAXR, false>(f, f == buf[0] ? buf[1] : buf[0], tws);
+    const float2* out = fixed_stages<1, H, NL, COL_NT, 8, false>(f, f == buf[0] ? buf[1] : buf[0], tws);
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = (int)threadIdx.x + q * COL_NT;
@@ -779,20 +737,19 @@ void fft_twiddle_table(std::vector<float>& out) {
     std::vector<float> tw;
     fft_twiddles(TW_LDS, tw);
     out.assign((size_t)2 * FIXED_TABLE_ENTRIES, 0.0f);
-    for (int maxr = 8; maxr <= 16; maxr *= 2)
-        for (int L = 16; L <= 2048; L *= 2) {
-            size_t o = (size_t)fixed_table_offset(L, maxr);
-            for (int ns = 1; ns < L; ns *= fixed_radix_of(L, maxr, ns)) {
-                const int R = fixed_radix_of(L, maxr, ns);
-                if (ns == 1) continue;
-                for (int r = 1; r < R; r++)
-                    for (int k = 0; k < ns; k++, o++) {
-                        const size_t m = (size_t)r * k * (L / (ns * R)) * (TW_LDS / L);
-                        out[2 * o] = tw[2 * m];
-                        out[2 * o + 1] = tw[2 * m + 1];
-                    }
-            }
+    for (int L = 16; L <= 2048; L *= 2) {
+        size_t o = (size_t)fixed_table_offset(L, 8);
+        for (int ns = 1; ns < L; ns *= fixed_radix_of(L, 8, ns)) {
+            const int R = fixed_radix_of(L, 8, ns);
+            if (ns == 1) continue;
+            for (int r = 1; r < R; r++)
+                for (int k = 0; k < ns; k++, o++) {
+                    const size_t m = (size_t)r * k * (L / (ns * R)) * (TW_LDS / L);
+                    out[2 * o] = tw[2 * m];
+                    out[2 * o + 1] = tw[2 * m + 1];
+                }
         }
+    }
 }
 
 int fft_row_lines_per_group(int L) { return L >= ROW_LDS ? 1 : ROW_LDS / L; }
