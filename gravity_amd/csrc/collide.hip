@@ -232,12 +232,18 @@ __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__
                                                      GridBuf grid, SleepBufs sleep,
                                                      const uint32_t* __restrict__ adapt) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+    // The body and its acceleration load before the simulation's constants
+    // are known (both arrays cover the padded slots; a slot past the active
+    // bodies returns without using them); the fence keeps the compiler from
+    // sinking the loads behind that wait.
+    const int g = b.first + (int)threadIdx.x;
+    float4 v = body[g];
+    const float2 a = acc[g];
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const SimDesc& s = sims[b.sim];
-    int g = b.first + (int)threadIdx.x;
     if (g - s.base >= s.n) return;
     if (adapt != nullptr && s.adaptive && driftDt != 0.0f)
         driftDt = s.dt / (float)max(adapt[ADAPT_WORDS * b.sim + 2], 1u);
-    float4 v = body[g];
     bool awake = true;
     if (s.sleepOn) {
         const SleepState st = sleep.state[b.sim];
@@ -258,7 +264,6 @@ __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__
     }
     if (awake && (kickDt != 0.0f || driftDt != 0.0f)) {
         if (kickDt != 0.0f) {
-            float2 a = acc[g];
             float h = 0.5f * kickDt;
             v.z += a.x * h;
             v.w += a.y * h;
@@ -404,18 +409,23 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
     const OrderedBlock ob = order[blockIdx.x];
     const int bi = ob.index;
     const BodyBlock b{ob.sim, ob.first};
+    // The body's cell in the previous pass's grid and its own node here load
+    // together, before the simulation's constants are known: both arrays
+    // cover the padded slots of every simulation, and a slot past the active
+    // bodies does not use them. The clearing store waits only for the first.
+    const int g = b.first + (int)threadIdx.x;
+    const int prevCell = prev.cellOf[g];
+    const float4 own = grid.nodes[g];
+    // A wavefront-scope fence emits no instruction; it keeps the compiler
+    // from sinking the two loads into the branch below, behind the wait for
+    // the simulation's constants.
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const SimDesc& s = sims[b.sim];
-    int g = b.first + (int)threadIdx.x;
     uint32_t n = 0;
     bool active = true;
     if (ADAPTIVE && s.adaptive && pass >= 0)
         active = (uint32_t)pass < adapt[ADAPT_WORDS * b.sim + 2];
     if (g - s.base < s.n && active) {
-        // The body's cell in the previous pass's grid and its own node here
-        // load together (cellOf is always allocated, so the first load is
-        // unconditional); the clearing store waits only for the first.
-        const int prevCell = prev.cellOf[g];
-        const float4 own = grid.nodes[g];
         if (clearPrev) prev.heads[prevCell] = 0u;
         // A sleeping body keeps its last count (S/collision_cache_count.comp:147-160).
         bool asleep = false;

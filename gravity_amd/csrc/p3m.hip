@@ -343,6 +343,60 @@ __device__ __forceinline__ void near_field_rows(const SimDesc& s, uint4 hrec, co
     }
 }
 
+// The same rows read from global memory, for a block whose box does not fit
+// the stage (the sparse outskirts of a disc, where about a third of a galaxy's
+// blocks lie). A cell-by-cell read would wait for every record in turn, two
+// round trips a cell; here the records of a part of a row (both halves of
+// each) load together, unconditionally (a wrapped cell index is always a record of
+// the simulation; a record of a cell outside the walls, or a half a cell does
+// not use, is read and left alone), and the cells are then added in the same
+// order with the same terms as near_field_rows.
+template <int R>
+__device__ __forceinline__ void near_field_rows_global(const SimDesc& s, uint4 hrec,
+                                                       const uint4* __restrict__ cellNear,
+                                                       const float4* __restrict__ cellBody,
+                                                       int g, float x, float y, float& ax,
+                                                       float& ay) {
+    const float k2 = s.invTwoSigma2;
+    int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
+    constexpr int D = 2 * R + 1;
+    // Three cells a part: with both halves of three records in flight the
+    // kernel stays within the registers of seven waves a SIMD, the most its
+    // LDS stage allows.
+    constexpr int PART = 3;
+    near_home(s, hrec, cellBody, g, hx, hy, x, y, k2, ax, ay);
+    float cpx[D], cex[D];
+    const uint32_t colOk = near_columns<R>(s, hx, x, k2, cpx, cex);
+#pragma unroll 1
+    for (int j = 0; j < D; j++) {
+        int wy;
+        float py, py2, ey;
+        const int cy = hy + j - R;
+        if (!near_row(s, cy, y, k2, wy, py, py2, ey)) continue;
+        const uint32_t rowOk = j == R ? colOk & ~(1u << R) : colOk;
+        const uint4* row = cellNear + 2 * (size_t)(s.physBase + wy * s.pw);
+#pragma unroll
+        for (int k0 = 0; k0 < D; k0 += PART) {
+            uint4 lo[PART], hi[PART];
+#pragma unroll
+            for (int q = 0; q < PART; q++) {
+                const uint4* rec = row + 2 * wrap_axis(hx + min(k0 + q, D - 1) - R, s.pw);
+                lo[q] = rec[0];
+                hi[q] = rec[1];
+            }
+#pragma unroll
+            for (int q = 0; q < PART; q++) {
+                const int k = k0 + q;
+                if (k < D) {
+                    const uint32_t count = (rowOk >> k) & 1u ? lo[q].x : 0u;
+                    near_cell(s, count, lo[q], hi[q], cpx[k], cex[k], py, py2, ey, x, y, k2, ax,
+                              ay);
+                }
+            }
+        }
+    }
+}
+
 // Near field of any radius, a cell at a time.
 __device__ __forceinline__ void near_field_any(const SimDesc& s, const uint4* __restrict__ cellRec,
                                                const float4* __restrict__ cellBody, int g,
@@ -487,7 +541,7 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
         if (staged)
             near_field_rows<3>(s, hrec, StagedRows{stage, x0, y0, bw}, cellBody, g, p.x, p.y, ax, ay);
         else
-            near_field_rows<3>(s, hrec, GlobalRows{cellNear}, cellBody, g, p.x, p.y, ax, ay);
+            near_field_rows_global<3>(s, hrec, cellNear, cellBody, g, p.x, p.y, ax, ay);
     } else {
         float2 mf = mesh_force(s, force, p.x, p.y);
         ax = mf.x + s.fieldX;
