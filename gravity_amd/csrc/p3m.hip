@@ -144,14 +144,28 @@ __global__ void __launch_bounds__(BIG_NT) k_cell_canon_big(const uint32_t* __res
 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 
-__device__ __forceinline__ float min_image(const SimDesc& s, float d) {
+// The near field's constants, read once per body into registers: the
+// per-cell helpers below take either this or the SimDesc itself (the same
+// member names), and a loop over the stencil then reloads nothing.
+struct NearK {
+    int bounded, pw, ph, exactLimit, homeLimit;
+    float aspect, invAspect, gOverM, minDist2;
+};
+__device__ __forceinline__ NearK near_k(const SimDesc& s) {
+    return NearK{s.bounded, s.pw, s.ph, s.exactLimit, s.homeLimit,
+                 s.aspect, s.invAspect, s.gOverM, s.minDist2};
+}
+
+template <typename K>
+__device__ __forceinline__ float min_image(const K& s, float d) {
     return s.bounded ? d : d - floorf(d + 0.5f);
 }
 
 // Short-range term of one source (S/kick.comp:66-80) from its physical
 // displacement (px = dx * aspect, py = dy), its squared length r2, its
 // Gaussian weight and its mass: G m w / (M max(r2, minDist2)) * (dx, dy).
-__device__ __forceinline__ void near_term(const SimDesc& s, float px, float py, float r2,
+template <typename K>
+__device__ __forceinline__ void near_term(const K& s, float px, float py, float r2,
                                           float weight, float mass, float& ax, float& ay) {
 #pragma clang fp contract(fast)
     float q = s.gOverM * mass * weight * __builtin_amdgcn_rcpf(fmaxf(r2, s.minDist2));
@@ -202,7 +216,8 @@ __device__ __forceinline__ float2 mesh_force(const SimDesc& s, const float2* __r
 // factors; the mesh gather, the field and the half kick round as written.
 
 // One source taken body by body (x, y its normalised position, z its mass).
-__device__ __forceinline__ void body_term(const SimDesc& s, float4 o, float x, float y, float k2,
+template <typename K>
+__device__ __forceinline__ void body_term(const K& s, float4 o, float x, float y, float k2,
                                           float& ax, float& ay) {
 #pragma clang fp contract(fast)
     float ox = min_image(s, o.x - x) * s.aspect;
@@ -217,12 +232,13 @@ __device__ __forceinline__ void body_term(const SimDesc& s, float4 o, float x, f
 // together; then each stencil row's 32-byte near records, which hold the
 // count, the mass and, for a cell taken body by body, its bodies, from the
 // workgroup's LDS stage of the records around its bodies or, for a block
-// whose box does not fit the stage, from global memory (near_field_rows);
-// both add the same terms in the same order.
+// whose box does not fit the stage, from global memory (near_field_rows,
+// the same code and the same order for both).
 
 // The home cell: its first homeLimit list entries, the body itself skipped
 // (under walls, none for a body on or beyond the far edge).
-__device__ __forceinline__ void near_home(const SimDesc& s, uint4 hrec,
+template <typename K>
+__device__ __forceinline__ void near_home(const K& s, uint4 hrec,
                                           const float4* __restrict__ cellBody, int g, int hx,
                                           int hy, float x, float y, float k2, float& ax,
                                           float& ay) {
@@ -266,7 +282,8 @@ __device__ __forceinline__ uint32_t near_columns(const SimDesc& s, int hx, float
 
 // One stencil cell of a row from its near record (lo, hi): a monopole at the
 // cell centre, or its one or two bodies.
-__device__ __forceinline__ void near_cell(const SimDesc& s, uint32_t count, uint4 lo, uint4 hi,
+template <typename K>
+__device__ __forceinline__ void near_cell(const K& s, uint32_t count, uint4 lo, uint4 hi,
                                           float px, float ex, float py, float py2, float ey,
                                           float x, float y, float k2, float& ax, float& ay) {
 #pragma clang fp contract(fast)
@@ -297,10 +314,17 @@ __device__ __forceinline__ bool near_row(const SimDesc& s, int cy, float y, floa
     return true;
 }
 
-// The same rows from `Rows` (a record source: rec(cy, cx) gives the near
-// record of the cell at unwrapped coordinates (cx, cy)), each cell's record
-// read as the row reaches it: the workgroup's LDS stage (StagedRows) or
-// global memory (GlobalRows).
+// The rows from `Rows`, a record source: rec(cy, cx) gives the near record
+// of the cell at unwrapped coordinates (cx, cy), from the workgroup's LDS
+// stage (StagedRows) or from global memory (GlobalRows, for a block whose box
+// does not fit the stage: the sparse outskirts of a disc, where about a third
+// of a galaxy's blocks lie). Reading a row cell by cell would wait for every
+// record in turn; here the records of a part of a row (both halves of each)
+// load together, unconditionally (every address is a record of the stage or
+// of the simulation; a record of a cell outside the walls, or a half a cell
+// does not use, is read and left alone), and the cells are then added one
+// by one in row order. Three cells a part keep the kernel within the
+// registers of seven waves a SIMD, the most its LDS stage allows.
 struct StagedRows {
     const uint4* stage;
     int x0, y0, bw;
@@ -320,51 +344,11 @@ __device__ __forceinline__ void near_field_rows(const SimDesc& s, uint4 hrec, co
                                                 const float4* __restrict__ cellBody, int g,
                                                 float x, float y, float& ax, float& ay) {
     const float k2 = s.invTwoSigma2;
+    const NearK c = near_k(s);
     int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
     constexpr int D = 2 * R + 1;
-    near_home(s, hrec, cellBody, g, hx, hy, x, y, k2, ax, ay);
-    float cpx[D], cex[D];
-    const uint32_t colOk = near_columns<R>(s, hx, x, k2, cpx, cex);
-#pragma unroll 1
-    for (int j = 0; j < D; j++) {
-        int wy;
-        float py, py2, ey;
-        const int cy = hy + j - R;
-        if (!near_row(s, cy, y, k2, wy, py, py2, ey)) continue;
-        const uint32_t rowOk = j == R ? colOk & ~(1u << R) : colOk;
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            const uint4* rec = rows.rec(s, cy, hx + k - R);
-            const uint4 lo = rec[0];
-            const uint32_t count = (rowOk >> k) & 1u ? lo.x : 0u;
-            const uint4 hi = count - 1u < (uint32_t)NEAR_INLINE ? rec[1] : make_uint4(0u, 0u, 0u, 0u);
-            near_cell(s, count, lo, hi, cpx[k], cex[k], py, py2, ey, x, y, k2, ax, ay);
-        }
-    }
-}
-
-// The same rows read from global memory, for a block whose box does not fit
-// the stage (the sparse outskirts of a disc, where about a third of a galaxy's
-// blocks lie). A cell-by-cell read would wait for every record in turn, two
-// round trips a cell; here the records of a part of a row (both halves of
-// each) load together, unconditionally (a wrapped cell index is always a record of
-// the simulation; a record of a cell outside the walls, or a half a cell does
-// not use, is read and left alone), and the cells are then added in the same
-// order with the same terms as near_field_rows.
-template <int R>
-__device__ __forceinline__ void near_field_rows_global(const SimDesc& s, uint4 hrec,
-                                                       const uint4* __restrict__ cellNear,
-                                                       const float4* __restrict__ cellBody,
-                                                       int g, float x, float y, float& ax,
-                                                       float& ay) {
-    const float k2 = s.invTwoSigma2;
-    int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
-    constexpr int D = 2 * R + 1;
-    // Three cells a part: with both halves of three records in flight the
-    // kernel stays within the registers of seven waves a SIMD, the most its
-    // LDS stage allows.
     constexpr int PART = 3;
-    near_home(s, hrec, cellBody, g, hx, hy, x, y, k2, ax, ay);
+    near_home(c, hrec, cellBody, g, hx, hy, x, y, k2, ax, ay);
     float cpx[D], cex[D];
     const uint32_t colOk = near_columns<R>(s, hx, x, k2, cpx, cex);
 #pragma unroll 1
@@ -374,13 +358,12 @@ __device__ __forceinline__ void near_field_rows_global(const SimDesc& s, uint4 h
         const int cy = hy + j - R;
         if (!near_row(s, cy, y, k2, wy, py, py2, ey)) continue;
         const uint32_t rowOk = j == R ? colOk & ~(1u << R) : colOk;
-        const uint4* row = cellNear + 2 * (size_t)(s.physBase + wy * s.pw);
 #pragma unroll
         for (int k0 = 0; k0 < D; k0 += PART) {
             uint4 lo[PART], hi[PART];
 #pragma unroll
             for (int q = 0; q < PART; q++) {
-                const uint4* rec = row + 2 * wrap_axis(hx + min(k0 + q, D - 1) - R, s.pw);
+                const uint4* rec = rows.rec(s, cy, hx + min(k0 + q, D - 1) - R);
                 lo[q] = rec[0];
                 hi[q] = rec[1];
             }
@@ -389,7 +372,7 @@ __device__ __forceinline__ void near_field_rows_global(const SimDesc& s, uint4 h
                 const int k = k0 + q;
                 if (k < D) {
                     const uint32_t count = (rowOk >> k) & 1u ? lo[q].x : 0u;
-                    near_cell(s, count, lo[q], hi[q], cpx[k], cex[k], py, py2, ey, x, y, k2, ax,
+                    near_cell(c, count, lo[q], hi[q], cpx[k], cex[k], py, py2, ey, x, y, k2, ax,
                               ay);
                 }
             }
@@ -541,7 +524,7 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
         if (staged)
             near_field_rows<3>(s, hrec, StagedRows{stage, x0, y0, bw}, cellBody, g, p.x, p.y, ax, ay);
         else
-            near_field_rows_global<3>(s, hrec, cellNear, cellBody, g, p.x, p.y, ax, ay);
+            near_field_rows<3>(s, hrec, GlobalRows{cellNear}, cellBody, g, p.x, p.y, ax, ay);
     } else {
         float2 mf = mesh_force(s, force, p.x, p.y);
         ax = mf.x + s.fieldX;
