@@ -473,6 +473,7 @@ void Engine::rebuild_tables() {
     std::vector<BodyBlock> blocks;
     std::vector<LineWork> rowsF, rowsI, cols;
     anyMutual_ = anyLiquid_ = anyCollide_ = anySleep_ = anyAdaptive_ = false;
+    bool anyWalls = false, anyTorus = false;
     gravitySlots_ = 0;
     liquidSlots_ = 0;
     anyViscous_ = false;
@@ -487,6 +488,7 @@ void Engine::rebuild_tables() {
         }
         anyCollide_ |= h.cfg.collisions_enabled != 0;
         anyAdaptive_ |= h.desc.adaptive != 0;
+        (h.bounded ? anyWalls : anyTorus) = true;
         if (!h.mutual) continue;
         anyMutual_ = true;
         gravitySlots_ += h.n;
@@ -533,6 +535,7 @@ void Engine::rebuild_tables() {
     shape_.waveWork = waveWork_;
     shape_.adapt = anyAdaptive_ ? adapt_ : nullptr;
     shape_.adaptDeepest = adaptDeepest_;
+    shape_.walls = anyWalls && anyTorus ? WALLS_MIXED : (anyWalls ? WALLS_ALL : WALLS_NONE);
     shape_.pool = ContactPool{poolItems_, poolBase_, poolTop_, (uint32_t)totalSlots_ * CONTACT_SLOTS};
     invalidate_graphs();
 }

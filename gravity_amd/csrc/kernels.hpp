@@ -46,7 +46,12 @@ struct LaunchShape {
     // adaptive collisions, else null (the passes then take the plain kernels).
     uint32_t* adapt;
     float* adaptDeepest;   // per body: its deepest pair penetration in the last final pass
+    // Boundaries of the batch's simulations: WALLS_NONE (all on the torus),
+    // WALLS_ALL, or WALLS_MIXED; kernels specialised on walls launch the
+    // variants the batch needs.
+    int walls;
 };
+enum { WALLS_NONE = 0, WALLS_ALL = 1, WALLS_MIXED = 2 };
 
 // Mesh work item: {simulation, first line, lines}.
 struct LineWork { int sim; int first; int count; int pad; };

@@ -146,14 +146,19 @@ __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1
 
 // The near field's constants, read once per body into registers: the
 // per-cell helpers below take either this or the SimDesc itself (the same
-// member names), and a loop over the stencil then reloads nothing.
+// member names), and a loop over the stencil then reloads nothing. Walls or
+// torus are part of the type, so a body term under walls carries no minimum
+// image.
+template <bool BOUNDED>
 struct NearK {
-    int bounded, pw, ph, exactLimit, homeLimit;
+    static constexpr bool bounded = BOUNDED;
+    int pw, ph, exactLimit, homeLimit;
     float aspect, invAspect, gOverM, minDist2;
 };
-__device__ __forceinline__ NearK near_k(const SimDesc& s) {
-    return NearK{s.bounded, s.pw, s.ph, s.exactLimit, s.homeLimit,
-                 s.aspect, s.invAspect, s.gOverM, s.minDist2};
+template <bool BOUNDED>
+__device__ __forceinline__ NearK<BOUNDED> near_k(const SimDesc& s) {
+    return NearK<BOUNDED>{s.pw, s.ph, s.exactLimit, s.homeLimit,
+                          s.aspect, s.invAspect, s.gOverM, s.minDist2};
 }
 
 template <typename K>
@@ -339,12 +344,12 @@ struct GlobalRows {
     }
 };
 
-template <int R, typename Rows>
+template <int R, bool BOUNDED, typename Rows>
 __device__ __forceinline__ void near_field_rows(const SimDesc& s, uint4 hrec, const Rows& rows,
                                                 const float4* __restrict__ cellBody, int g,
                                                 float x, float y, float& ax, float& ay) {
     const float k2 = s.invTwoSigma2;
-    const NearK c = near_k(s);
+    const NearK<BOUNDED> c = near_k<BOUNDED>(s);
     int hx = (int)floorf(x * (float)s.pw), hy = (int)floorf(y * (float)s.ph);
     constexpr int D = 2 * R + 1;
     constexpr int PART = 3;
@@ -430,6 +435,10 @@ __device__ __forceinline__ void near_field_any(const SimDesc& s, const uint4* __
 // block across the torus seam, a non-finite position) reads global memory.
 constexpr int KICK_STAGE = 640;   // near records a workgroup stages (20 KB of LDS)
 
+// WALLS: the simulations this launch kicks (1 under walls, 0 on the torus);
+// a block of a simulation of the other kind leaves at once (a batch that
+// mixes both launches both).
+template <int WALLS>
 __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
                                              const BodyBlock* __restrict__ blocks,
                                              float4* __restrict__ body,
@@ -443,6 +452,7 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
     __shared__ int waveBox[WG / 64][4];
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const SimDesc& s = sims[b.sim];
+    if (s.bounded != WALLS) return;
     int g = b.first + (int)threadIdx.x;
     // Every lane stays to the staging barriers; a lane without a body to kick
     // leaves after them.
@@ -522,9 +532,11 @@ __global__ void __launch_bounds__(WG) k_kick(const SimDesc* __restrict__ sims,
         ax = mf.x + s.fieldX;
         ay = mf.y + s.fieldY;
         if (staged)
-            near_field_rows<3>(s, hrec, StagedRows{stage, x0, y0, bw}, cellBody, g, p.x, p.y, ax, ay);
+            near_field_rows<3, WALLS != 0>(s, hrec, StagedRows{stage, x0, y0, bw}, cellBody, g,
+                                           p.x, p.y, ax, ay);
         else
-            near_field_rows<3>(s, hrec, GlobalRows{cellNear}, cellBody, g, p.x, p.y, ax, ay);
+            near_field_rows<3, WALLS != 0>(s, hrec, GlobalRows{cellNear}, cellBody, g, p.x, p.y,
+                                           ax, ay);
     } else {
         float2 mf = mesh_force(s, force, p.x, p.y);
         ax = mf.x + s.fieldX;
@@ -570,8 +582,12 @@ void launch_kick(hipStream_t st, const LaunchShape& ls, float4* body, float2* ac
                  const float2* force, const uint4* cellRec, const uint4* cellNear,
                  const float4* cellBody, int initial) {
     if (ls.bodyBlocks == 0) return;
-    launch(k_kick, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
-                       force, cellRec, cellNear, cellBody, initial, ls.sleep);
+    if (ls.walls != WALLS_ALL)
+        launch(k_kick<0>, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
+               force, cellRec, cellNear, cellBody, initial, ls.sleep);
+    if (ls.walls != WALLS_NONE)
+        launch(k_kick<1>, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks, body, acc,
+               force, cellRec, cellNear, cellBody, initial, ls.sleep);
 }
 
 }  // namespace gs
