@@ -26,51 +26,74 @@ __device__ __forceinline__ int liquid_cell(const SimDesc& s, float x, float y) {
 
 // Every other body of the 3x3 start cells, in the canonical order, with the
 // data `load` fetches for it. On a grid narrower than three cells a wrapped
-// row or column met twice is taken once. The nine cells' lists are taken as
-// one sequence, LIQ_BATCH entries at a time: a batch's list entries load
-// together, then the neighbours' data together (each clamped to a valid
-// entry), so a batch costs two memory round trips where an entry at a time
-// cost two per neighbour; the visits keep the canonical order.
+// row or column met twice is taken once. Consecutive cells are contiguous
+// runs of `items` (they are counting-sorted by cell), so a stencil row's
+// cells, taken in the canonical column order -1, 0, 1, are at most two runs:
+// one under walls or away from the torus's seam, two where the row wraps.
+// The three rows' runs are taken as one sequence of six pieces, LIQ_BATCH
+// entries at a time: a batch's list entries load together, then the
+// neighbours' data together (each clamped to a valid entry), so a batch
+// costs two memory round trips where an entry at a time cost two per
+// neighbour; the visits keep the canonical order.
 constexpr int LIQ_BATCH = 8;
 template <typename Load, typename Visit>
 __device__ __forceinline__ void liquid_neighbours(const SimDesc& s, int g, int home,
                                                   const uint32_t* __restrict__ prefix,
                                                   const uint32_t* __restrict__ items, Load&& load,
                                                   Visit&& visit) {
-    int local = home - s.liqCellBase;
-    int hx = local % s.lgw, hy = local / s.lgw;
-    int ys[3], xs[3];
-    bool oky[3], okx[3];
+    const int local = home - s.liqCellBase;
+    const int hx = local % s.lgw, hy = local / s.lgw, lgw = s.lgw;
+    // The row's column runs [aLo, aHi] and [bLo, bHi] (empty when lo > hi).
+    int aLo, aHi, bLo = 1, bHi = 0;
+    if (s.bounded) {
+        aLo = max(hx - 1, 0);
+        aHi = min(hx + 1, lgw - 1);
+    } else if (lgw >= 3 && hx == 0) {
+        aLo = aHi = lgw - 1;
+        bLo = 0;
+        bHi = 1;
+    } else if (lgw >= 3 && hx == lgw - 1) {
+        aLo = lgw - 2;
+        aHi = lgw - 1;
+        bLo = bHi = 0;
+    } else if (lgw >= 3) {
+        aLo = hx - 1;
+        aHi = hx + 1;
+    } else if (lgw == 2 && hx == 0) {
+        aLo = aHi = 1;
+        bLo = bHi = 0;
+    } else {
+        aLo = 0;
+        aHi = lgw - 1;
+    }
+    const bool bRun = bLo <= bHi;
+    // Piece 2j + r is run r of stencil row j: entries [first, first + its
+    // length) of `items`, entries [cum, next cum) of the sequence. A row
+    // outside the walls, or met twice on a grid narrower than three rows,
+    // reads the home cell's bounds and is empty.
+    uint32_t first[6], cum[7];
+    cum[0] = 0u;
+    int ys[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        int y = hy + j - 1, x = hx + j - 1;
+        int y = hy + j - 1;
+        bool ok;
         if (s.bounded) {
-            oky[j] = y >= 0 && y < s.lgh;
-            okx[j] = x >= 0 && x < s.lgw;
+            ok = y >= 0 && y < s.lgh;
         } else {
             y = wrap_axis(y, s.lgh);
-            x = wrap_axis(x, s.lgw);
-            oky[j] = (j < 1 || y != ys[0]) && (j < 2 || y != ys[1]);
-            okx[j] = (j < 1 || x != xs[0]) && (j < 2 || x != xs[1]);
+            ok = (j < 1 || y != ys[0]) && (j < 2 || y != ys[1]);
         }
         ys[j] = y;
-        xs[j] = x;
+        const int row = ok ? s.liqCellBase + y * lgw : home - hx;
+        const uint32_t a0 = prefix[row + aLo], a1 = prefix[row + aHi + 1];
+        const uint32_t b0 = prefix[row + (bRun ? bLo : aLo)], b1 = prefix[row + (bRun ? bHi + 1 : aLo)];
+        first[2 * j] = a0;
+        first[2 * j + 1] = b0;
+        cum[2 * j + 1] = cum[2 * j] + (ok ? a1 - a0 : 0u);
+        cum[2 * j + 2] = cum[2 * j + 1] + (ok && bRun ? b1 - b0 : 0u);
     }
-    // Cell k's list starts at first[k] and covers entries [cum[k], cum[k+1])
-    // of the sequence; a cell that does not take part reads the home cell's
-    // bounds and counts as empty.
-    uint32_t first[9], cum[10];
-    cum[0] = 0u;
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const int j = k / 3, i = k % 3;
-        const bool ok = oky[j] && okx[i];
-        const int c = ok ? s.liqCellBase + ys[j] * s.lgw + xs[i] : home;
-        const uint32_t t0 = prefix[c], t1 = prefix[c + 1];
-        first[k] = t0;
-        cum[k + 1] = cum[k] + (ok ? t1 - t0 : 0u);
-    }
-    const uint32_t total = cum[9];
+    const uint32_t total = cum[6];
     using Data = decltype(load(0));
     for (uint32_t k0 = 0; k0 < total; k0 += LIQ_BATCH) {
         int id[LIQ_BATCH];
@@ -79,7 +102,7 @@ __device__ __forceinline__ void liquid_neighbours(const SimDesc& s, int g, int h
             const uint32_t e = min(k0 + (uint32_t)q, total - 1u);
             uint32_t pos = first[0] + e;
 #pragma unroll
-            for (int k = 1; k < 9; k++)
+            for (int k = 1; k < 6; k++)
                 if (e >= cum[k]) pos = first[k] + (e - cum[k]);
             id[q] = (int)items[pos];
         }

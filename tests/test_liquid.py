@@ -218,6 +218,13 @@ def _scenes():
     # a torus narrower than three liquid cells: wrapped cells met twice
     xy = rng.uniform(0, 1, (30, 2)) * np.array([5.0, 24.0])
     out.append(("narrow_torus", 5, 24, False, pack(xy, 5, 24, False), fluid(30)))
+    # a band across the whole width of a torus: stencils on both seams, whose
+    # rows wrap from the last liquid column to the first
+    xy = rng.uniform(0, 1, (400, 2)) * np.array([40.0, 8.0]) + np.array([0.0, 11.0])
+    out.append(("torus_seam", 40, 30, False, pack(xy, 40, 30, False), fluid(400)))
+    # a box narrower than three liquid cells
+    xy = rng.uniform(0, 1, (30, 2)) * np.array([5.0, 20.0]) + np.array([0.0, 2.0])
+    out.append(("narrow_box", 5, 24, True, pack(xy, 5, 24, True), fluid(30)))
     return out
 
 
