@@ -18,6 +18,9 @@ S/kick.comp:66-156), each check chosen so that a plausible mistake fails:
 - on the torus the mean pair force is the nearest image's, across the seams
   too; the FP64 Ewald sum (oracle/ewald.py) states how far that periodic
   model is from the periodic field with its uniform background;
+- the torus has no preferred origin: a clump moved by half the torus onto
+  the corner where both seams meet feels the same forces, so a wrong
+  minimum image in either axis fails;
 - the FP32 density spectrum is within the reference's measured bound
   against an FP64 transform of the same deposit.
 """
@@ -162,6 +165,28 @@ def test_mean_pair_force_on_the_torus_is_the_nearest_images(oracle, cells, bias,
         err.append(np.hypot(*(g - newton)) / np.hypot(*newton))
     assert abs(np.mean(radial)) < bias, np.mean(radial)
     assert np.mean(err) < spread, np.mean(err)
+
+
+def test_forces_are_translation_invariant_across_the_torus_seams(oracle):
+    # The torus has no preferred origin: a clump shifted by half the torus,
+    # whole mesh cells each way, so that it straddles both seams where it
+    # sat in the middle, feels the same force on every body. The shift moves
+    # the CIC deposit by whole cells, and every near-field pair and stencil
+    # cell that crossed no seam now crosses one, so a wrong minimum image in
+    # either axis, of the mesh gather or of the near field, fails.
+    rng = np.random.default_rng(23)
+    n = 400
+    xy = 0.5 + rng.normal(0.0, 1.0, (n, 2)) * np.array([0.02, 0.04])
+    a0 = physical(accelerations(oracle, xy.astype(np.float32), bounded=False))
+    xy1 = np.mod(xy + np.array([(PW // 2) / PW, (PH // 2) / PH]), 1.0)
+    a1 = physical(accelerations(oracle, xy1.astype(np.float32), bounded=False))
+    # pairs within the near field (three cells) across each seam after the shift
+    d = xy1[None] - xy1[:, None]
+    close = np.hypot(*physical(d - np.round(d)).transpose(2, 0, 1)) < 3.0 / PH
+    assert (close & (np.abs(d[..., 0]) > 0.5)).sum() > 1000
+    assert (close & (np.abs(d[..., 1]) > 0.5)).sum() > 1000
+    scale = np.sqrt((a0 ** 2).sum(1).mean())
+    assert np.abs(a1 - a0).max() < 1e-4 * scale, np.abs(a1 - a0).max() / scale
 
 
 def test_ewald_sum_is_periodic_odd_and_independent_of_its_split():
