@@ -223,6 +223,10 @@ __device__ __forceinline__ void walk_stencil_serial(const SimDesc& s, int cx, in
 // its velocity dropped, and no pass of the step moves it.
 // With adaptive collisions a drift takes the step's substep, dt over the
 // substeps chosen (S/collision_cache_drift.comp:79-80).
+// `drifts` consecutive drifts of driftDt (after the kick, if any) run in one
+// launch: without contact passes between them a body's drifts depend on
+// nothing but itself, so the gravity-only step's substep drifts take one
+// launch, each drift rounding and wrapping exactly as a launch of its own.
 __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__ sims,
                                                      const BodyBlock* __restrict__ blocks,
                                                      float4* __restrict__ body,
@@ -230,7 +234,8 @@ __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__
                                                      const float4* __restrict__ mats,
                                                      float kickDt, float driftDt, int insert,
                                                      GridBuf grid, SleepBufs sleep,
-                                                     const uint32_t* __restrict__ adapt) {
+                                                     const uint32_t* __restrict__ adapt,
+                                                     int drifts) {
     BodyBlock b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     // The body and its acceleration load before the simulation's constants
     // are known (both arrays cover the padded slots; a slot past the active
@@ -268,14 +273,16 @@ __global__ void __launch_bounds__(WG) k_drift_insert(const SimDesc* __restrict__
             v.z += a.x * h;
             v.w += a.y * h;
         }
-        v.x += v.z * driftDt;
-        v.y += v.w * driftDt;
-        if (s.bounded) {
-            v.x = clampf(v.x, 0.0f, MAXIMUM_UNIT);
-            v.y = clampf(v.y, 0.0f, MAXIMUM_UNIT);
-        } else {
-            v.x = fractf(v.x);
-            v.y = fractf(v.y);
+        for (int r = 0; r < drifts; r++) {
+            v.x += v.z * driftDt;
+            v.y += v.w * driftDt;
+            if (s.bounded) {
+                v.x = clampf(v.x, 0.0f, MAXIMUM_UNIT);
+                v.y = clampf(v.y, 0.0f, MAXIMUM_UNIT);
+            } else {
+                v.x = fractf(v.x);
+                v.y = fractf(v.y);
+            }
         }
         body[g] = v;
     }
@@ -1048,11 +1055,11 @@ void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b) {
 
 void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
                          const float4* mats, float kickDt, float driftDt, int insert,
-                         const CollideGrid& grid) {
+                         const CollideGrid& grid, int drifts) {
     if (ls.bodyBlocks == 0) return;
     launch(k_drift_insert, dim3(ls.bodyBlocks), dim3(WG), 0, st, ls.sims, ls.blocks,
                        body, acc, mats, kickDt, driftDt, insert, grid_of(grid), ls.sleep,
-                       (const uint32_t*)ls.adapt);
+                       (const uint32_t*)ls.adapt, drifts);
 }
 
 void launch_adapt(hipStream_t st, const LaunchShape& ls, int simCount, const float4* body) {

@@ -842,13 +842,12 @@ void Engine::enqueue_step(bool sortDue) {
             });
         }
     } else {
-        // Gravity-only configurations: kick-drift-kick with plain drifts.
-        for (int k = 0; k < S; k++) {
-            timed("drift", [&] {
-                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_],
-                                    k == 0 ? dt : 0.0f, sub, 0, grid_[gridPar_]);
-            });
-        }
+        // Gravity-only configurations: kick-drift-kick with plain drifts, the
+        // S substep drifts in one launch.
+        timed("drift", [&] {
+            launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_], dt, sub,
+                                0, grid_[gridPar_], S);
+        });
         enqueue_force_and_kick(false);
     }
 }
@@ -1096,9 +1095,8 @@ void Engine::adjoint_vjp(int count, const float* dLdx, float* dLdx0, float* dLdv
                                      hipMemcpyDeviceToDevice, st_), "adjoint slot");
             }
             if (j + 1 == k) break;  // the last step's build is not needed here
-            for (int q = 0; q < S; q++)
-                launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_],
-                                    q == 0 ? dt : 0.0f, sub, 0, grid_[gridPar_]);
+            launch_drift_insert(st_, shape_, body_[bodyPar_], acc_[auxPar_], mat_[auxPar_], dt, sub, 0,
+                                grid_[gridPar_], S);
             enqueue_force();
             adjoint_save_slot(j + 1);
             enqueue_kick(false);

@@ -83,9 +83,11 @@ void launch_sort_canon(hipStream_t st, const LaunchShape& ls, const uint32_t* sl
 
 // ---- collisions ----
 void launch_stop(hipStream_t st, const LaunchShape& ls, float4* a, float4* b);
+// `drifts` drifts of driftDt after the kick, in one launch (the gravity-only
+// step's substeps); a contact pass's drift is one.
 void launch_drift_insert(hipStream_t st, const LaunchShape& ls, float4* body, const float2* acc,
                          const float4* mats, float kickDt, float driftDt, int insert,
-                         const CollideGrid& grid);
+                         const CollideGrid& grid, int drifts = 1);
 // Rebuilds ls.order: on every XCD, its body blocks by descending contacts of
 // the last count, so the heaviest blocks are dispatched first, one per CU.
 void launch_block_order(hipStream_t st, const LaunchShape& ls, OrderedBlock* order);

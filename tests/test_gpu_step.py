@@ -9,20 +9,25 @@ from gravity_amd import scenarios
 pytestmark = pytest.mark.gpu
 
 
-def test_gravity_only_steps_follow_the_oracle(oracle):
-    # The P3M-only workload at reduced N: positions within 1e-3 px.
+@pytest.mark.parametrize("substeps,bounded", [(1, False), (16, False), (16, True)])
+def test_gravity_only_steps_follow_the_oracle(oracle, substeps, bounded):
+    # The P3M-only workload at reduced N: positions within 1e-3 px; with 16
+    # substeps the step's drifts run in one launch, each wrapping (torus) or
+    # clamping (walls) as the oracle's pass does.
     W, H, n, steps = 1440, 720, 5000, 20
+    print(f"[step] gravity only, {substeps} substeps, walls {bounded}", flush=True)
     bodies, mats = scenarios.uniform(n, W, H, seed=1)
     cfg = GravityConfig(particles=ParticleConfig(capacity=n, initialBodies=n),
                         gravity=GravityFieldConfig(gridWidth=512, gridHeight=256),
-                        collisions=CollisionConfig(maximumSubsteps=1), collisionsEnabled=False)
-    with GravityPipeline(cfg, W, H, scenarioState=bodies) as pipe:
+                        collisions=CollisionConfig(maximumSubsteps=substeps), collisionsEnabled=False)
+    with GravityPipeline(cfg, W, H, scenarioState=bodies, bounded=bounded) as pipe:
         pipe.initializeAcceleration(n)
         for _ in range(steps):
             pipe.step(n)
         got = pipe.bodies(n)
         dest = pipe.sortDestinations(n)
-    sim = oracle.Sim(oracle.params(W, H, collide=False, substeps=1, grid=(512, 256)), bodies, mats)
+    sim = oracle.Sim(oracle.params(W, H, bounded=bounded, collide=False, substeps=substeps, grid=(512, 256)),
+                     bodies, mats)
     sim.initialize_acceleration()
     for _ in range(steps):
         sim.step()
