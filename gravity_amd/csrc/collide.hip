@@ -459,6 +459,7 @@ __global__ void __launch_bounds__(WG) k_count(const SimDesc* __restrict__ sims,
 // last full XCD round, and XCDs with more than ORDER_MAX blocks, keep the
 // xcd_block order.
 constexpr int ORDER_MAX = 512;
+constexpr int CUS_PER_XCD = 32;
 __device__ __forceinline__ OrderedBlock ordered(const BodyBlock* __restrict__ blocks, int i) {
     const BodyBlock b = blocks[i];
     return OrderedBlock{b.sim, b.first, i, 0};
@@ -491,7 +492,13 @@ __global__ void __launch_bounds__(WG) k_block_order(int nb, const BodyBlock* __r
         uint32_t kj = key[j];
         int rank = 0;
         for (int k = 0; k < m; k++) rank += key[k] > kj ? 1 : 0;
-        order[x + XCDS * rank] = ordered(blocks, item[j]);
+        // Ranks are dealt to the XCD's workgroup positions in rounds of
+        // CUS_PER_XCD, every other full round reversed, so each CU's share
+        // of the heaviest blocks evens out.
+        const int round = rank / CUS_PER_XCD, in = rank % CUS_PER_XCD;
+        const bool reversed = (round & 1) && (round + 1) * CUS_PER_XCD <= m;
+        const int pos = round * CUS_PER_XCD + (reversed ? CUS_PER_XCD - 1 - in : in);
+        order[x + XCDS * pos] = ordered(blocks, item[j]);
     }
 }
 
