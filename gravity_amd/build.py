@@ -18,7 +18,7 @@ PACKAGE = Path(__file__).resolve().parent
 CSRC = PACKAGE / "csrc"
 BUILD = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libgravity_hip.so"
-SOURCES = ["sort.hip", "collide.hip", "p3m.hip", "fft.hip", "liquid.hip", "sleep.hip", "adjoint.hip", "trajectory.hip",
+SOURCES = ["sort.hip", "collide.hip", "p3m.hip", "fft.hip", "liquid.hip", "sleep.hip", "adjoint.hip", "trajectory.hip", "rigid.hip",
            "engine.hip", "green.cpp", "members.cpp", "tuning.cpp", "capi.cpp"]
 ARCH = os.environ.get("GRAVITY_OFFLOAD_ARCH", "gfx950")
 

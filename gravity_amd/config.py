@@ -166,6 +166,21 @@ class TrajectoryConfig:
 
 
 @dataclass(frozen=True)
+class ProjectileConfig:
+    """The block fields of ProjectileConfig (K/GravityConfig.kt:760-810)."""
+    blockDensity: float = 0.5
+    blockFriction: float = 0.5
+    blockRestitution: float = 0.2
+    blockStrength: float = 400.0
+
+    def __post_init__(self):
+        _require(0.05 <= self.blockDensity <= 10.0, "block density must be in 0.05..10.0")
+        _require(self.blockFriction >= 0 and math.isfinite(self.blockFriction), "block friction must be non-negative")
+        _require(0.0 <= self.blockRestitution <= 1.0, "block restitution must be in 0..1")
+        _require(self.blockStrength > 0 and not math.isnan(self.blockStrength), "block strength must be positive")
+
+
+@dataclass(frozen=True)
 class StartConfig:
     """StartConfig.UNIFORM, the start benchmarks were measured with."""
     walls: bool = False
@@ -181,6 +196,7 @@ class GravityConfig:
     liquid: LiquidConfig = field(default_factory=LiquidConfig)
     start: StartConfig = field(default_factory=StartConfig)
     trajectory: TrajectoryConfig = field(default_factory=TrajectoryConfig)
+    projectiles: ProjectileConfig = field(default_factory=ProjectileConfig)
     # Extension: False runs plain drifts instead of contact passes.
     collisionsEnabled: bool = True
 
@@ -319,8 +335,14 @@ class GravityConfig:
             updatesPerSecond=integer("trajectory-updates-per-second", d.trajectory.updatesPerSecond),
             minimumDashPixels=number("trajectory-min-dash-px", d.trajectory.minimumDashPixels),
         )
+        projectiles = ProjectileConfig(
+            blockDensity=number("block-density", d.projectiles.blockDensity),
+            blockFriction=number("block-friction", d.projectiles.blockFriction),
+            blockRestitution=number("block-restitution", d.projectiles.blockRestitution),
+            blockStrength=number("block-strength", d.projectiles.blockStrength),
+        )
         return GravityConfig(particles=particles, gravity=gravity, collisions=collisions, liquid=liquid,
-                             start=start, trajectory=trajectory)
+                             start=start, trajectory=trajectory, projectiles=projectiles)
 
     def native(self, width: int, height: int, bounded: bool) -> native.GsConfig:
         """The flat C struct gs_create takes."""
@@ -370,6 +392,7 @@ class GravityConfig:
         c.trajectory_target_segment_pixels = t.targetSegmentPixels
         c.trajectory_maximum_path_diagonals = t.maximumPathViewportDiagonals
         c.trajectory_minimum_dash_pixels = t.minimumDashPixels
+        c.block_friction = self.projectiles.blockFriction
         return c
 
 

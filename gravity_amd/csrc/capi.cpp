@@ -84,6 +84,7 @@ void gs_config_defaults(gs_config* c) {
     c->trajectory_target_segment_pixels = 6.0f;
     c->trajectory_maximum_path_diagonals = 1.0f;
     c->trajectory_minimum_dash_pixels = 3.0f;
+    c->block_friction = 0.5f;
 }
 
 int gs_create(const gs_config* cfg, const float* state, int n, gs_sim** out) {
@@ -439,6 +440,39 @@ int gs_profile_steps(gs_sim* sim, int steps, gs_kernel_stat* out, int capacity, 
         }
         if (written) *written = k;
     });
+}
+
+int gs_rigid_glue(gs_sim* sim, int first_body, int count, const float* offsets_xy, float center_x,
+                  float center_y, int* group) {
+    SIM_OR_FAIL
+    if (count > 0 && !offsets_xy) return fail(GS_EINVAL, "null argument");
+    return guarded([&] {
+        const int g = e.rigid_glue(first_body, count, offsets_xy, center_x, center_y);
+        if (group) *group = g;
+    });
+}
+
+int gs_rigid_project(gs_sim* sim, int contacts) {
+    SIM_OR_FAIL
+    return guarded([&] { e.rigid_project(contacts != 0); });
+}
+
+int gs_rigid_size(gs_sim* sim, int* out) {
+    SIM_OR_FAIL
+    if (!out) return fail(GS_EINVAL, "null argument");
+    return guarded([&] { e.rigid_size(out); });
+}
+
+int gs_rigid_read(gs_sim* sim, int what, int first, int count, void* dst) {
+    SIM_OR_FAIL
+    if (count > 0 && !dst) return fail(GS_EINVAL, "null argument");
+    return guarded([&] { e.rigid_access(what, first, count, dst, false); });
+}
+
+int gs_rigid_write(gs_sim* sim, int what, int first, int count, const void* src) {
+    SIM_OR_FAIL
+    if (count > 0 && !src) return fail(GS_EINVAL, "null argument");
+    return guarded([&] { e.rigid_access(what, first, count, const_cast<void*>(src), true); });
 }
 
 int gs_set_graphs(gs_sim* sim, int enabled) {

@@ -28,6 +28,28 @@ constexpr int CONTACT_PLANES = 112;   // planes of the contact array: the listed
 constexpr int XCDS = 8;               // accelerator dies of an MI355X, each with its own L2
 constexpr float MAXIMUM_UNIT = 0.99999994f;  // largest float below 1
 
+// Glued blocks (S/rigid_layout.glsl): a block's partners in the contact
+// memory are the other blocks, the four walls and the loose bodies as one;
+// a pair belongs to its lower block.
+constexpr int RIGID_MAXIMUM_GROUPS = 256;
+constexpr int RIGID_CONTACT_PARTNERS = RIGID_MAXIMUM_GROUPS + 5;
+constexpr int RIGID_CONTACT_KEYS = RIGID_MAXIMUM_GROUPS * RIGID_CONTACT_PARTNERS;
+
+// RigidGroup (S/rigid_layout.glsl:4-15), 160 bytes.
+struct RigidGroup {
+    float4 frame;    // reference centre (px), centre (px)
+    float4 motion;   // velocity (px/s), rotation (cos, sin)
+    float4 scalars;  // angular velocity (rad/s), inertia sum |r0|^2, mass (members), struck this pass
+    int4 tiles;      // first tile, tile count, first hull value, hull points
+    float4 shape;    // distance field: rest-frame origin (px), spacing (px), smallest border value
+    int4 grid;       // first shape value, grid width, grid height, partners in contact this pass
+    float4 bounds;   // rest-frame extent of the member centres: min x, max x, min y, max y (px)
+    int4 guard;      // shift out of other blocks asked for: +x, +y, -x, -y (1/65536 px)
+    int4 support;    // shift off bodies wedged against a wall, likewise
+    float4 pose;     // centre (px) and rotation after the current pass's drift
+};
+static_assert(sizeof(RigidGroup) == 160, "RigidGroup is the shader's 160-byte record");
+
 // Per-simulation constants, mirrored from GravityConfig and the viewport
 // (K/GravityPipeline.kt:73-241). Derived floats are computed on the host
 // with the same operations the shaders apply to their uniforms.

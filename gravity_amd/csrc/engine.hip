@@ -328,6 +328,7 @@ Engine::Engine(const std::vector<gs_config>& cfgs, const std::vector<const float
 Engine::~Engine() {
     invalidate_graphs();
     adjoint_free();
+    rigid_free();
     for (void* p : adjWork_) hipFree(p);
     if (debugDensity_) hipFree(debugDensity_);
     for (auto& e : events_) {
@@ -726,6 +727,8 @@ void Engine::enqueue_sort() {
     });
     bodyPar_ = o;
     auxPar_ = a;
+    // K/GravityPipeline.kt:1315: the blocks' members follow their bodies
+    rigid_remap();
 }
 
 void Engine::enqueue_force_and_kick(bool initial) {
@@ -890,6 +893,7 @@ void Engine::enqueue_liquid() {
 }
 
 void Engine::debug_liquid() {
+    refuse_blocks("a liquid pass");
     normalize();
     if (bodyBlocks_ == 0 || !anyLiquid_) return;
     enqueue_liquid();
@@ -951,6 +955,7 @@ static void require_single_base0(const std::vector<HostSim>& sims, const char* w
 }
 
 void Engine::adjoint_record(int steps, int every) {
+    refuse_blocks("the adjoint");
     require_single_base0(sims_, "the adjoint runs on a single simulation");
     const HostSim& h = sims_[0];
     if (h.cfg.collisions_enabled) throw ArgumentError("the adjoint differentiates the gravity step without contact passes");
@@ -1150,7 +1155,15 @@ void Engine::wake(int s) {
     check(hipGetLastError(), "wake");
 }
 
+void Engine::refuse_blocks(const char* what) const {
+    // The contact passes of a pipeline with glued blocks are the RIGID
+    // variants (S/collision_rigid*.glsl), which this library does not have.
+    if (rigid_.groupCount > 0)
+        throw StateError(std::string(what) + " with glued blocks needs the RIGID contact passes, which are not built");
+}
+
 void Engine::step() {
+    refuse_blocks("a step");
     normalize();
     // K/GravityPipeline.kt:446-452: new bodies, walls, field, gravity or
     // liquid wake everything for a while (nothing has marked the tiles).
@@ -1204,6 +1217,7 @@ void Engine::stop_bodies(int s) {
 }
 
 void Engine::debug_collide(float kickDt, float driftDt, bool writePressure) {
+    refuse_blocks("a contact pass");
     normalize();
     if (bodyBlocks_ == 0) return;
     enqueue_pass(kickDt, driftDt, writePressure);

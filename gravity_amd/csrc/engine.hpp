@@ -138,6 +138,14 @@ public:
     // against an FP64 transform of the same deposit: out = {max abs error,
     // max abs error / max |reference|, RMS error}.
     void validate_fft(double* out);
+    // Glued blocks of simulation 0 (rigid.hip): RigidBlocks.add without the
+    // distance field (K/RigidBlocks.kt:280-301), project for blocks nothing
+    // breaks (:309-357), the size, and access to the block buffers.
+    int rigid_glue(int firstBody, int count, const float* offsets, float centerX, float centerY);
+    void rigid_project(bool contacts);
+    void rigid_size(int* out) const;
+    void rigid_access(int what, long long first, long long count, void* host, bool write);
+
     void debug_collide(float kickDt, float driftDt, bool writePressure);
     void debug_sort();
     void debug_liquid();
@@ -163,6 +171,10 @@ private:
     void enqueue_liquid();
     void clear_grids();
     void invalidate_graphs();
+    void rigid_allocate();
+    void rigid_free();
+    void rigid_remap();
+    void refuse_blocks(const char* what) const;
     template <typename F> void timed(const char* name, F&& f);
 
     std::vector<HostSim> sims_;
@@ -220,6 +232,19 @@ private:
     float4* trajectory_ = nullptr;   // simulation 0's preview vertices
     int trajectoryVertices_ = 0;
     LaunchShape shape_{};
+
+    // glued blocks (K/RigidBlocks.kt:84-113), allocated by the first block
+    struct RigidState {
+        RigidGroup* groups = nullptr;   // RIGID_MAXIMUM_GROUPS records
+        int* memory = nullptr;          // impacts, next impacts, states, idle passes, pair contacts
+        float* shapes = nullptr;        // per block: its hull's rest-frame corners
+        int* members = nullptr;         // per member slot: its body, -1 for padding
+        float2* rest = nullptr;         // per member slot: rest offset about the centre (px)
+        int* tileGroups = nullptr;      // per tile: its block
+        float4* partials = nullptr;     // per tile: two sums
+        int groupCount = 0, tileCount = 0, memberCap = 0, tileCap = 0;
+        long long valueCap = 0, valueTop = 0;
+    } rigid_;
 
     // graphs keyed by (bodyPar, auxPar, gridPar, sortDue), with the parities
     // the captured schedule ends in

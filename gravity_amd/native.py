@@ -17,6 +17,8 @@ GS_OK, GS_EINVAL, GS_ESTATE = 0, -1, -2
 GS_BOUNDED, GS_MUTUAL_GRAVITY, GS_LIQUID = 1, 2, 3
 GS_BODIES_XYVV, GS_PRESSURES, GS_ACCEL, GS_MATERIALS = 0, 1, 2, 3
 GS_CONTACT_COUNTS, GS_SORT_DEST, GS_FORCE_MESH, GS_CONTACTS = 4, 5, 6, 7
+GS_RIGID_GROUPS, GS_RIGID_CONTACT_MEMORY, GS_RIGID_MEMBERS = 0, 1, 2
+GS_RIGID_REST, GS_RIGID_SHAPES, GS_RIGID_TILE_GROUPS = 3, 4, 5
 CONTACT_SLOTS = 16
 
 
@@ -69,6 +71,7 @@ class GsConfig(ctypes.Structure):
         ("trajectory_target_segment_pixels", ctypes.c_float),
         ("trajectory_maximum_path_diagonals", ctypes.c_float),
         ("trajectory_minimum_dash_pixels", ctypes.c_float),
+        ("block_friction", ctypes.c_float),
     ]
 
 
@@ -134,6 +137,11 @@ SIGNATURES = {
     "gs_download_trajectory": (_I, [_P, _I, _I, _P]),
     "gs_render_view": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "gs_render_copy": (_I, [_P, _I, _P, _P]),
+    "gs_rigid_glue": (_I, [_P, _I, _I, _P, _F, _F, ctypes.POINTER(_I)]),
+    "gs_rigid_project": (_I, [_P, _I]),
+    "gs_rigid_size": (_I, [_P, _P]),
+    "gs_rigid_read": (_I, [_P, _I, _I, _I, _P]),
+    "gs_rigid_write": (_I, [_P, _I, _I, _I, _P]),
     "gs_synchronize": (_I, [_P]),
     "gs_query": (_I, [_P, ctypes.POINTER(_I)]),
     "gs_profile_steps": (_I, [_P, _I, ctypes.POINTER(GsKernelStat), _I, ctypes.POINTER(_I)]),

@@ -254,6 +254,41 @@ class GravityPipeline:
                                                    ctypes.byref(spawned)))
         return spawned.value
 
+    # -- glued blocks (K/RigidBlocks.kt) ------------------------------------
+    def glueBlock(self, firstBody: int, offsets: np.ndarray, centerX: float, centerY: float) -> int:
+        """RigidBlocks.add (K/RigidBlocks.kt:280-301): bodies firstBody.. at
+        centre + offsets (px, uploaded already, materials naming the block)
+        glued as one block; returns its group index. The RIGID contact passes
+        are not built: stepping a pipeline with a block raises."""
+        xy = np.ascontiguousarray(offsets, dtype=np.float32).reshape(-1, 2)
+        group = ctypes.c_int(-1)
+        native.check(self._lib.gs_rigid_glue(self._h, int(firstBody), len(xy), native.ptr(xy), float(centerX),
+                                             float(centerY), ctypes.byref(group)))
+        return group.value
+
+    def projectBlocks(self, contacts: bool) -> None:
+        """RigidBlocks.project (K/RigidBlocks.kt:309-357) for blocks nothing breaks."""
+        native.check(self._lib.gs_rigid_project(self._h, int(bool(contacts))))
+
+    def blockSize(self) -> tuple[int, int, int]:
+        """(group indices in use, tiles in use, member slots)."""
+        out = (ctypes.c_int * 3)()
+        native.check(self._lib.gs_rigid_size(self._h, out))
+        return out[0], out[1], out[2]
+
+    def blockRead(self, what: int, first: int, count: int, dtype=np.int32) -> np.ndarray:
+        """32-bit words of a block buffer (native.GS_RIGID_*)."""
+        out = np.zeros(count, dtype=dtype)
+        assert out.itemsize == 4
+        native.check(self._lib.gs_rigid_read(self._h, int(what), int(first), int(count), native.ptr(out)))
+        return out
+
+    def blockWrite(self, what: int, first: int, words: np.ndarray) -> None:
+        """Writes 32-bit words into the groups or the contact memory."""
+        w = np.ascontiguousarray(words)
+        assert w.itemsize == 4
+        native.check(self._lib.gs_rigid_write(self._h, int(what), int(first), w.size, native.ptr(w)))
+
     # -- render view (bindBodiesForRender, K/GravityPipeline.kt:509-512) -----
     def renderView(self) -> tuple[int, int]:
         """Device addresses of the current bodies (vec4) and pressures (float)

@@ -93,6 +93,9 @@ typedef struct gs_config {
     float trajectory_target_segment_pixels;
     float trajectory_maximum_path_diagonals;
     float trajectory_minimum_dash_pixels;
+    /* Coulomb friction of glued blocks against walls
+     * (ProjectileConfig.blockFriction, K/GravityConfig.kt:773) */
+    float block_friction;
 } gs_config;
 
 typedef struct gs_sim gs_sim;
@@ -238,6 +241,43 @@ int gs_supports_walls(int grid_width, int grid_height);
  * through hipGraphicsGLRegisterBuffer); either destination nullable. */
 int gs_render_view(gs_sim *sim, void **bodies, void **pressures);
 int gs_render_copy(gs_sim *sim, int count, void *bodies, void *pressures);
+
+/* Glued blocks (K/RigidBlocks.kt): bodies that move as one rigid body. A
+ * block's bodies are uploaded first (gs_upload, at centre + offset, their
+ * materials naming the block as GravityPipeline.uploadBodies does); then
+ * gs_rigid_glue is RigidBlocks.add (K/RigidBlocks.kt:280-301, called by
+ * glueBlock, K/GravityPipeline.kt:692-717): bodies first_body ..
+ * first_body + count - 1 at offsets_xy (2 floats each, px) from
+ * (center_x, center_y), glued as block *group. The block's region of the
+ * shape values holds its hull only: the distance field (GlueShape) is read
+ * by the RIGID contact passes alone, which are not built, so gs_step,
+ * gs_debug_resolve_collisions, gs_debug_resolve_liquid and the adjoint fail
+ * with GS_ESTATE once a block is glued. Single-simulation handles only.
+ * gs_debug_sort moves the members with their bodies (RigidBlocks.remap,
+ * :391-399). */
+int gs_rigid_glue(gs_sim *sim, int first_body, int count, const float *offsets_xy, float center_x,
+                  float center_y, int *group);
+/* RigidBlocks.project(bodies, materials, bounded, contacts)
+ * (K/RigidBlocks.kt:309-357; S/rigid_reduce.comp, S/rigid_apply.comp) for
+ * blocks nothing breaks: every block fitted to its members (momentum and
+ * angular momentum kept) and its members snapped onto the fit; contacts != 0
+ * after a contact pass (settles the contact memory and stops the blocks
+ * against the pushes it asked for). Queued on the handle's stream. */
+int gs_rigid_project(gs_sim *sim, int contacts);
+/* RigidBlocks.size (K/RigidBlocks.kt:172-174): out[0] group indices in use,
+ * out[1] tiles in use, out[2] member slots. */
+int gs_rigid_size(gs_sim *sim, int *out);
+/* The block buffers, as T/RigidBlockIntegrationTest.kt:100-108 reads them:
+ * 32-bit words from `first`, `count` of them. Writable: the groups and the
+ * contact memory (which the RIGID contact passes write). Blocks. */
+#define GS_RIGID_GROUPS 0          /* 40 words per group (RigidGroup) */
+#define GS_RIGID_CONTACT_MEMORY 1  /* int impacts, next impacts, states, idle passes, pair contacts: 66816 each */
+#define GS_RIGID_MEMBERS 2         /* int per member slot, -1 for none */
+#define GS_RIGID_REST 3            /* 2 floats per member slot (px) */
+#define GS_RIGID_SHAPES 4          /* float: each block's hull corners from tiles.z */
+#define GS_RIGID_TILE_GROUPS 5     /* int per tile */
+int gs_rigid_read(gs_sim *sim, int what, int first, int count, void *dst);
+int gs_rigid_write(gs_sim *sim, int what, int first, int count, const void *src);
 
 /* Test hooks mirroring the reflection hooks of T/CollisionGpuSupport.kt:68-90:
  * resolveCollisions(count, kickDt, driftDt, writePressure, substep) (one

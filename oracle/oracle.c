@@ -1669,3 +1669,307 @@ double orc_grav64_loss(const orc_params *p, int n, const float *bodies, const fl
     g64_free(&g);
     return n > 0 ? loss / n : 0.0;
 }
+
+/* ==== glued blocks: the rigid projection ================================== */
+
+#define RG_TILE 256
+#define RG_MAXIMUM_GROUPS 256
+#define RG_PARTNERS (RG_MAXIMUM_GROUPS + 5)
+#define RG_KEYS (RG_MAXIMUM_GROUPS * RG_PARTNERS)
+
+/* S/rigid_layout.glsl:4-15 */
+typedef struct rg_group {
+    float frame[4], motion[4], scalars[4];
+    int32_t tiles[4];
+    float shape[4];
+    int32_t grid[4];
+    float bounds[4];
+    int32_t guard[4], support[4];
+    float pose[4];
+} rg_group;
+
+/* Kotlin's Float.compareTo (-0 below +0) and data-class float equality */
+static int rg_before(float a, float b) {
+    if (a < b) return 1;
+    return a == b && signbit(a) && !signbit(b);
+}
+static int rg_same(float a, float b) { return a == b && !signbit(a) == !signbit(b); }
+
+/* K/GlueShape.kt:158-190 */
+int orc_rigid_hull(const float *xs, const float *ys, int n, int *corners) {
+    if (n <= 0) return 0;
+    float minY = ys[0], maxY = ys[0];
+    for (int i = 1; i < n; i++) {
+        if (ys[i] < minY) minY = ys[i];
+        if (ys[i] > maxY) maxY = ys[i];
+    }
+    const int rows = (int)((maxY - minY) / 0.01f) + 1;
+    int *lowest = malloc(sizeof(int) * (size_t)rows), *highest = malloc(sizeof(int) * (size_t)rows);
+    for (int r = 0; r < rows; r++) lowest[r] = highest[r] = -1;
+    for (int i = 0; i < n; i++) {
+        const int r = (int)((ys[i] - minY) / 0.01f);
+        if (lowest[r] < 0 || xs[i] < xs[lowest[r]]) lowest[r] = i;
+        if (highest[r] < 0 || xs[i] > xs[highest[r]]) highest[r] = i;
+    }
+    int *pts = malloc(sizeof(int) * 2 * (size_t)rows);
+    int m = 0;
+    for (int r = 0; r < rows; r++) {
+        if (lowest[r] < 0) continue;
+        const int ends[2] = {lowest[r], highest[r]};
+        for (int e = 0; e < 2; e++) {
+            int seen = 0;
+            for (int k = 0; k < m && !seen; k++)
+                seen = rg_same(xs[pts[k]], xs[ends[e]]) && rg_same(ys[pts[k]], ys[ends[e]]);
+            if (!seen) pts[m++] = ends[e];
+        }
+    }
+    /* stable insertion sort by (x, y) */
+    for (int i = 1; i < m; i++) {
+        const int v = pts[i];
+        int j = i - 1;
+        while (j >= 0 && (rg_before(xs[v], xs[pts[j]]) ||
+                          (rg_same(xs[v], xs[pts[j]]) && rg_before(ys[v], ys[pts[j]])))) {
+            pts[j + 1] = pts[j];
+            j--;
+        }
+        pts[j + 1] = v;
+    }
+    int count = 0;
+    if (m < 3) {
+        for (int k = 0; k < m; k++) corners[count++] = pts[k];
+    } else {
+        /* Andrew's monotone chain, lower then upper, collinear points dropped */
+        int *chain = malloc(sizeof(int) * 2 * (size_t)(m + 1));
+        int len = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            const int start = len;
+            for (int k = 0; k < m; k++) {
+                const int q = pass == 0 ? pts[k] : pts[m - 1 - k];
+                while (len >= start + 2) {
+                    const int o = chain[len - 2], a = chain[len - 1];
+                    const double turn = (double)(xs[a] - xs[o]) * (double)(ys[q] - ys[o]) -
+                                        (double)(ys[a] - ys[o]) * (double)(xs[q] - xs[o]);
+                    if (turn > 0.0) break;
+                    len--;
+                }
+                chain[len++] = q;
+            }
+            len--;
+        }
+        for (int k = 0; k < len; k++) corners[count++] = chain[k];
+        free(chain);
+    }
+    free(pts);
+    free(lowest);
+    free(highest);
+    return count;
+}
+
+/* K/RigidBlocks.kt:718-721, 751-777, 838-840 */
+void orc_rigid_prepare(const float *offsets_xy, int n, float *rest_xy, float *centroid_xy,
+                       double *inertia, float *bounds) {
+    double sx = 0.0, sy = 0.0;
+    for (int i = 0; i < n; i++) {
+        sx += (double)offsets_xy[2 * i];
+        sy += (double)offsets_xy[2 * i + 1];
+    }
+    const float cx = (float)(sx / n), cy = (float)(sy / n);
+    double sum = 0.0;
+    float lx = offsets_xy[0], hx = offsets_xy[0], ly = offsets_xy[1], hy = offsets_xy[1];
+    for (int i = 0; i < n; i++) {
+        const float x = offsets_xy[2 * i] - cx, y = offsets_xy[2 * i + 1] - cy;
+        rest_xy[2 * i] = x;
+        rest_xy[2 * i + 1] = y;
+        const float term = x * x + y * y;
+        sum += (double)term;
+        if (offsets_xy[2 * i] < lx) lx = offsets_xy[2 * i];
+        if (offsets_xy[2 * i] > hx) hx = offsets_xy[2 * i];
+        if (offsets_xy[2 * i + 1] < ly) ly = offsets_xy[2 * i + 1];
+        if (offsets_xy[2 * i + 1] > hy) hy = offsets_xy[2 * i + 1];
+    }
+    centroid_xy[0] = cx;
+    centroid_xy[1] = cy;
+    *inertia = sum;
+    bounds[0] = lx - cx;
+    bounds[1] = hx - cx;
+    bounds[2] = ly - cy;
+    bounds[3] = hy - cy;
+}
+
+/* S/rigid_reduce.comp:60-69: the workgroup's tree, stride 128 down to 1 */
+static void rg_tree(float (*a)[4], float (*b)[4]) {
+    for (int stride = RG_TILE / 2; stride > 0; stride >>= 1)
+        for (int l = 0; l < stride; l++)
+            for (int c = 0; c < 4; c++) {
+                a[l][c] = a[l][c] + a[l + stride][c];
+                b[l][c] = b[l][c] + b[l + stride][c];
+            }
+}
+
+/* S/rigid_reduce.comp:42-58 */
+static void rg_stop_against(float *v, float px, float py, float friction, float substep_dt) {
+    const float distance = sqrtf(px * px + py * py);
+    if (distance == 0.0f) return;
+    const float dx = px / distance, dy = py / distance;
+    const float against = v[0] * dx + v[1] * dy;
+    if (against >= 0.0f) return;
+    const float stop = fminf(-against, distance / substep_dt);
+    v[0] = v[0] + dx * stop;
+    v[1] = v[1] + dy * stop;
+    const float along = v[0] * dx + v[1] * dy;
+    const float sx = v[0] - along * dx, sy = v[1] - along * dy;
+    const float sliding = sqrtf(sx * sx + sy * sy);
+    if (sliding > 0.0f) {
+        const float f = fminf(sliding, friction * stop) / sliding;
+        v[0] = v[0] - sx * f;
+        v[1] = v[1] - sy * f;
+    }
+}
+
+/* S/rigid_reduce.comp:78-100 */
+static void rg_settle(int32_t *memory, int group, int group_count) {
+    int32_t *impacts = memory, *next = memory + RG_KEYS, *states = memory + 2 * RG_KEYS;
+    int32_t *idle = memory + 3 * RG_KEYS, *pairs = memory + 4 * RG_KEYS;
+    const int partners = group_count - group;
+    for (int index = 0; index < partners + 5; index++) {
+        const int partner = index < partners ? group + index : RG_MAXIMUM_GROUPS + index - partners;
+        const int key = group * RG_PARTNERS + partner;
+        if (states[key] == 2) {
+            int speed = impacts[key] & 0xffffff;
+            if (next[key] > speed) speed = next[key];
+            const int age = (impacts[key] >> 24) + 1;
+            impacts[key] = speed == 0 || age > 32 ? 0 : speed | (age << 24);
+            idle[key] = 0;
+        } else if (states[key] == 1) {
+            impacts[key] = 0;
+            idle[key] = 0;
+        } else if (impacts[key] != 0 && ++idle[key] >= 4) {
+            impacts[key] = 0;
+            idle[key] = 0;
+        }
+        next[key] = 0;
+        states[key] = 0;
+        pairs[key] = 0;
+    }
+}
+
+void orc_rigid_project(const orc_rigid_params *p, void *groups_words, int32_t *memory, const float *shapes,
+                       const int32_t *members, const float *rest, const int32_t *tile_groups,
+                       float *bodies) {
+    rg_group *groups = (rg_group *)groups_words;
+    const float W = p->width, H = p->height;
+    float (*partials)[2][4] = calloc((size_t)(p->tile_count > 0 ? p->tile_count : 1), sizeof *partials);
+    float (*sa)[4] = malloc(sizeof(float[4]) * RG_TILE), (*sb)[4] = malloc(sizeof(float[4]) * RG_TILE);
+    /* S/rigid_reduce.comp:203-229: the tile sums */
+    for (int tile = 0; tile < p->tile_count; tile++) {
+        const rg_group *r = &groups[tile_groups[tile]];
+        if (tile < r->tiles[0] || tile >= r->tiles[0] + r->tiles[1]) continue;
+        for (int lane = 0; lane < RG_TILE; lane++) {
+            const int member = tile * RG_TILE + lane;
+            const int b = members[member];
+            for (int c = 0; c < 4; c++) sa[lane][c] = sb[lane][c] = 0.0f;
+            if (b < 0) continue;
+            const float *state = bodies + 4 * (size_t)b;
+            float px = state[0] * W - r->frame[0], py = state[1] * H - r->frame[1];
+            if (!p->bounded) {
+                px -= floorf(px / W + 0.5f) * W;
+                py -= floorf(py / H + 0.5f) * H;
+            }
+            const float vx = state[2] * W, vy = state[3] * H;
+            const float ox = rest[2 * member], oy = rest[2 * member + 1];
+            const float rx = r->motion[2], ry = r->motion[3];
+            const float ax = rx * ox - ry * oy, ay = ry * ox + rx * oy;
+            sa[lane][0] = px; sa[lane][1] = py; sa[lane][2] = vx; sa[lane][3] = vy;
+            sb[lane][0] = ox * py - oy * px;
+            sb[lane][1] = ox * px + oy * py;
+            sb[lane][2] = ax * vy - ay * vx;
+        }
+        rg_tree(sa, sb);
+        memcpy(partials[tile][0], sa[0], sizeof(float[4]));
+        memcpy(partials[tile][1], sb[0], sizeof(float[4]));
+    }
+    /* S/rigid_reduce.comp:102-201: the fit of every block */
+    for (int g = 0; g < p->group_count; g++) {
+        rg_group *r = &groups[g];
+        if (r->tiles[1] == 0) continue;
+        for (int lane = 0; lane < RG_TILE; lane++) {
+            for (int c = 0; c < 4; c++) sa[lane][c] = sb[lane][c] = 0.0f;
+            for (int tile = r->tiles[0] + lane; tile < r->tiles[0] + r->tiles[1]; tile += RG_TILE)
+                for (int c = 0; c < 4; c++) {
+                    sa[lane][c] = sa[lane][c] + partials[tile][0][c];
+                    sb[lane][c] = sb[lane][c] + partials[tile][1][c];
+                }
+        }
+        rg_tree(sa, sb);
+        if (p->contacts) rg_settle(memory, g, p->group_count);
+        const float mass = r->scalars[2];
+        if (mass <= 0.0f) continue;
+        float cx = sa[0][0] / mass, cy = sa[0][1] / mass;
+        float v[2] = {sa[0][2] / mass, sa[0][3] / mass};
+        const float fx = sb[0][1], fy = sb[0][0];
+        const float fit = sqrtf(fx * fx + fy * fy);
+        const float rx = fit > 0.0f ? fx / fit : r->motion[2];
+        const float ry = fit > 0.0f ? fy / fit : r->motion[3];
+        const float spin = r->scalars[1] > 0.0f ? sb[0][2] / r->scalars[1] : 0.0f;
+        cx += r->frame[0];
+        cy += r->frame[1];
+        cx += (float)(r->guard[0] - r->guard[2]) / 65536.0f;
+        cy += (float)(r->guard[1] - r->guard[3]) / 65536.0f;
+        const float sx = (float)(r->support[0] - r->support[2]) / 65536.0f;
+        const float sy = (float)(r->support[1] - r->support[3]) / 65536.0f;
+        for (int c = 0; c < 4; c++) r->guard[c] = r->support[c] = 0;
+        cx += sx;
+        cy += sy;
+        if (p->contacts) rg_stop_against(v, sx, sy, 0.0f, p->substep_dt);
+        if (!p->bounded) {
+            cx -= floorf(cx / W) * W;
+            cy -= floorf(cy / H) * H;
+        } else {
+            float lx = 1e30f, ly = 1e30f, hx = -1e30f, hy = -1e30f;
+            for (int k = 0; k < r->tiles[3]; k++) {
+                const float ox = shapes[r->tiles[2] + 2 * k], oy = shapes[r->tiles[2] + 2 * k + 1];
+                const float ax = rx * ox - ry * oy, ay = ry * ox + rx * oy;
+                lx = fminf(lx, ax); ly = fminf(ly, ay);
+                hx = fmaxf(hx, ax); hy = fmaxf(hy, ay);
+            }
+            const float rad = p->body_radius;
+            float shx = fmaxf(rad - (cx + lx), 0.0f) - fmaxf(cx + hx + rad - W, 0.0f);
+            float shy = fmaxf(rad - (cy + ly), 0.0f) - fmaxf(cy + hy + rad - H, 0.0f);
+            shx *= hx - lx + 2.0f * rad < W ? 1.0f : 0.0f;
+            shy *= hy - ly + 2.0f * rad < H ? 1.0f : 0.0f;
+            cx += shx;
+            cy += shy;
+            if (p->contacts) rg_stop_against(v, shx, shy, p->block_friction, p->substep_dt);
+        }
+        r->frame[0] = r->frame[2] = cx;
+        r->frame[1] = r->frame[3] = cy;
+        r->motion[0] = v[0]; r->motion[1] = v[1]; r->motion[2] = rx; r->motion[3] = ry;
+        r->scalars[0] = spin;
+    }
+    /* S/rigid_apply.comp:48-69: every member onto its block's motion */
+    for (int tile = 0; tile < p->tile_count; tile++) {
+        const rg_group *r = &groups[tile_groups[tile]];
+        for (int lane = 0; lane < RG_TILE; lane++) {
+            const int member = tile * RG_TILE + lane;
+            const int b = members[member];
+            if (b < 0) continue;
+            const float ox = rest[2 * member], oy = rest[2 * member + 1];
+            const float rx = r->motion[2], ry = r->motion[3];
+            const float ax = rx * ox - ry * oy, ay = ry * ox + rx * oy;
+            float px = (r->frame[2] + ax) / W, py = (r->frame[3] + ay) / H;
+            const float vx = r->motion[0] + r->scalars[0] * -ay, vy = r->motion[1] + r->scalars[0] * ax;
+            if (p->bounded) {
+                px = fminf(fmaxf(px, 0.0f), 0.99999994f);
+                py = fminf(fmaxf(py, 0.0f), 0.99999994f);
+            } else {
+                px = px - floorf(px);
+                py = py - floorf(py);
+            }
+            float *out = bodies + 4 * (size_t)b;
+            out[0] = px; out[1] = py; out[2] = vx / W; out[3] = vy / H;
+        }
+    }
+    free(partials);
+    free(sa);
+    free(sb);
+}

@@ -153,6 +153,36 @@ double orc_grav64_loss(const orc_params *p, int n, const float *bodies,
                        const float *mats, int steps, const float *targets,
                        const double *dv, double *final_xy);
 
+/* ---- glued blocks: the rigid projection ---------------------------------- */
+/* GlueShape.hullIndices (K/GlueShape.kt:158-190): corner indices of the
+ * convex hull of the points, counter-clockwise from the lowest x; returns
+ * their number (at most n). */
+int orc_rigid_hull(const float *xs, const float *ys, int n, int *corners);
+/* RigidBlocks.prepare and upload (K/RigidBlocks.kt:751-857) of a block of n
+ * members at offsets_xy (px): rest_xy the offsets about their centroid,
+ * centroid_xy that centroid, *inertia the sum of |r0|^2 (summed in double),
+ * bounds the rest-frame extent (min x, max x, min y, max y). */
+void orc_rigid_prepare(const float *offsets_xy, int n, float *rest_xy, float *centroid_xy,
+                       double *inertia, float *bounds);
+/* RigidBlocks.project (K/RigidBlocks.kt:309-357; S/rigid_reduce.comp,
+ * S/rigid_apply.comp) for blocks nothing breaks and that are awake. groups:
+ * 40 words per group (S/rigid_layout.glsl RigidGroup); memory: the five
+ * contact planes of 66816 ints; members, rest (2 floats) and the tile groups
+ * per member slot and tile; bodies vec4 (normalised), updated in place. */
+typedef struct orc_rigid_params {
+    float width, height;      /* viewport, px */
+    int bounded;
+    int contacts;             /* after a contact pass */
+    int group_count;          /* group indices in use */
+    int tile_count;           /* tiles in use */
+    float substep_dt;         /* s */
+    float body_radius;        /* px */
+    float block_friction;
+} orc_rigid_params;
+void orc_rigid_project(const orc_rigid_params *p, void *groups, int32_t *memory, const float *shapes,
+                       const int32_t *members, const float *rest, const int32_t *tile_groups,
+                       float *bodies);
+
 /* ---- d: liquid ---------------------------------------------------------- */
 void orc_liquid_constants(float diameter, float radius_diameters, float *h,
                           float *poly6, float *spiky, float *rest_density,

@@ -18,6 +18,13 @@ class OrcTrajectoryParams(ctypes.Structure):
                 ("path_diagonals", ctypes.c_float), ("minimum_dash_px", ctypes.c_float)]
 
 
+class OrcRigidParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_float), ("height", ctypes.c_float), ("bounded", ctypes.c_int),
+                ("contacts", ctypes.c_int), ("group_count", ctypes.c_int), ("tile_count", ctypes.c_int),
+                ("substep_dt", ctypes.c_float), ("body_radius", ctypes.c_float),
+                ("block_friction", ctypes.c_float)]
+
+
 class OrcParams(ctypes.Structure):
     _fields_ = [
         ("width", ctypes.c_int), ("height", ctypes.c_int), ("bounded", ctypes.c_int),
@@ -85,6 +92,9 @@ def load():
             "orc_sim_initialize_acceleration": (None, [P]),
             "orc_sim_step": (None, [P]),
             "orc_sim_wake": (None, [P]),
+            "orc_rigid_hull": (I, [P, P, I, P]),
+            "orc_rigid_prepare": (None, [P, I, P, P, ctypes.POINTER(ctypes.c_double), P]),
+            "orc_rigid_project": (None, [ctypes.POINTER(OrcRigidParams), P, P, P, P, P, P, P]),
             "orc_grav64_loss": (ctypes.c_double, [P, I, P, P, I, P, P, P]),
             "orc_sim_sleep_stamp": (U, [P]),
             "orc_sim_body_awake": (None, [P, P]),
@@ -315,3 +325,48 @@ class Sim:
         load().orc_sim_trajectory(self._h, float(origin[0]), float(origin[1]), float(velocity[0]),
                                   float(velocity[1]), ctypes.byref(t), _p(out))
         return out
+
+
+# ---- glued blocks: the rigid projection ------------------------------------
+RIGID_TILE = 256
+RIGID_MAXIMUM_GROUPS = 256
+RIGID_CONTACT_PARTNERS = RIGID_MAXIMUM_GROUPS + 5
+RIGID_CONTACT_KEYS = RIGID_MAXIMUM_GROUPS * RIGID_CONTACT_PARTNERS
+
+
+def rigid_hull(xs, ys):
+    """GlueShape.hullIndices: corner indices, counter-clockwise."""
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.float32)
+    out = np.zeros(max(len(xs), 1), np.int32)
+    n = load().orc_rigid_hull(_p(xs), _p(ys), len(xs), _p(out))
+    return out[:n].copy()
+
+
+def rigid_prepare(offsets):
+    """RigidBlocks.prepare: (rest offsets (n, 2), centroid (2,), inertia, bounds (4,))."""
+    offsets = np.ascontiguousarray(offsets, np.float32).reshape(-1, 2)
+    rest = np.zeros_like(offsets)
+    centroid = np.zeros(2, np.float32)
+    bounds = np.zeros(4, np.float32)
+    inertia = ctypes.c_double()
+    load().orc_rigid_prepare(_p(offsets), len(offsets), _p(rest), _p(centroid), ctypes.byref(inertia),
+                             _p(bounds))
+    return rest, centroid, inertia.value, bounds
+
+
+def rigid_project(width, height, bounded, contacts, group_count, tile_count, groups, memory, shapes,
+                  members, rest, tile_groups, bodies, substep_dt=1.0 / 120.0 / 16, body_radius=0.56419,
+                  block_friction=0.5):
+    """RigidBlocks.project for blocks nothing breaks. groups (uint32/int32/float32 words, 40 per group),
+    memory (int32) and bodies (float32, 4 per body) are updated in place."""
+    p = OrcRigidParams(float(width), float(height), int(bounded), int(contacts), int(group_count),
+                       int(tile_count), float(substep_dt), float(body_radius), float(block_friction))
+    for a, t in ((groups, None), (memory, np.int32), (shapes, np.float32), (members, np.int32),
+                 (rest, np.float32), (tile_groups, np.int32), (bodies, np.float32)):
+        assert a.flags.c_contiguous and (t is None or a.dtype == t)
+    assert groups.dtype.itemsize == 4 and groups.size >= 40 * group_count
+    assert memory.size >= 5 * RIGID_CONTACT_KEYS and members.size >= RIGID_TILE * tile_count
+    assert rest.size >= 2 * RIGID_TILE * tile_count and tile_groups.size >= tile_count
+    load().orc_rigid_project(ctypes.byref(p), _p(groups), _p(memory), _p(shapes), _p(members), _p(rest),
+                             _p(tile_groups), _p(bodies))
