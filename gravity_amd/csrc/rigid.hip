@@ -520,6 +520,10 @@ void Engine::rigid_access(int what, long long first, long long count, void* host
     if (write && what != GS_RIGID_GROUPS && what != GS_RIGID_CONTACT_MEMORY)
         throw ArgumentError("only the groups and the contact memory are writable");
     if (first + count > limit) throw ArgumentError("range out of bounds");
+    // the tile and hull ranges of a block index the other buffers: the library's own
+    if (write && what == GS_RIGID_GROUPS)
+        for (long long w = first; w < first + count; w++)
+            if (w % 40 >= 12 && w % 40 < 16) throw ArgumentError("a block's tile and hull words are not writable");
     synchronize();
     if (write)
         check(hipMemcpy(base + first * unit, host, count * unit, hipMemcpyHostToDevice), "rigid write");

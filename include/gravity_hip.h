@@ -268,8 +268,9 @@ int gs_rigid_project(gs_sim *sim, int contacts);
  * out[1] tiles in use, out[2] member slots. */
 int gs_rigid_size(gs_sim *sim, int *out);
 /* The block buffers, as T/RigidBlockIntegrationTest.kt:100-108 reads them:
- * 32-bit words from `first`, `count` of them. Writable: the groups and the
- * contact memory (which the RIGID contact passes write). Blocks. */
+ * 32-bit words from `first`, `count` of them. Writable: the groups (but
+ * for their tile and hull words, 12-15 of each) and the contact memory,
+ * which the RIGID contact passes write. Blocks. */
 #define GS_RIGID_GROUPS 0          /* 40 words per group (RigidGroup) */
 #define GS_RIGID_CONTACT_MEMORY 1  /* int impacts, next impacts, states, idle passes, pair contacts: 66816 each */
 #define GS_RIGID_MEMBERS 2         /* int per member slot, -1 for none */

@@ -156,3 +156,5 @@ def test_a_pipeline_with_blocks_refuses_to_step():
             pipe.glueBlock(len(state) + 10_000, hex_lattice(10), 100.0, 100.0)
         with pytest.raises(ValueError):
             pipe.blockWrite(native.GS_RIGID_MEMBERS, 0, np.zeros(4, np.int32))
+        with pytest.raises(ValueError):
+            pipe.blockWrite(native.GS_RIGID_GROUPS, 10, np.zeros(4, np.int32))
