@@ -196,6 +196,17 @@ def test_walls_hold_the_hull_and_stop_the_block_in_closed_form():
             assert f[4] == pytest.approx(0.0, abs=1e-3) and f[5] == pytest.approx(15.0, abs=1e-3)
         else:
             assert f[4] == pytest.approx(50.0, abs=1e-3) and f[5] == pytest.approx(40.0, abs=1e-3)
+    # friction stops the sliding at most: vy 10 under a 25 friction budget ends at rest
+    lay, g, rest = _one_block(n, (cx, 500.0), 0.0, True, frame_offset=(0.0, 0.0))
+    bodies = place(rest, (cx, 500.0), 0.0, (50.0, 10.0), 0.0, W, H, True)
+    lay.project(bodies, W, H, True, True, SUB, RADIUS, block_friction=0.5)
+    assert lay.floats(g)[5] == pytest.approx(0.0, abs=1e-3)
+    # a block wider than the screen is not moved along that axis
+    narrow = 2 * reach - 4.0
+    lay, g, rest = _one_block(n, (narrow / 2 + 1.0, 500.0), 0.0, True, frame_offset=(0.0, 0.0))
+    bodies = place(rest, (narrow / 2 + 1.0, 500.0), 0.0, (0.0, 0.0), 0.0, narrow, H, True)
+    lay.project(bodies, narrow, H, True, True, SUB, RADIUS)
+    assert lay.floats(g)[0] == pytest.approx(narrow / 2 + 1.0, abs=2e-3)
     # a push larger than the substep allows: the stop is the push over a substep
     lay, g, rest = _one_block(n, (cx, 500.0), 0.0, True, frame_offset=(0.0, 0.0))
     bodies = place(rest, (cx, 500.0), 0.0, (9000.0, 0.0), 0.0, W, H, True)
@@ -239,6 +250,7 @@ def test_contact_memory_settles_pair_by_pair():
         # (group, partner): (impact, next, state, idle) -> (impact, idle)
         (0, 1): ((100 | 3 << 24), 150, 2, 2, (150 | 4 << 24), 0),     # pushing: faster, one pass older
         (0, 2): ((100 | 32 << 24), 50, 2, 0, 0, 0),                   # pushed 32 passes: over
+        (0, 0): ((100 | 31 << 24), 20, 2, 0, (100 | 32 << 24), 0),    # the 32nd pass still pushes
         (1, 2): (0, 0, 2, 0, 0, 0),                                   # pushing with no speed: over
         (0, wall): ((80 | 1 << 24), 0, 1, 3, 0, 0),                   # touching without pushing: over
         (1, wall): ((80 | 1 << 24), 0, 0, 2, (80 | 1 << 24), 3),      # apart, third pass: kept
@@ -248,14 +260,17 @@ def test_contact_memory_settles_pair_by_pair():
     for (a, b), (i0, n0, s0, d0, _, _) in cases.items():
         k = key(a, b)
         imp[k], nxt[k], st[k], idle[k], pairs[k] = i0, n0, s0, d0, 7
-    # a pair owned by no block in use (group 3): untouched
+    # pairs owned by no block in use (group 3), or with one: untouched
     k3 = key(3, wall)
     imp[k3], st[k3], pairs[k3] = 9, 2, 4
+    k13 = key(1, 3)
+    imp[k13], st[k13], pairs[k13] = 9, 2, 4
     lay.project(bodies, W, H, True, True, SUB, RADIUS)
     for (a, b), (_, _, _, _, i1, d1) in cases.items():
         k = key(a, b)
         assert (imp[k], idle[k], nxt[k], st[k], pairs[k]) == (i1, d1, 0, 0, 0), (a, b)
     assert (imp[k3], st[k3], pairs[k3]) == (9, 2, 4)
+    assert (imp[k13], st[k13], pairs[k13]) == (9, 2, 4)
     # a projection after no contact pass leaves the memory alone
     imp[key(0, 1)], st[key(0, 1)] = 5, 2
     lay.project(bodies, W, H, True, False, SUB, RADIUS)
