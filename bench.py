@@ -219,6 +219,56 @@ def p3m_only_leg(steps=100, warmup=10):
             "unit": "body-steps/s", "finite": finite}
 
 
+def rigid_leg(projections=200, warmup=10):
+    """Glued blocks: 400,000 bodies in 64 blocks of 6,250 members (hexagonal
+    packings at the contact diameter) on a 2880x1440 walled viewport, each
+    moving and spinning; `projections` rigid projections after a contact pass
+    (tile sums, fit, snap), timed back to back. Algorithmic bytes: 28 B per
+    member slot read by the tile sums (body 16, member 4, rest offset 8) and
+    28 by the snap (member and rest offset read, body written)."""
+    from gravity_amd import CollisionConfig, GravityConfig, GravityFieldConfig, GravityPipeline, ParticleConfig
+    blocks, per, W, H, d = 64, 6250, 2880, 1440, 1.12838
+    n = blocks * per
+    cfg = GravityConfig(particles=ParticleConfig(capacity=n, initialBodies=n, seed=1),
+                        gravity=GravityFieldConfig(gridWidth=1024, gridHeight=512),
+                        collisions=CollisionConfig(contactDiameterPixels=d))
+    rng = np.random.default_rng(1)
+    cols = int(np.ceil(np.sqrt(per * 1.2)))
+    k = np.arange(per)
+    row, col = k // cols, k % cols
+    offsets = np.stack([(col + 0.5 * (row % 2)) * d, row * d * np.sqrt(3.0) / 2], 1)
+    offsets = (offsets - offsets.mean(axis=0)).astype(np.float32)
+    state = np.zeros((n, 4), np.float32)
+    mats = np.zeros((n, 4), np.float32)
+    centres = [((i % 8 + 0.5) * W / 8, (i // 8 + 0.5) * H / 8) for i in range(blocks)]
+    for i, (cx, cy) in enumerate(centres):
+        v = rng.uniform(-40, 40, 2)
+        spin = rng.uniform(-0.5, 0.5)
+        pos = np.array([cx, cy]) + offsets
+        vel = v + spin * np.stack([-offsets[:, 1], offsets[:, 0]], 1)
+        state[i * per:(i + 1) * per] = np.concatenate([pos / [W, H], vel / [W, H]], 1)
+        mats[i * per:(i + 1) * per] = (0.5, 0.5, 1.0, float(i))
+    with GravityPipeline(cfg, W, H, scenarioState=state, bounded=True) as pipe:
+        pipe.upload(state, mats)
+        for i, (cx, cy) in enumerate(centres):
+            pipe.glueBlock(i * per, offsets, cx, cy)
+        for _ in range(warmup):
+            pipe.projectBlocks(True)
+        pipe.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(projections):
+            pipe.projectBlocks(True)
+        pipe.synchronize()
+        t = (time.perf_counter() - t0) / projections
+        finite = bool(np.isfinite(pipe.bodies(n)).all())
+        _, tiles, _ = pipe.blockSize()
+    alg = 56 * tiles * 256
+    return {"workload": "rigid-400k: 64 glued blocks of 6,250 members, walls, 2880x1440",
+            "bodies": n, "projections": projections, "us_per_projection": t * 1e6,
+            "value": n / t, "unit": "member-projections/s", "algorithmic_bytes": alg,
+            "achieved_GBs": alg / t / 1e9, "launches_per_projection": 3, "finite": finite}
+
+
 def force_error_leg(sample=1000):
     """FP64 direct-summation force error of the HIP path and of the oracle on
     the galaxy workload's initial state (walls: the isolated 2D sum
@@ -393,9 +443,10 @@ def main():
     ap.add_argument("--no-liquid", action="store_true", help="skip the liquid-and-sand leg")
     ap.add_argument("--no-p3m", action="store_true", help="skip the P3M-only leg")
     ap.add_argument("--no-force-error", action="store_true", help="skip the FP64 force-error leg")
+    ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-projection leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel eagerly instead of through captured graphs")
-    ap.add_argument("--leg", choices=("batched", "batch", "adjoint", "liquid", "p3m", "force"),
+    ap.add_argument("--leg", choices=("batched", "batch", "adjoint", "liquid", "p3m", "force", "rigid"),
                     help="run only this leg and print its record (batch: the batched ensemble "
                          "without the plain loop), for kernel profiles of one workload")
     ap.add_argument("--leg-steps", type=int, default=0,
@@ -427,7 +478,8 @@ def main():
                 "adjoint": adjoint_leg,
                 "liquid": (lambda: liquid_leg(k, 2)) if k else liquid_leg,
                 "p3m": (lambda: p3m_only_leg(k, 2)) if k else p3m_only_leg,
-                "force": force_error_leg}
+                "force": force_error_leg,
+                "rigid": (lambda: rigid_leg(k, 2)) if k else rigid_leg}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
 
@@ -496,6 +548,11 @@ def main():
         print("[bench] liquid and sand: 40 steps", file=sys.stderr, flush=True)
         liquid = liquid_leg()
 
+    rigid = None
+    if not args.no_rigid:
+        print("[bench] rigid projection: 64 blocks, 400k members", file=sys.stderr, flush=True)
+        rigid = rigid_leg()
+
     cpu = None
     if not args.no_cpu:
         print(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps", file=sys.stderr, flush=True)
@@ -533,6 +590,7 @@ def main():
         "liquid": liquid,
         "p3m_only": p3m,
         "force_error": force,
+        "rigid": rigid,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

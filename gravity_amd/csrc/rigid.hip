@@ -73,13 +73,16 @@ __global__ void __launch_bounds__(RIGID_TILE) k_rigid_reduce(
     const RigidGroup* __restrict__ groups, float4* __restrict__ partials, RigidArgs a) {
     __shared__ float4 sa[RIGID_TILE], sb[RIGID_TILE];
     const int tile = blockIdx.x, lane = threadIdx.x;
+    // the slot's member and rest offset load with the tile's block
+    const int member = tile * RIGID_TILE + lane;
+    const int b = members[member];
+    const float2 o = rest[member];
     const int group = tileGroups[tile];
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const RigidGroup& r = groups[group];
     const int4 tiles = r.tiles;
     // a tile no block holds
     if (tile < tiles.x || tile >= tiles.x + tiles.y) return;
-    const int member = tile * RIGID_TILE + lane;
-    const int b = members[member];
     float4 va = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vb = va;
     if (b >= 0) {
         const float4 state = body[b];
@@ -90,7 +93,6 @@ __global__ void __launch_bounds__(RIGID_TILE) k_rigid_reduce(
             py -= floorf(py / a.height + 0.5f) * a.height;
         }
         const float vx = state.z * a.width, vy = state.w * a.height;
-        const float2 o = rest[member];
         const float rx = r.motion.z, ry = r.motion.w;
         const float armx = rx * o.x - ry * o.y, army = ry * o.x + rx * o.y;
         va = make_float4(px, py, vx, vy);
@@ -253,9 +255,11 @@ __global__ void __launch_bounds__(RIGID_TILE) k_rigid_apply(
     const int tile = blockIdx.x;
     const int member = tile * RIGID_TILE + (int)threadIdx.x;
     const int b = members[member];
-    if (b < 0) return;
-    const RigidGroup& r = groups[tileGroups[tile]];
     const float2 o = rest[member];
+    const int group = tileGroups[tile];
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (b < 0) return;
+    const RigidGroup& r = groups[group];
     const float rx = r.motion.z, ry = r.motion.w;
     const float armx = rx * o.x - ry * o.y, army = ry * o.x + rx * o.y;
     float px = (r.frame.z + armx) / a.width, py = (r.frame.w + army) / a.height;
@@ -269,6 +273,15 @@ __global__ void __launch_bounds__(RIGID_TILE) k_rigid_apply(
         py = fractf(py);
     }
     body[b] = make_float4(px, py, vx / a.width, vy / a.height);
+}
+
+// S/rigid_adopt.comp:66-75: block index `group` starts afresh, its contact
+// memory with every partner forgotten.
+__global__ void __launch_bounds__(RIGID_TILE) k_rigid_adopt(int* __restrict__ memory, int group) {
+    const int partner = blockIdx.x * RIGID_TILE + (int)threadIdx.x;
+    if (partner >= RIGID_CONTACT_PARTNERS) return;
+    const int key = min(group, partner) * RIGID_CONTACT_PARTNERS + max(group, partner);
+    for (int plane = 0; plane < 5; plane++) memory[plane * RIGID_CONTACT_KEYS + key] = 0;
 }
 
 // S/rigid_remap.comp: members follow the bodies spatial sorting renamed.
@@ -384,6 +397,8 @@ int Engine::rigid_glue(int firstBody, int count, const float* offsets, float cen
     if (count <= 0 || firstBody < 0 || (long long)firstBody + count > h.cap)
         throw ArgumentError("block bodies must lie within capacity");
     rigid_allocate();
+    // the uploads below are blocking copies beside the handle's stream: let its work end first
+    synchronize();
     const int group = rigid_.groupCount;   // nextGroup: no block is ever freed here
     const int tiles = (count + RIGID_TILE - 1) / RIGID_TILE;
     // canAdd (K/RigidBlocks.kt:276-278)
@@ -455,15 +470,11 @@ int Engine::rigid_glue(int firstBody, int count, const float* offsets, float cen
         check(hipMemcpy(rigid_.shapes + rigid_.valueTop, hull.data(), sizeof(float) * values,
                         hipMemcpyHostToDevice), "rigid hull");
     check(hipMemcpy(rigid_.groups + group, &g, sizeof(RigidGroup), hipMemcpyHostToDevice), "rigid group");
-    // adoptGroup (S/rigid_adopt.comp:66-75): the contact memory of the index
+    // adoptGroup (K/RigidBlocks.kt:859-870): the contact memory of the index
     // with every partner starts empty
-    std::vector<int> zero(1, 0);
-    for (int partner = 0; partner < RIGID_CONTACT_PARTNERS; partner++) {
-        const size_t key = (size_t)std::min(group, partner) * RIGID_CONTACT_PARTNERS + std::max(group, partner);
-        for (int plane = 0; plane < 5; plane++)
-            check(hipMemcpy(rigid_.memory + plane * (size_t)RIGID_CONTACT_KEYS + key, zero.data(),
-                            sizeof(int), hipMemcpyHostToDevice), "rigid contact memory");
-    }
+    launch(k_rigid_adopt, dim3((RIGID_CONTACT_PARTNERS + RIGID_TILE - 1) / RIGID_TILE), dim3(RIGID_TILE), 0,
+           st_, rigid_.memory, group);
+    check(hipStreamSynchronize(st_), "rigid adopt");
     rigid_.groupCount = group + 1;
     rigid_.tileCount = firstTile + tiles;
     rigid_.valueTop += values;
