@@ -452,6 +452,27 @@ int gs_rigid_glue(gs_sim* sim, int first_body, int count, const float* offsets_x
     });
 }
 
+int gs_spawn_block(gs_sim* sim, int first_body, float min_x, float min_y, float max_x, float max_y,
+                   float density, float velocity_x, float velocity_y, float spin, int* spawned) {
+    SIM_OR_FAIL
+    return guarded([&] {
+        const int k = e.spawn_block(first_body, min_x, min_y, max_x, max_y, density, velocity_x, velocity_y, spin);
+        if (spawned) *spawned = k;
+    });
+}
+
+int gs_block_offsets(float contact_diameter_px, float width_px, float height_px, float* xy, int capacity,
+                     int* count) {
+    return guarded([&] {
+        std::vector<float> offsets;
+        gs::block_offsets(contact_diameter_px, width_px, height_px, offsets);
+        const int n = (int)(offsets.size() / 2);
+        if (xy)
+            for (int i = 0; i < 2 * std::min(n, std::max(capacity, 0)); i++) xy[i] = offsets[i];
+        if (count) *count = n;
+    });
+}
+
 int gs_rigid_project(gs_sim* sim, int contacts) {
     SIM_OR_FAIL
     return guarded([&] { e.rigid_project(contacts != 0); });

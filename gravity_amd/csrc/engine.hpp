@@ -52,6 +52,8 @@ struct HostSim {
 // (GravityPipeline.supportsWalls, K/GravityPipeline.kt:1628), host only.
 void projectile_offsets(float contactDiameter, float projectileDiameter, std::vector<float>& xy);
 float projectile_diameter_for(float contactDiameter, int bodies);
+// RectangularBlock.offsets (K/RectangularBlock.kt:10-77), host only
+void block_offsets(float diameter, float width, float height, std::vector<float>& xy);
 bool supports_walls(int gridWidth, int gridHeight);
 
 // Launch shapes of the step that the autotuner chooses among; no choice
@@ -142,6 +144,9 @@ public:
     // distance field (K/RigidBlocks.kt:280-301), project for blocks nothing
     // breaks (:309-357), the size, and access to the block buffers.
     int rigid_glue(int firstBody, int count, const float* offsets, float centerX, float centerY);
+    // spawnBlock without a mask (K/GravityPipeline.kt:647-717): bodies written, 0 when they do not fit
+    int spawn_block(int firstBody, float minX, float minY, float maxX, float maxY, float density,
+                    float velocityX, float velocityY, float spin);
     void rigid_project(bool contacts);
     void rigid_size(int* out) const;
     void rigid_access(int what, long long first, long long count, void* host, bool write);

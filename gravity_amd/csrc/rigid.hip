@@ -353,6 +353,32 @@ std::vector<int> hull_indices(const std::vector<float>& xs, const std::vector<fl
 
 }  // namespace
 
+// RectangularBlock.offsets (K/RectangularBlock.kt:10-77): a rectangle filled
+// with hexagonally packed bodies about its centre, row by row, each row from
+// left to right, every body inside.
+void block_offsets(float diameter, float width, float height, std::vector<float>& xy) {
+    xy.clear();
+    if (!(diameter > 0.0f)) throw ArgumentError("contact diameter must be positive");
+    constexpr float EPSILON = 1e-4f;
+    const float halfX = 0.5f * (width - diameter), halfY = 0.5f * (height - diameter);
+    if (halfX < 0.0f || halfY < 0.0f) return;
+    const float rowStep = diameter * (float)(std::sqrt(3.0) * 0.5);
+    const int rows = (int)std::floor(2.0f * halfY / rowStep + EPSILON) + 1;
+    const int evenColumns = (int)std::floor(2.0f * halfX / diameter + EPSILON) + 1;
+    const float oddSpan = 2.0f * halfX - 0.5f * diameter;
+    const int oddColumns = oddSpan < -EPSILON * diameter ? 0 : (int)std::floor(oddSpan / diameter + EPSILON) + 1;
+    for (int row = 0; row < rows; row++) {
+        const bool odd = (row & 1) == 1;
+        const int columns = odd ? oddColumns : evenColumns;
+        const float startX = -halfX + (odd ? 0.5f * diameter : 0.0f);
+        const float y = -halfY + (float)row * rowStep;
+        for (int column = 0; column < columns; column++) {
+            xy.push_back(startX + (float)column * diameter);
+            xy.push_back(y);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Engine: the blocks of simulation 0
 
@@ -479,6 +505,60 @@ int Engine::rigid_glue(int firstBody, int count, const float* offsets, float cen
     rigid_.tileCount = firstTile + tiles;
     rigid_.valueTop += values;
     return group;
+}
+
+int Engine::spawn_block(int firstBody, float minX, float minY, float maxX, float maxY, float density,
+                        float velocityX, float velocityY, float spin) {
+    // K/GravityPipeline.kt:647-672 (no mask), glueBlock :692-717, uploadBodies :733-784
+    if (sims_.size() != 1) throw ArgumentError("glued blocks need a pipeline of one simulation");
+    const HostSim& h = sims_[0];
+    if (!(density >= 0.05f && density <= 10.0f)) throw ArgumentError("block density must be in 0.05..10.0");
+    const float diameter = h.cfg.contact_diameter_pixels;
+    std::vector<float> lattice;
+    block_offsets(diameter, maxX - minX, maxY - minY, lattice);
+    const int count = (int)(lattice.size() / 2);
+    if (count == 0 || !(firstBody >= 0 && firstBody <= h.cap && count <= h.cap - firstBody)) return 0;
+    const float centerX = 0.5f * (minX + maxX), centerY = 0.5f * (minY + maxY);
+    // the rigid fit needs offsets about the centre of mass
+    double sx = 0.0, sy = 0.0;
+    for (int i = 0; i < count; i++) {
+        sx += (double)lattice[2 * i];
+        sy += (double)lattice[2 * i + 1];
+    }
+    const float meanX = (float)sx / (float)count, meanY = (float)sy / (float)count;
+    std::vector<float> offsets(lattice.size());
+    for (int i = 0; i < count; i++) {
+        offsets[2 * i] = lattice[2 * i] - meanX;
+        offsets[2 * i + 1] = lattice[2 * i + 1] - meanY;
+    }
+    // canAdd (K/RigidBlocks.kt:276-278), the hull standing for the shape's values
+    rigid_allocate();
+    const int tiles = (count + RIGID_TILE - 1) / RIGID_TILE;
+    if (rigid_.groupCount >= RIGID_MAXIMUM_GROUPS || rigid_.tileCount + tiles > rigid_.tileCap ||
+        rigid_.valueTop + 2LL * count > rigid_.valueCap)
+        return 0;
+    const int group = rigid_.groupCount;
+    const float cx = centerX + meanX, cy = centerY + meanY;
+    const float W = (float)h.cfg.viewport_width, H = (float)h.cfg.viewport_height;
+    auto place = [&](float v, float extent, int size) {
+        if (h.bounded) return std::min(std::max(v, 0.0f), extent * MAXIMUM_UNIT);
+        return v - std::floor(v / (float)size) * (float)size;
+    };
+    std::vector<float> bodies((size_t)4 * count), mats((size_t)4 * count);
+    for (int i = 0; i < count; i++) {
+        const float ox = offsets[2 * i], oy = offsets[2 * i + 1];
+        bodies[4 * i] = place(cx + ox, W, h.cfg.viewport_width) / W;
+        bodies[4 * i + 1] = place(cy + oy, H, h.cfg.viewport_height) / H;
+        bodies[4 * i + 2] = (velocityX - spin * oy) / W;
+        bodies[4 * i + 3] = (velocityY + spin * ox) / H;
+        mats[4 * i] = density;
+        mats[4 * i + 1] = h.cfg.block_friction;
+        mats[4 * i + 2] = 1.0f;
+        mats[4 * i + 3] = (float)group;
+    }
+    upload(0, firstBody, count, bodies.data(), mats.data());
+    rigid_glue(firstBody, count, offsets.data(), cx, cy);
+    return count;
 }
 
 void Engine::rigid_project(bool contacts) {

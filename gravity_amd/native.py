@@ -138,6 +138,8 @@ SIGNATURES = {
     "gs_render_view": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "gs_render_copy": (_I, [_P, _I, _P, _P]),
     "gs_rigid_glue": (_I, [_P, _I, _I, _P, _F, _F, ctypes.POINTER(_I)]),
+    "gs_spawn_block": (_I, [_P, _I, _F, _F, _F, _F, _F, _F, _F, _F, ctypes.POINTER(_I)]),
+    "gs_block_offsets": (_I, [_F, _F, _F, _P, _I, ctypes.POINTER(_I)]),
     "gs_rigid_project": (_I, [_P, _I]),
     "gs_rigid_size": (_I, [_P, _P]),
     "gs_rigid_read": (_I, [_P, _I, _I, _I, _P]),

@@ -33,6 +33,18 @@ class ClusterStatistics:
     nonFiniteBodies: int
 
 
+def block_offsets(contact_diameter: float, width: float, height: float) -> np.ndarray:
+    """RectangularBlock.offsets (K/RectangularBlock.kt:10-77): (n, 2) px, row by row."""
+    lib = native.load()
+    count = ctypes.c_int(0)
+    native.check(lib.gs_block_offsets(float(contact_diameter), float(width), float(height), None, 0,
+                                      ctypes.byref(count)))
+    out = np.zeros((count.value, 2), np.float32)
+    native.check(lib.gs_block_offsets(float(contact_diameter), float(width), float(height), native.ptr(out),
+                                      count.value, ctypes.byref(count)))
+    return out
+
+
 def projectile_offsets(contact_diameter: float, projectile_diameter: float) -> np.ndarray:
     """CircularProjectile.offsets (K/CircularProjectile.kt:10-37): the packing's
     centre offsets in px, shape (count, 2), from the library (host only)."""
@@ -265,6 +277,20 @@ class GravityPipeline:
         native.check(self._lib.gs_rigid_glue(self._h, int(firstBody), len(xy), native.ptr(xy), float(centerX),
                                              float(centerY), ctypes.byref(group)))
         return group.value
+
+    def spawnBlock(self, firstBody: int, minX: float, minY: float, maxX: float, maxY: float,
+                   density: float | None = None, velocityX: float = 0.0, velocityY: float = 0.0,
+                   spin: float = 0.0) -> int:
+        """spawnBlock without a mask (K/GravityPipeline.kt:647-717): the
+        rectangle filled with a hexagonal packing glued into one block;
+        returns the bodies written (0 when they do not fit)."""
+        if density is None:
+            density = self.config.projectiles.blockDensity
+        spawned = ctypes.c_int(0)
+        native.check(self._lib.gs_spawn_block(self._h, int(firstBody), float(minX), float(minY), float(maxX),
+                                              float(maxY), float(density), float(velocityX), float(velocityY),
+                                              float(spin), ctypes.byref(spawned)))
+        return spawned.value
 
     def projectBlocks(self, contacts: bool) -> None:
         """RigidBlocks.project (K/RigidBlocks.kt:309-357) for blocks nothing breaks."""

@@ -257,6 +257,20 @@ int gs_render_copy(gs_sim *sim, int count, void *bodies, void *pressures);
  * :391-399). */
 int gs_rigid_glue(gs_sim *sim, int first_body, int count, const float *offsets_xy, float center_x,
                   float center_y, int *group);
+/* spawnBlock(firstBody, minX, minY, maxX, maxY, density, velocityX,
+ * velocityY, spin) without a mask (K/GravityPipeline.kt:647-717): the
+ * rectangle filled with a hexagonal packing (RectangularBlock), uploaded as
+ * uploadBodies does (mass = density, the block friction, solid, the block's
+ * index; velocity v + spin x offset) and glued (gs_rigid_glue). *spawned
+ * receives the bodies written: 0 when none fit the rectangle, the capacity
+ * or the block slots. */
+int gs_spawn_block(gs_sim *sim, int first_body, float min_x, float min_y, float max_x, float max_y,
+                   float density, float velocity_x, float velocity_y, float spin, int *spawned);
+/* RectangularBlock.offsets / bodyCount (K/RectangularBlock.kt:10-77), host
+ * only: up to `capacity` offsets (2 floats each, px, row by row) into xy
+ * (nullable), the full count into *count. */
+int gs_block_offsets(float contact_diameter_px, float width_px, float height_px, float *xy, int capacity,
+                     int *count);
 /* RigidBlocks.project(bodies, materials, bounded, contacts)
  * (K/RigidBlocks.kt:309-357; S/rigid_reduce.comp, S/rigid_apply.comp) for
  * blocks nothing breaks: every block fitted to its members (momentum and

@@ -158,3 +158,45 @@ def test_a_pipeline_with_blocks_refuses_to_step():
             pipe.blockWrite(native.GS_RIGID_MEMBERS, 0, np.zeros(4, np.int32))
         with pytest.raises(ValueError):
             pipe.blockWrite(native.GS_RIGID_GROUPS, 10, np.zeros(4, np.int32))
+
+
+@pytest.mark.parametrize("bounded", [False, True])
+def test_spawned_block_is_the_restated_upload_and_glue(bounded):
+    from gravity_amd.pipeline import block_offsets
+    f32 = np.float32
+    loose = 500
+    bodies, mats = scenarios.uniform(loose, W, H, seed=9)
+    cfg = GravityConfig(particles=ParticleConfig(capacity=6000, initialBodies=loose),
+                        gravity=GravityFieldConfig(gridWidth=256, gridHeight=128),
+                        collisions=CollisionConfig(contactDiameterPixels=DIAMETER))
+    minX, minY, maxX, maxY = 1880.25, 100.5, 1950.0, 160.0   # across the seam on the torus, clamped in walls
+    density, vx, vy, spin = 0.75, 30.0, -12.0, 0.4
+    with GravityPipeline(cfg, W, H, scenarioState=bodies, bounded=bounded) as pipe:
+        pipe.upload(bodies, mats)
+        k = pipe.spawnBlock(loose, minX, minY, maxX, maxY, density, vx, vy, spin)
+        got = pipe.bodies(loose + k)[loose:]
+        got_mats = pipe.materials(loose + k)[loose:]
+        groups = pipe.blockRead(native.GS_RIGID_GROUPS, 0, 40, np.uint32)
+        assert pipe.spawnBlock(6000 - 10, 0.0, 0.0, 50.0, 50.0) == 0      # does not fit
+        assert pipe.spawnBlock(loose + k, 10.0, 10.0, 10.4, 10.4) == 0    # narrower than a body
+    lattice = block_offsets(DIAMETER, f32(maxX) - f32(minX), f32(maxY) - f32(minY))
+    assert k == len(lattice) > 0
+    mean = (lattice.astype(np.float64).sum(axis=0).astype(f32) / f32(k)).astype(f32)
+    offsets = (lattice - mean).astype(f32)
+    cx = f32(f32(0.5) * f32(f32(minX) + f32(maxX))) + mean[0]
+    cy = f32(f32(0.5) * f32(f32(minY) + f32(maxY))) + mean[1]
+    px = (cx + offsets[:, 0]).astype(f32)
+    py = (cy + offsets[:, 1]).astype(f32)
+    if bounded:
+        px = np.minimum(np.maximum(px, f32(0)), f32(W) * f32(0.99999994))
+        py = np.minimum(np.maximum(py, f32(0)), f32(H) * f32(0.99999994))
+    else:
+        px = (px - np.floor(px / f32(W)) * f32(W)).astype(f32)
+        py = (py - np.floor(py / f32(H)) * f32(H)).astype(f32)
+    want = np.stack([px / f32(W), py / f32(H), (f32(vx) - f32(spin) * offsets[:, 1]) / f32(W),
+                     (f32(vy) + f32(spin) * offsets[:, 0]) / f32(H)], 1).astype(f32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.all(got_mats == np.array([density, 0.5, 1.0, 0.0], f32))
+    lay = Layout(6000)
+    lay.glue(loose, offsets, cx, cy)
+    assert np.array_equal(groups, lay.groups[:40])

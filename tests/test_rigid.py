@@ -275,3 +275,49 @@ def test_contact_memory_settles_pair_by_pair():
     imp[key(0, 1)], st[key(0, 1)] = 5, 2
     lay.project(bodies, W, H, True, False, SUB, RADIUS)
     assert (imp[key(0, 1)], st[key(0, 1)]) == (5, 2)
+
+
+# ---- RectangularBlock (the reference's T/RectangularBlockTest.kt cases) ----
+def _block_offsets(width, height, d=np.float32(1.12838)):
+    from gravity_amd.pipeline import block_offsets
+    return block_offsets(d, width, height)
+
+
+def test_block_bodies_lie_inside_the_rectangle_without_overlapping():
+    d = np.float32(1.12838)
+    w, h = 40.0, 25.0
+    xy = _block_offsets(w, h).astype(np.float64)
+    r = 0.5 * float(d)
+    assert len(xy) > 0
+    assert np.all(xy[:, 0] - r >= -0.5 * w - 1e-3) and np.all(xy[:, 0] + r <= 0.5 * w + 1e-3)
+    assert np.all(xy[:, 1] - r >= -0.5 * h - 1e-3) and np.all(xy[:, 1] + r <= 0.5 * h + 1e-3)
+    diff = xy[:, None, :] - xy[None, :, :]
+    dist = np.sqrt((diff ** 2).sum(-1)) + np.eye(len(xy)) * 1e9
+    assert dist.min() >= float(d) - 1e-3
+
+
+def test_block_is_hexagonally_dense():
+    d = 1.12838
+    count = len(_block_offsets(100.0, 60.0))
+    hexagonal = 100.0 * 60.0 / (d * d * 0.8660254)
+    assert 0.9 * hexagonal < count <= hexagonal
+
+
+def test_rectangle_narrower_than_a_body_holds_nothing():
+    d = np.float32(1.12838)
+    assert len(_block_offsets(0.5, 100.0)) == 0
+    assert len(_block_offsets(100.0, 0.5)) == 0
+    assert len(_block_offsets(d, d)) == 1
+
+
+def test_block_rows_are_the_hexagonal_lattice():
+    d = np.float32(1.12838)
+    xy = _block_offsets(10.0, 6.0)
+    ys = np.unique(xy[:, 1])
+    step = np.float32(d * np.float32(np.sqrt(3.0) * 0.5))
+    assert np.allclose(np.diff(ys), step, atol=1e-5)
+    for k, y in enumerate(ys):
+        row = np.sort(xy[xy[:, 1] == y, 0])
+        assert np.allclose(np.diff(row), d, atol=1e-5)
+        # odd rows start half a diameter in
+        assert row[0] == pytest.approx(-0.5 * (10.0 - d) + (0.5 * d if k % 2 else 0.0), abs=1e-5)
