@@ -312,6 +312,9 @@ std::vector<int> hull_indices(const std::vector<float>& xs, const std::vector<fl
     if (xs.empty()) return chain;
     const float minY = *std::min_element(ys.begin(), ys.end());
     const float maxY = *std::max_element(ys.begin(), ys.end());
+    // a row table over the block's height (the reference's IntArray per row)
+    if (!((maxY - minY) / HULL_ROW_PIXELS < (float)(1 << 24)))
+        throw ArgumentError("a block's members must lie within 167,772 px of one another");
     const int rows = (int)((maxY - minY) / HULL_ROW_PIXELS) + 1;
     std::vector<int> lowest(rows, -1), highest(rows, -1);
     for (int i = 0; i < (int)xs.size(); i++) {
