@@ -284,13 +284,15 @@ __global__ void __launch_bounds__(RIGID_TILE) k_rigid_adopt(int* __restrict__ me
     for (int plane = 0; plane < 5; plane++) memory[plane * RIGID_CONTACT_KEYS + key] = 0;
 }
 
-// S/rigid_remap.comp: members follow the bodies spatial sorting renamed.
+// S/rigid_remap.comp: members follow the bodies spatial sorting renamed. The
+// sort moves the `active` bodies only: a member beyond them keeps its index.
 __global__ void __launch_bounds__(RIGID_TILE) k_rigid_remap(int* __restrict__ members,
-                                                           const uint32_t* __restrict__ dest, int count) {
+                                                           const uint32_t* __restrict__ dest, int count,
+                                                           int active) {
     const int member = blockIdx.x * RIGID_TILE + (int)threadIdx.x;
     if (member >= count) return;
     const int b = members[member];
-    if (b >= 0) members[member] = (int)dest[b];
+    if (b >= 0 && b < active) members[member] = (int)dest[b];
 }
 
 // Kotlin's total order on floats (Float.compareTo: -0 below +0) and its
@@ -586,7 +588,7 @@ void Engine::rigid_remap() {
     if (rigid_.groupCount == 0) return;
     const int count = rigid_.tileCount * RIGID_TILE;
     launch(k_rigid_remap, dim3(rigid_.tileCount), dim3(RIGID_TILE), 0, st_, rigid_.members,
-           (const uint32_t*)dest_, count);
+           (const uint32_t*)dest_, count, sims_[0].n);
 }
 
 void Engine::rigid_size(int* out) const {

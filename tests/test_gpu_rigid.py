@@ -147,6 +147,17 @@ def test_members_follow_their_bodies_through_a_sort():
     assert np.array_equal(projected[after[listed]].view(np.uint32), state[before[listed]].view(np.uint32))
 
 
+def test_members_beyond_the_sorted_bodies_keep_their_indices():
+    pipe, lay, state = _scene(False)
+    with pipe:
+        before = pipe.blockRead(native.GS_RIGID_MEMBERS, 0, lay.member_cap, np.int32)
+        pipe.sortBodies(1000)          # the loose bodies only; every member lies beyond them
+        after = pipe.blockRead(native.GS_RIGID_MEMBERS, 0, lay.member_cap, np.int32)
+        bodies = pipe.bodies(len(state))
+    assert np.array_equal(after, before)
+    assert np.array_equal(bodies[1000:].view(np.uint32), state[1000:].view(np.uint32))
+
+
 def test_a_pipeline_with_blocks_refuses_to_step():
     pipe, lay, state = _scene(True, sizes=(300,), loose=500)
     with pipe:
