@@ -321,3 +321,26 @@ def test_block_rows_are_the_hexagonal_lattice():
         assert np.allclose(np.diff(row), d, atol=1e-5)
         # odd rows start half a diameter in
         assert row[0] == pytest.approx(-0.5 * (10.0 - d) + (0.5 * d if k % 2 else 0.0), abs=1e-5)
+
+
+@pytest.mark.parametrize("w,h", [(40.0, 25.0), (10.0, 6.0), (1.7, 3.1), (200.0, 55.0), (1.12838, 1.12838)])
+def test_block_offsets_are_the_restated_lattice(w, h):
+    """RectangularBlock.Lattice and forEachOffset (K/RectangularBlock.kt:22-77)
+    restated in float32 scalars, offset for offset."""
+    f = np.float32
+    d = f(1.12838)
+    eps = f(1e-4)
+    half_x, half_y = f(f(0.5) * f(f(w) - d)), f(f(0.5) * f(f(h) - d))
+    row_step = f(d * f(np.sqrt(3.0) * 0.5))
+    rows = int(np.floor(f(f(f(2) * half_y) / row_step) + eps)) + 1
+    even = int(np.floor(f(f(f(2) * half_x) / d) + eps)) + 1
+    odd_span = f(f(f(2) * half_x) - f(f(0.5) * d))
+    odd = 0 if odd_span < -eps * d else int(np.floor(f(odd_span / d) + eps)) + 1
+    want = []
+    for row in range(rows):
+        columns = odd if row & 1 else even
+        start = f(-half_x + (f(f(0.5) * d) if row & 1 else f(0)))
+        y = f(-half_y + f(f(row) * row_step))
+        want += [(f(start + f(f(c) * d)), y) for c in range(columns)]
+    got = _block_offsets(w, h)
+    assert np.array_equal(got, np.asarray(want, np.float32).reshape(-1, 2))
